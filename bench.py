@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""Benchmark of the render chain on one MI355X.
+
+Workload (the benchmark's configuration 2): one 6000 x 4000 float32 linear ProPhoto frame of the
+synthetic generator (seed 0) on the synthetic 3-layer stock; Philox Poisson grain (film format
+35 mm); halation = one mixture of 3 Gaussians sigma 2/8/32 px, weights 0.6/0.3/0.1, scatter off;
+FIR for sigma <= 16 (crossover 16.5), recursive above; print, scan, CAM16 gamut, unsharp, RGBA16.
+A step is one spk_render of the frame (develop + print + encode) with the input already resident
+in HBM and the output written to HBM (spkx_render_device); K steps are timed between two device
+synchronisations.
+
+The JSON line reports Mpx/s, the live HIP-event duration and algorithmic-byte roofline of the
+dominant kernel, and a CPU baseline: the oracle (oracle/chain.cpp, binary32) timed on a bounded
+sample (a 1536 x 1024 crop of the same workload) on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per pixel of each stage: a pointwise run reads its input planes once and
+# writes its output once; each separable pass of a Gaussian reads one 3-channel plane and writes
+# one, plus one read for an accumulating pass (12 B per 3-channel float plane, 8 B per RGBA16 px).
+STAGE_BYTES = {"upsample": 24, "develop": 36, "grain_blur": 48, "print": 24, "unsharp_blur": 48, "encode": 32}
+STAGE_KERNEL = {"upsample": "k_upsample", "develop": "k_develop", "print": "k_print", "encode": "k_unsharp_encode",
+                "halation": "k_fir_v/k_fir_h/k_iir_row/k_iir_col", "grain_blur": "k_fir_v/k_fir_h",
+                "unsharp_blur": "k_fir_v/k_fir_h"}
+METRIC = "Mpx/s full chain @24MP (frac of HBM peak); batched frames/s; fwd+bwd Mpx/s"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def halation_bytes(n_comp):
+    return 48 + 60 * (n_comp - 1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=6000)
+    ap.add_argument("--height", type=int, default=4000)
+    ap.add_argument("--policy", choices=["cfg", "ref"], default="cfg",
+                    help="cfg: FIR for sigma <= 16; ref: the reference's FIR for sigma < 3")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-sample", type=str, default="1536x1024")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the render chain runs on a single "
+              "MI355X and multi-GPU scaling is not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import spektralab_engine_amd as spk
+    from spektralab_engine_amd import synth
+
+    crossover = 16.5 if args.policy == "cfg" else 3.0
+    delta = dict(film_stock=synth.FILM_STOCK, print_stock=synth.PAPER_STOCK, auto_exposure=False,
+                 dir_couplers_active=False, glare_active=False, grain_active=True, halation_active=True,
+                 film_format_mm=35.0)
+    overrides = {"halation_mixture": [[2.0, 0.6], [8.0, 0.3], [32.0, 0.1]], "halation_scatter": False,
+                 "fir_crossover": crossover}
+    res = synth.default_resources_dir(192)
+    W, H = args.width, args.height
+    log(f"[bench] generating the {W}x{H} frame")
+    frame = synth.make_frame(W, H, 0)
+    eng = spk.Engine(res)
+    sess = eng.open(frame, delta)
+    sess.set_overrides(overrides)
+    out = torch.empty((H, W, 4), dtype=torch.int16, device="cuda")
+    log("[bench] warm-up")
+    for _ in range(args.warmup):
+        sess.render_device(out.data_ptr(), W)
+    torch.cuda.synchronize()
+    sess.set_profiling(True)
+    log(f"[bench] timing {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sess.render_device(out.data_ptr(), W)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stages = sess.stage_times(reset=True)
+    sess.set_profiling(False)
+    ms = (t1 - t0) * 1000.0 / args.steps
+    px = W * H
+    mpxs = px / (ms / 1000.0) / 1e6
+    b_alg = sum(STAGE_BYTES.values()) + halation_bytes(3)
+    chain_gbs = b_alg * px / (ms / 1000.0) / 1e9
+    per_stage = {k: v["ms"] / v["count"] for k, v in stages.items()}
+    dom = max((k for k in per_stage if k in STAGE_BYTES), key=lambda k: per_stage[k])
+    dom_ms = per_stage[dom]
+    dom_bytes = STAGE_BYTES[dom] * px
+    achieved = dom_bytes / (dom_ms / 1000.0) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / f"traffic_{STAGE_KERNEL[dom]}_24mp.json"
+    if tf.exists() and (W, H) == (6000, 4000):
+        traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+    log(f"[bench] {ms:.3f} ms/frame, {mpxs:.1f} Mpx/s; stages (ms): " +
+        ", ".join(f"{k} {v:.3f}" for k, v in per_stage.items()))
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        cw, ch = (int(v) for v in args.cpu_sample.split("x"))
+        log(f"[bench] CPU baseline: oracle binary32 on a {cw}x{ch} sample, {orc.lib().oracle_threads()} threads")
+        plan = spk.host_plan(res, delta, cw, ch, film_seed=0x4ABEA221, overrides=overrides)
+        sample = np.ascontiguousarray(frame[:ch, :cw])
+        orc.render(plan, sample)  # warm-up pass (page-in, thread pool)
+        c0 = time.perf_counter()
+        reps = 0
+        while True:
+            orc.render(plan, sample)
+            reps += 1
+            if time.perf_counter() - c0 > 10.0 or reps >= 5:
+                break
+        cms = (time.perf_counter() - c0) * 1000.0 / reps
+        cpu = {"value": round(cw * ch / (cms / 1000.0) / 1e6, 3), "unit": "Mpx/s", "cores": orc.lib().oracle_threads(),
+               "kind": "port", "sample": f"{cw}x{ch} crop of the same frame and stock, same chain (binary32 oracle), "
+                                         f"{reps} renders, {cms:.0f} ms each"}
+
+    line = {
+        "metric": METRIC,
+        "value": round(mpxs, 2),
+        "unit": "Mpx/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"config 2: plain chain {W}x{H}, Poisson grain + 3-Gaussian halation (2/8/32 px) "
+                               f"+ print/scan + CAM16 + unsharp -> RGBA16, policy {args.policy}",
+                   "width": W, "height": H, "fir_crossover": crossover, "film_format_mm": 35.0,
+                   "stock": "synthetic 3-layer negative / synthetic paper"},
+        "chain_hbm_frac": round(chain_gbs / HBM_PEAK_GBS, 4),
+        "chain_algorithmic_bytes_per_px": b_alg,
+        "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
+        "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "launch_ms": round(dom_ms, 4), "bytes_per_px": STAGE_BYTES[dom]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    sess.close()
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

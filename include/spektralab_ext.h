@@ -1,0 +1,69 @@
+/* spektralab_ext.h -- entry points this engine adds to the reference ABI.
+ *
+ * None of these has a counterpart in the reference's include/spektralab.h; every one is an
+ * addition with the same conventions (status codes, per-thread last error, borrowed inputs).
+ *   - device-resident rendering for hosts and benchmarks that keep frames in HBM
+ *   - the render plan and cached negative, so a CPU checker can replay a render
+ *   - build-defined overrides of the benchmark workloads (explicit halation mixture, FIR/IIR
+ *     crossover)
+ *   - self-tests of the Philox generator, the Poisson sampler and the Gaussian machinery
+ */
+#ifndef SPEKTRALAB_EXT_H
+#define SPEKTRALAB_EXT_H
+
+#include "spektralab.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Overrides for later renders of the session (drops its cached negatives). JSON object, keys
+ * optional:
+ *   "halation_mixture": [[sigma_px, weight], ...]   1..8 components replacing the bounce mixture
+ *                                                    (same sigma and weight on every channel)
+ *   "halation_scatter": bool                         false: no scatter mixture
+ *   "fir_crossover": number > 0                      FIR for 0 < sigma < crossover (3 = reference)
+ * Returns SPK_OK | SPK_ERR_INVALID_ARG | SPK_ERR_USER. */
+SPK_API spk_status spkx_set_overrides(spk_session* session, const char* json);
+
+/* spk_render / spk_reprint writing RGBA16 into a caller device buffer (row pitch row_stride_px
+ * pixels, >= width), enqueued on `stream` (a hipStream_t; NULL = the session's own stream).
+ * The call returns after enqueueing; *out_w, *out_h receive the size.  No host copy is made. */
+SPK_API spk_status spkx_render_device(spk_session* session, const char* tier, int32_t reprint, void* dev_out,
+                                      uint32_t row_stride_px, void* stream, uint32_t* out_w, uint32_t* out_h);
+/* Waits for the session's stream. */
+SPK_API spk_status spkx_session_sync(spk_session* session);
+
+/* Stage timing with HIP events on the render stream (off by default).  spkx_stage_times returns
+ * {"<stage>": {"ms": total, "count": renders}, ...} accumulated since enabling (or the last reset). */
+SPK_API spk_status spkx_set_profiling(spk_session* session, int32_t on);
+SPK_API spk_status spkx_stage_times(spk_session* session, int32_t reset, char** out_json);
+
+/* The plan of the session's most recent render (struct spk_plan of spektralab_plan.h).
+ * Returns sizeof(spk_plan); copies it when cap >= that size.  0 when nothing was rendered. */
+SPK_API size_t spkx_last_plan(spk_session* session, void* out, size_t cap);
+/* The cached negative of `tier` as 3 planar float32 planes (w*h*3 floats) into host memory. */
+SPK_API spk_status spkx_negative(spk_session* session, const char* tier, float* out, size_t count);
+
+/* Host-only: the plan a session opened on `resources_dir` with `params_delta_json` (and the
+ * overrides of spkx_set_overrides, NULL = none) would use for a w x h full-tier render whose film
+ * seed is `film_seed`.  Needs no GPU.  Returns sizeof(spk_plan) (copied when cap allows), or 0 with
+ * the reason in spk_last_error. */
+SPK_API size_t spkx_plan(const char* resources_dir, const char* params_delta_json, const char* overrides_json,
+                         uint32_t width, uint32_t height, uint32_t film_seed, void* out, size_t cap);
+
+/* Self-tests on the engine's GPU:
+ *   out[4i..4i+3] = Philox4x32-10(ctr[4i..4i+3], key[2i..2i+1])
+ *   out[i] = Poisson(lam[i]) of pixel i on (stream, seed), the grain sampler itself
+ *   gaussian: 3-channel interleaved host image -> per-channel Gaussian (the render's machinery) */
+SPK_API spk_status spkx_philox(spk_engine* engine, const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n);
+SPK_API spk_status spkx_poisson(spk_engine* engine, const float* lam, uint32_t n, uint32_t stream, uint32_t seed,
+                                float* out);
+SPK_API spk_status spkx_gaussian(spk_engine* engine, const float* in, float* out, uint32_t width, uint32_t height,
+                                 const double* sigma3, double fir_crossover);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPEKTRALAB_EXT_H */

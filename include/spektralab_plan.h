@@ -1,0 +1,126 @@
+/* spektralab_plan.h -- the render plan: every constant and table one render of the chain reads.
+ *
+ * The engine's float64 set-up (film set-up, print set-up, per-run constants, CAM16 tables) is
+ * narrowed once into this POD struct.  The HIP kernels are fed from its float32 members; the
+ * float64 members carry the same quantities before narrowing, for the float64 CPU oracle and the
+ * precision study.  spkx_session_plan() (spektralab_ext.h) copies a session's plan out so a
+ * checker can run the same chain on the CPU.
+ *
+ * Reference counterparts of the members:
+ *   film side      src/film/film_internal.hpp:81-100 (FilmSetup::Impl), kernels/film_shared.h:52-90
+ *   halation       src/fx/fx_internal.hpp:47-55 (HalationPlan), src/fx/halation.cpp:104-195
+ *   grain          src/fx/fx_internal.hpp:61-71 (GrainConstants), kernels/fx_shared.h:73-88
+ *   print side     src/print/internal.hpp:172-182 (PrintRun), kernels/print_shared.h:29-112
+ *   scanner        src/fx/scanner.hpp:14-18
+ * Layout: little-endian LP64, natural alignment; SPK_PLAN_VERSION changes with any layout change.
+ */
+#ifndef SPEKTRALAB_PLAN_H
+#define SPEKTRALAB_PLAN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPK_PLAN_MAGIC 0x504B5053u /* "SPKP" */
+#define SPK_PLAN_VERSION 1u
+
+#define SPK_NL 81        /* wavelength samples 380..780 nm, 5 nm */
+#define SPK_NL3 243      /* [lambda][channel] */
+#define SPK_PLAN_MAX_L 256
+#define SPK_PLAN_MAX_K 1024
+#define SPK_PLAN_MAX_MIX 8
+#define SPK_CMAX_ROWS 64
+#define SPK_CMAX_COLS 720
+#define SPK_CMAX_N (SPK_CMAX_ROWS * SPK_CMAX_COLS)
+
+/* Grain draw modes (kernels/fx_shared.h:70-72). */
+#define SPK_GRAIN_SUM 0u
+#define SPK_GRAIN_PLANES 1u
+#define SPK_GRAIN_SIMPLE 2u
+
+/* One term of a Gaussian mixture M(x) = sum_k w_k G_{sigma_k}(x), per channel, in pixels. */
+typedef struct spk_mix_component {
+    double weight[3];
+    double sigma[3];
+} spk_mix_component;
+
+typedef struct spk_plan {
+    uint32_t magic, version;
+    uint32_t width, height;
+    double pitch_um;       /* pixel pitch of this tier, micrometres */
+    double rho_tier;       /* tier long edge / full long edge */
+    double fir_crossover;  /* per channel: 0 < sigma < fir_crossover -> FIR, else recursive (3.0: reference) */
+
+    /* ---- film side: upsample, exposure compensation, halation, log, curves, grain ---- */
+    uint32_t L;            /* spectral table size */
+    uint32_t K;            /* film curve points */
+    float a[9];            /* input matrix A (RGB -> XYZ adapted), row-major */
+    double a64[9];
+    float ec_gain;         /* f32(2^exposure_compensation_ev); multiply skipped when 1 */
+    uint32_t pad0;
+
+    uint32_t scatter_active, n_scatter;
+    spk_mix_component scatter[4];
+    double scatter_a[3], scatter_b[3];     /* raw <- a raw + b M(raw) */
+    uint32_t bounce_active, n_bounce;
+    spk_mix_component bounce[SPK_PLAN_MAX_MIX];
+    double mix_a[3], mix_b[3];             /* raw <- a raw + b H(raw) */
+
+    uint32_t grain_active;  /* draws happen */
+    uint32_t grain_mode;    /* SPK_GRAIN_* */
+    uint32_t grain_positive, grain_shift, grain_seed, grain_k, grain_n_sub;
+    uint32_t grain_dye;     /* dye-cloud planes (not produced by the configured stocks) */
+    float grain_inv_n_sub;
+    float dmin_l[9], sat_d[9], count[9], uniformity[3], dmin_c[3];  /* [l*3 + c] */
+    double grain_final_blur;  /* sigma px, 0 = none */
+    double grain_amount;
+
+    /* ---- print side: enlarger, gain, paper, scan, levels, to_rgb, gamut, unsharp, encode ---- */
+    uint32_t printed;       /* !scan_film */
+    uint32_t paper_k;
+    float enl_eps[SPK_NL3], enl_beta[SPK_NL], enl_wt[SPK_NL3], enl_g[3], enl_o[3];
+    double enl_eps64[SPK_NL3], enl_beta64[SPK_NL], enl_wt64[SPK_NL3], enl_g64[3], enl_o64[3];
+    float k_gain;
+    uint32_t levels_active;
+    double k_gain64;
+    float levels_m, levels_q;
+    float scan_eps[SPK_NL3], scan_beta[SPK_NL], scan_wt[SPK_NL3];
+    double scan_eps64[SPK_NL3], scan_beta64[SPK_NL], scan_wt64[SPK_NL3];
+    float to_rgb[9];
+    double to_rgb64[9];
+
+    /* CAM16-UCS gamut compression (kernels/print_shared.h:69-95) */
+    uint32_t gamut_active;
+    uint32_t lc_active;
+    float m_to_xyz[9], m_to_rgb[9], m16[9], m16_inv[9], d_rgb[3];
+    float f_l, n_bb, n_cb, n_, z, a_w, c, n_c, j_first, j_last, h_first, h_step, knee[3], lc[3], j_white;
+    double cam64[32]; /* float64 of the scalars above, same order: f_l n_bb n_cb n z a_w c n_c j_first j_last
+                         h_first h_step knee0..2 lc0..2 j_white d_rgb0..2 */
+    double m_to_xyz64[9], m_to_rgb64[9], m16_64[9], m16_inv64[9];
+
+    double scanner_lens_blur_px;   /* plain pixels */
+    double unsharp_sigma;          /* 0.7 rho */
+    double unsharp_amount;         /* 0.7 */
+    uint32_t cctf_active, cctf_mode;
+    float m_nid[9];
+    double m_nid64[9];
+
+    /* ---- tables ---- */
+    float table[SPK_PLAN_MAX_L * SPK_PLAN_MAX_L * 3];      /* L x L x 3 [i][j][c] */
+    double table64[SPK_PLAN_MAX_L * SPK_PLAN_MAX_L * 3];
+    float curves[8 * SPK_PLAN_MAX_K];                      /* [x K][inv K][y 3K][y0 3K], y channel-major */
+    double curves_x64[SPK_PLAN_MAX_K], curves_y64[3 * SPK_PLAN_MAX_K];
+    float grain_axis[3 * SPK_PLAN_MAX_K], grain_inv[3 * SPK_PLAN_MAX_K], grain_layers[9 * SPK_PLAN_MAX_K];
+    float paper[5 * SPK_PLAN_MAX_K];                       /* x[K], r[K], y[K][3] */
+    double paper_x64[SPK_PLAN_MAX_K], paper_y64[3 * SPK_PLAN_MAX_K];
+    float cmax[SPK_CMAX_N];
+    double cmax64[SPK_CMAX_N];
+} spk_plan;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPEKTRALAB_PLAN_H */

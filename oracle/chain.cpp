@@ -1,0 +1,879 @@
+// ORACLE -- test infrastructure only.  A CPU restatement of the reference render chain's
+// per-pixel and per-line arithmetic, used by tests/, __graft_entry__.smoke() and bench.py's CPU
+// baseline as the checker of the HIP product path.  Nothing in the product links or calls it.
+//
+// It consumes the render plan (include/spektralab_plan.h) the engine's float64 set-up produced,
+// and restates, templated on the arithmetic type T (float = the reference's binary32 kernels,
+// double = the float64 accuracy reference):
+//   upsample             kernels/film.metal:58-82 (Mitchell axis), :189-227
+//   exposure comp.       kernels/film.metal:176-180, src/film/develop.cpp:154-158
+//   FIR taps, radius     src/util/numeric.cpp:92-112 (truncate 3), src/fx/gaussian.cpp:19-24
+//   Young-van Vliet      src/fx/gaussian.cpp:34-53 (coefficients), :354-378 (line recursion with
+//                        float64 state, replicated-edge starts)
+//   blur branch rules    src/fx/gaussian.cpp:201-245 (identity / FIR v-then-h / IIR h-then-v, the
+//                        accumulation fused into the last pass), mixtures :254-273
+//   halation             src/fx/halation.cpp:307-329 (scatter lincomb, renormalised bounce mix)
+//   log, curves          kernels/film.metal:88-101, :242-261
+//   grain                kernels/fx_grain.metal:39-97 (SUM mode + shift), kernels/fx_random.h:19-99
+//                        (lgamma, Poisson PTRS / multiplication, sub-layer draw),
+//                        kernels/spk_common.h:23-67 (Philox4x32-10, word stream, uniform map),
+//                        src/fx/grain.cpp:245-266 (final 0.65 px blur, strength mix)
+//   print                kernels/print.metal:165-198 (spectral, gain), :201-232 (paper),
+//                        :249-258 (matrix), :31-127 + :267-317 (CAM16-UCS gamut), :136-156
+//                        (transfer, 16-bit), :350-366 (encode); src/fx/scanner.cpp:16-33 (unsharp)
+// Arithmetic conventions (binary32 mode): every float operation rounds as written (compiled with
+// -ffp-contract=off); FIR taps accumulate with fused multiply-add in ascending order (the HIP
+// kernels do the same); film-side log10 and the sampler's log / exp use the shared reproducible
+// routines of spk_math.h; print-side transcendentals are the C library's.
+#include <omp.h>
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../include/spektralab_plan.h"
+#include "../spektralab_engine_amd/csrc/kernels/spk_math.h"
+
+namespace {
+
+// ---- type-dependent elementary functions -------------------------------------------------------
+inline float o_log10(float x) { return spkm::log10f_(x); }
+inline double o_log10(double x) { return std::log10(x); }
+inline float p_log10(float x) { return log10f(x); }
+inline double p_log10(double x) { return std::log10(x); }
+inline float p_exp2(float x) { return exp2f(x); }
+inline double p_exp2(double x) { return std::exp2(x); }
+
+template <class T>
+struct K;  // per-type constants
+template <>
+struct K<float> {
+    static constexpr float log2_10 = 3.321928094887362f;
+    static constexpr float tiny = 1e-10f;
+};
+template <>
+struct K<double> {
+    static constexpr double log2_10 = 3.3219280948873623478703194294894;
+    static constexpr double tiny = 1e-10;
+};
+
+// ---- numeric helpers ----------------------------------------------------------------------------
+int reflect_index(int64_t i, int64_t n) {
+    if (n <= 1) return 0;
+    const int64_t period = 2 * n;
+    int64_t m = i % period;
+    if (m < 0) m += period;
+    return int(m < n ? m : period - 1 - m);
+}
+
+int fir_radius(double sigma) { return sigma > 0.0 ? int(std::floor(3.0 * sigma + 0.5)) : 0; }
+
+std::vector<double> fir_taps64(double sigma) {
+    const int r = fir_radius(sigma);
+    if (r == 0) return {1.0};
+    std::vector<double> w(size_t(2 * r + 1));
+    double sum = 0.0;
+    for (int k = -r; k <= r; ++k) {
+        const double t = double(k) / sigma;
+        const double e = std::exp(-0.5 * t * t);
+        w[size_t(k + r)] = e;
+        sum += e;
+    }
+    for (double& e : w) e /= sum;
+    return w;
+}
+
+struct Yvv {
+    double B, B1, B2, B3;
+};
+Yvv yvv(double sigma) {
+    const double q = sigma >= 2.5 ? 0.98711 * sigma - 0.96330 : 3.97156 - 4.14554 * std::sqrt(1.0 - 0.26891 * sigma);
+    const double q2 = q * q, q3 = q2 * q;
+    const double b0 = ((1.57825 + 2.44413 * q) + 1.4281 * q2) + 0.422205 * q3;
+    const double b1 = (2.44413 * q + 2.85619 * q2) + 1.26661 * q3;
+    const double b2 = -(1.4281 * q2 + 1.26661 * q3);
+    const double b3 = 0.422205 * q3;
+    return {1.0 - ((b1 + b2) + b3) / b0, b1 / b0, b2 / b0, b3 / b0};
+}
+
+template <class T>
+T fma_t(T a, T b, T c) {
+    return std::fma(a, b, c);
+}
+
+// ---- the Gaussian machinery on one channel plane (w x h, row-major) ----------------------------
+// mode 0: out = v; 1: out = 0 + wt v; 2: out = acc + wt v.
+template <class T>
+T fuse(int mode, T acc, T wt, T v) {
+    if (mode == 0) return v;
+    if (mode == 1) return T(0) + wt * v;
+    return acc + wt * v;
+}
+
+template <class T>
+void fir_plane(const T* in, T* out, const T* acc, int mode, T wt, double sigma, int w, int h) {
+    const std::vector<double> t64 = fir_taps64(sigma);
+    std::vector<T> taps(t64.size());
+    for (size_t i = 0; i < t64.size(); ++i) taps[i] = T(t64[i]);
+    const int r = int(taps.size() / 2);
+    std::vector<T> tmp(size_t(w) * h);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            T a = T(0);
+            for (int i = -r; i <= r; ++i) a = fma_t(taps[size_t(i + r)], in[size_t(reflect_index(y + i, h)) * w + x], a);
+            tmp[size_t(y) * w + x] = a;
+        }
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            T a = T(0);
+            for (int i = -r; i <= r; ++i) a = fma_t(taps[size_t(i + r)], tmp[size_t(y) * w + reflect_index(x + i, w)], a);
+            const size_t o = size_t(y) * w + x;
+            out[o] = fuse(mode, acc ? acc[o] : T(0), wt, a);
+        }
+}
+
+// One line: forward w[-1..-3] = x[0], backward y[n..n+2] = u[n-1]; state float64, storage T.
+template <class T>
+void yvv_line(const T* in, T* out, size_t n, size_t stride, const Yvv& k) {
+    std::vector<T> u(n);
+    double w1 = double(in[0]), w2 = w1, w3 = w1;
+    for (size_t i = 0; i < n; ++i) {
+        const double t = ((k.B * double(in[i * stride]) + k.B1 * w1) + k.B2 * w2) + k.B3 * w3;
+        w3 = w2;
+        w2 = w1;
+        w1 = t;
+        u[i] = T(t);
+    }
+    double y1 = double(u[n - 1]), y2 = y1, y3 = y1;
+    for (size_t i = n; i-- > 0;) {
+        const double t = ((k.B * double(u[i]) + k.B1 * y1) + k.B2 * y2) + k.B3 * y3;
+        y3 = y2;
+        y2 = y1;
+        y1 = t;
+        out[i * stride] = T(t);
+    }
+}
+
+template <class T>
+void iir_plane(const T* in, T* out, const T* acc, int mode, T wt, double sigma, int w, int h) {
+    const Yvv k = yvv(sigma);
+    std::vector<T> tmp(size_t(w) * h);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < h; ++y) yvv_line(in + size_t(y) * w, tmp.data() + size_t(y) * w, size_t(w), 1, k);
+    std::vector<T> col(size_t(w) * h);
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < w; ++x) yvv_line(tmp.data() + x, col.data() + x, size_t(h), size_t(w), k);
+    for (size_t o = 0; o < size_t(w) * h; ++o) out[o] = fuse(mode, acc ? acc[o] : T(0), wt, col[o]);
+}
+
+// Per-channel Gaussian on 3 planar planes with the reference's branch rules.
+template <class T>
+void blur3(const T* in, T* out, const double sigma[3], int mode, const T* acc, const double weight[3], int w, int h,
+           double crossover) {
+    const size_t np = size_t(w) * h;
+    for (int c = 0; c < 3; ++c) {
+        const T* ic = in + c * np;
+        T* oc = out + c * np;
+        const T* ac = acc ? acc + c * np : nullptr;
+        const double s = sigma[c];
+        const T wt = T(weight[c]);
+        if (!(s > 0.0) || (s < crossover && fir_radius(s) == 0)) {
+            std::vector<T> cp(ic, ic + np);
+            for (size_t i = 0; i < np; ++i) oc[i] = fuse(mode, ac ? ac[i] : T(0), wt, cp[i]);
+        } else if (s < crossover) {
+            fir_plane(ic, oc, ac, mode, wt, s, w, h);
+        } else {
+            iir_plane(ic, oc, ac, mode, wt, s, w, h);
+        }
+    }
+}
+
+// ---- film side -----------------------------------------------------------------------------------
+template <class T>
+T mitchell(T u) {
+    const T a = std::fabs(u);
+    if (a < T(1)) return (T(1) / T(6)) * (T(7) * a * a * a - T(12) * a * a + T(16) / T(3));
+    if (a < T(2)) return (T(1) / T(6)) * ((T(-7) / T(3)) * a * a * a + T(12) * a * a - T(20) * a + T(32) / T(3));
+    return T(0);
+}
+
+template <class T>
+void mitchell_axis(T t, uint32_t L, int* idx, T* w) {
+    const T last = T(L - 1);
+    T u = t * last;
+    int base;
+    T frac;
+    if (u <= T(0)) u = T(0);
+    if (u >= last) {
+        base = int(L) - 2;
+        frac = T(1);
+    } else {
+        base = int(std::floor(u));
+        frac = u - T(base);
+    }
+    const int n = int(L);
+    for (int k = 0; k < 4; ++k) {
+        int i = base - 1 + k;
+        if (i < 0) i = -i;
+        if (i >= n) i = 2 * (n - 1) - i;
+        idx[k] = i;
+    }
+    w[0] = mitchell(frac + T(1));
+    w[1] = mitchell(frac);
+    w[2] = mitchell(frac - T(1));
+    w[3] = mitchell(frac - T(2));
+}
+
+template <class T>
+const T* table_of(const spk_plan& p);
+template <>
+const float* table_of<float>(const spk_plan& p) {
+    return p.table;
+}
+template <>
+const double* table_of<double>(const spk_plan& p) {
+    return p.table64;
+}
+
+template <class T>
+T curve_lookup(const T* x, const T* inv, const T* y, uint32_t K, T v) {
+    if (v <= x[0]) return y[0];
+    if (v >= x[K - 1]) return y[K - 1];
+    uint32_t lo = 0, hi = K;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (x[mid] <= v) lo = mid + 1;
+        else hi = mid;
+    }
+    const uint32_t l = lo == 0 ? 0 : lo - 1;
+    const T t = (v - x[l]) * inv[l];
+    return y[l] + t * (y[l + 1] - y[l]);
+}
+
+// Philox4x32-10 (Salmon et al. 2011): counter (pixel, stream, j, 0), key (seed, 0).
+void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c[0];
+        const uint64_t p1 = uint64_t(0xCD9E8D57u) * c[2];
+        const uint32_t n0 = uint32_t(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = uint32_t(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = uint32_t(p1);
+        c[3] = uint32_t(p0);
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+struct Stream {
+    uint32_t pixel, stream, seed, j = 0, used = 4, b[4] = {};
+    float uniform() {
+        if (used == 4) {
+            b[0] = pixel;
+            b[1] = stream;
+            b[2] = j;
+            b[3] = 0;
+            philox(b, seed, 0);
+            ++j;
+            used = 0;
+        }
+        const uint32_t w = b[used++];
+        return (float(w >> 8) + 0.5f) * 0x1p-24f;
+    }
+};
+
+float lgamma_k(float x) {
+    if (x == 1.0f || x == 2.0f) return 0.0f;
+    float x0 = x;
+    int n = 0;
+    if (x <= 7.0f) {
+        n = int(7.0f - x);
+        x0 = x + float(n);
+    }
+    const float x2 = 1.0f / (x0 * x0);
+    float g = 8.417508417508418e-04f;
+    g = g * x2 + -5.952380952380952e-04f;
+    g = g * x2 + 7.936507936507937e-04f;
+    g = g * x2 + -2.777777777777778e-03f;
+    g = g * x2 + 8.333333333333333e-02f;
+    float gl = ((g / x0 + 0.5f * spkm::logf_(6.283185307179586f)) + (x0 - 0.5f) * spkm::logf_(x0)) - x0;
+    if (x <= 7.0f)
+        for (int i = 0; i < n; ++i) {
+            gl = gl - spkm::logf_(x0 - 1.0f);
+            x0 = x0 - 1.0f;
+        }
+    return gl;
+}
+
+float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t seed) {
+    Stream g{pixel, stream, seed};
+    if (lam >= 10.0f) {
+        const float slam = std::sqrt(lam);
+        const float loglam = spkm::logf_(lam);
+        const float b = 0.931f + (2.53f * slam);
+        const float a = -0.059f + (0.02483f * b);
+        const float invalpha = 1.1239f + (1.1328f / (b - 3.4f));
+        const float vr = 0.9277f - (3.6224f / (b - 2.0f));
+        for (int it = 0; it < 64; ++it) {
+            const float U = g.uniform() - 0.5f;
+            const float V = g.uniform();
+            const float us = 0.5f - std::fabs(U);
+            const float k = std::floor(((((2.0f * a) / us + b) * U + lam) + 0.43f));
+            if (us >= 0.07f && V <= vr) return k;
+            if (k < 0.0f || (us < 0.013f && V > us)) continue;
+            const float L = ((spkm::logf_(V) + spkm::logf_(invalpha)) - spkm::logf_(a / (us * us) + b));
+            const float Rt = ((-lam + k * loglam) - lgamma_k(k + 1.0f));
+            if (L <= Rt) return k;
+        }
+        return std::floor(lam);
+    }
+    const float E = spkm::expf_(-lam);
+    float X = 0.0f, prod = 1.0f;
+    for (int it = 0; it < 512; ++it) {
+        prod = prod * g.uniform();
+        if (prod > E) X = X + 1.0f;
+        else return X;
+    }
+    return X;
+}
+
+float sublayer_draw(float d, float D, float n, float u, uint32_t pixel, uint32_t stream, uint32_t seed) {
+    float q = d / D;
+    q = q < 1e-6f ? 1e-6f : (q > 0.999999f ? 0.999999f : q);
+    const float sat = 1.0f - ((q * u) * 0.999999f);
+    const float lam = (n / sat) * q;
+    return poisson(lam, pixel, stream, seed) * ((D / n) * sat);
+}
+
+// Grain SUM draw of one pixel and channel from its density D (binary32 by definition).
+float grain_sum(const spk_plan& p, float D, uint32_t pixel, uint32_t c) {
+    const float v = p.grain_positive ? -D : D;
+    const uint32_t K = p.grain_k;
+    const float* axis = p.grain_axis;
+    const float* inv = p.grain_inv;
+    const float* layers = p.grain_layers;
+    float d[3];
+    if (v <= axis[c]) {
+        for (uint32_t l = 0; l < 3; ++l) d[l] = layers[l * 3 + c];
+    } else if (v >= axis[3 * (K - 1) + c]) {
+        for (uint32_t l = 0; l < 3; ++l) d[l] = layers[((K - 1) * 3 + l) * 3 + c];
+    } else {
+        uint32_t lo = 0, hi = K;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (axis[3 * mid + c] <= v) lo = mid + 1;
+            else hi = mid;
+        }
+        const uint32_t l0 = lo > 0 ? lo - 1 : 0;
+        const float t = (v - axis[3 * l0 + c]) * inv[3 * l0 + c];
+        for (uint32_t l = 0; l < 3; ++l) {
+            const float y0 = layers[(l0 * 3 + l) * 3 + c];
+            const float y1 = layers[((l0 + 1) * 3 + l) * 3 + c];
+            d[l] = y0 + t * (y1 - y0);
+        }
+    }
+    float val[3];
+    for (uint32_t l = 0; l < 3; ++l) {
+        const uint32_t lc = l * 3 + c;
+        val[l] = sublayer_draw(d[l] + p.dmin_l[lc], p.sat_d[lc], p.count[lc], p.uniformity[c], pixel, c + 10u * l, p.grain_seed);
+    }
+    float G = ((0.0f + val[0]) + val[1]) + val[2];
+    return G * 1.0f + (-p.dmin_c[c]);
+}
+
+template <class T>
+void develop(const spk_plan& p, const float* rgb, T* neg) {
+    const int w = int(p.width), h = int(p.height);
+    const size_t np = size_t(w) * h;
+    std::vector<T> H(3 * np);
+    const T* table = table_of<T>(p);
+    T a[9];
+    for (int i = 0; i < 9; ++i) a[i] = sizeof(T) == 4 ? T(p.a[i]) : T(p.a64[i]);
+    const T ec = sizeof(T) == 4 ? T(p.ec_gain) : T(double(p.ec_gain));
+    const bool ec_active = p.ec_gain != 1.0f;
+    const uint32_t L = p.L;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < int64_t(np); ++i) {
+        const T r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+        const T X = (a[0] * r + a[1] * g) + a[2] * b;
+        const T Y = (a[3] * r + a[4] * g) + a[5] * b;
+        const T Z = (a[6] * r + a[7] * g) + a[8] * b;
+        const T bs = (X + Y) + Z;
+        const T d = bs > T(1e-10) ? bs : T(1e-10);
+        const T x = X / d, y = Y / d, o = T(1) - x;
+        const T t0 = std::fmin(std::fmax(o * o, T(0)), T(1));
+        const T t1 = std::fmin(std::fmax(y / (o > T(1e-10) ? o : T(1e-10)), T(0)), T(1));
+        const T bp = std::isnan(bs) ? T(0) : bs;
+        int ia[4], ib[4];
+        T wa[4], wb[4];
+        mitchell_axis(t0, L, ia, wa);
+        mitchell_axis(t1, L, ib, wb);
+        T s = 0, a0 = 0, a1 = 0, a2 = 0;
+        for (int ii = 0; ii < 4; ++ii)
+            for (int jj = 0; jj < 4; ++jj) {
+                const T wgt = wa[ii] * wb[jj];
+                const T* t = table + 3 * (size_t(ia[ii]) * L + size_t(ib[jj]));
+                s += wgt;
+                a0 += wgt * t[0];
+                a1 += wgt * t[1];
+                a2 += wgt * t[2];
+            }
+        if (s != T(0)) {
+            a0 = a0 / s;
+            a1 = a1 / s;
+            a2 = a2 / s;
+        }
+        T h0 = bp * a0, h1 = bp * a1, h2 = bp * a2;
+        if (ec_active) {
+            h0 = h0 * ec;
+            h1 = h1 * ec;
+            h2 = h2 * ec;
+        }
+        H[i] = h0;
+        H[np + i] = h1;
+        H[2 * np + i] = h2;
+    }
+    std::vector<T> acc;
+    if (p.scatter_active || p.bounce_active) acc.assign(3 * np, T(0));
+    auto mixture = [&](const spk_mix_component* comps, uint32_t n) {
+        for (uint32_t k = 0; k < n; ++k)
+            blur3<T>(H.data(), acc.data(), comps[k].sigma, k == 0 ? 1 : 2, acc.data(), comps[k].weight, w, h, p.fir_crossover);
+    };
+    if (p.scatter_active) {
+        mixture(p.scatter, p.n_scatter);
+        for (int c = 0; c < 3; ++c) {
+            const T sa = T(float(p.scatter_a[c])), sb = T(float(p.scatter_b[c]));
+            for (size_t i = 0; i < np; ++i) H[c * np + i] = sa * H[c * np + i] + sb * acc[c * np + i];
+        }
+    }
+    if (p.bounce_active) mixture(p.bounce, p.n_bounce);
+    const uint32_t K = p.K;
+    std::vector<T> cx(K), cinv(K), cy(3 * K);
+    for (uint32_t k = 0; k < K; ++k) {
+        cx[k] = sizeof(T) == 4 ? T(p.curves[k]) : T(p.curves_x64[k]);
+        cinv[k] = sizeof(T) == 4 ? T(p.curves[K + k])
+                                 : (k + 1 < K && p.curves_x64[k + 1] != p.curves_x64[k] ? T(1.0 / (p.curves_x64[k + 1] - p.curves_x64[k])) : T(0));
+    }
+    for (uint32_t i = 0; i < 3 * K; ++i) cy[i] = sizeof(T) == 4 ? T(p.curves[2 * K + i]) : T(p.curves_y64[i]);
+    std::vector<T> G(p.grain_active ? 3 * np : 0), D(p.grain_active ? 3 * np : 0);
+    for (int c = 0; c < 3; ++c) {
+        const T ma = T(float(p.mix_a[c])), mb = T(float(p.mix_b[c]));
+#pragma omp parallel for schedule(dynamic, 4096)
+        for (int64_t i = 0; i < int64_t(np); ++i) {
+            const size_t o = size_t(c) * np + size_t(i);
+            T v = H[o];
+            if (p.bounce_active) v = ma * v + mb * acc[o];
+            const T Lg = o_log10(std::fmax(v, T(0)) + T(1e-10));
+            const T Dv = curve_lookup(cx.data(), cinv.data(), cy.data() + c * K, K, Lg);
+            if (!p.grain_active) {
+                neg[o] = Dv;
+            } else {
+                D[o] = Dv;
+                G[o] = T(grain_sum(p, float(Dv), uint32_t(i), uint32_t(c)));
+            }
+        }
+    }
+    if (!p.grain_active) return;
+    std::vector<T> Gb(3 * np);
+    if (p.grain_final_blur > 0.0) {
+        const double sg[3] = {p.grain_final_blur, p.grain_final_blur, p.grain_final_blur};
+        const double one[3] = {1, 1, 1};
+        blur3<T>(G.data(), Gb.data(), sg, 0, nullptr, one, w, h, p.fir_crossover);
+    } else {
+        Gb = G;
+    }
+    if (p.grain_amount == 1.0) {
+        std::memcpy(neg, Gb.data(), 3 * np * sizeof(T));
+    } else {
+        const T ka = T(float(1.0 - p.grain_amount)), kb = T(float(p.grain_amount));
+        for (size_t i = 0; i < 3 * np; ++i) neg[i] = ka * D[i] + kb * Gb[i];
+    }
+}
+
+// ---- print side ----------------------------------------------------------------------------------
+template <class T>
+T sgn(T v) {
+    return v > T(0) ? T(1) : (v < T(0) ? T(-1) : T(0));
+}
+
+template <class T>
+struct Cam {
+    T m_to_xyz[9], m_to_rgb[9], m16[9], m16_inv[9], d_rgb[3];
+    T f_l, n_bb, n_cb, n, z, a_w, c, n_c, j_first, j_last, h_first, h_step, knee[3], lc[3], j_white;
+    T cz, inv_cz, f_l_qrt, chroma_k, hundred_over_f_l;
+};
+
+template <class T>
+void mat_apply(const T* m, const T v[3], T o[3]) {
+    o[0] = (m[0] * v[0] + m[1] * v[1]) + m[2] * v[2];
+    o[1] = (m[3] * v[0] + m[4] * v[1]) + m[5] * v[2];
+    o[2] = (m[6] * v[0] + m[7] * v[1]) + m[8] * v[2];
+}
+
+template <class T>
+T knee(T d, T tau, T lim, T pw) {
+    if (d <= tau) return d;
+    const T x = (d - tau) / (lim - tau);
+    return tau + (lim - tau) * (x / std::pow(T(1) + std::pow(x, pw), T(1) / pw));
+}
+
+template <class T>
+T post_adapt(T v, T f_l) {
+    const T f = std::pow((f_l * std::fabs(v)) / T(100), T(0.42));
+    return ((T(400) * sgn(v)) * f) / (T(27.13) + f) + T(0.1);
+}
+
+template <class T>
+void cam16_forward(const T xyz[3], const Cam<T>& q, T out[3]) {
+    const T x100[3] = {T(100) * xyz[0], T(100) * xyz[1], T(100) * xyz[2]};
+    T rgb[3];
+    mat_apply(q.m16, x100, rgb);
+    const T ra = post_adapt(q.d_rgb[0] * rgb[0], q.f_l);
+    const T ga = post_adapt(q.d_rgb[1] * rgb[1], q.f_l);
+    const T ba = post_adapt(q.d_rgb[2] * rgb[2], q.f_l);
+    const T a = (ra - (T(12) * ga) / T(11)) + ba / T(11);
+    const T b = ((ra + ga) - T(2) * ba) / T(9);
+    const T h = std::atan2(b, a);
+    const T et = T(0.25) * (std::cos(h + T(2)) + T(3.8));
+    const T A = (((T(2) * ra + ga) + ba / T(20)) - T(0.305)) * q.n_bb;
+    const T ratio = A / q.a_w;
+    const T J = (T(100) * sgn(ratio)) * std::pow(std::fabs(ratio), q.cz);
+    const T den = (ra + ga) + (T(21) * ba) / T(20);
+    const T t = den != T(0) ? (((((T(50000) / T(13)) * q.n_c) * q.n_cb) * et) * std::sqrt(a * a + b * b)) / den : T(0);
+    const T C = t > T(0) ? (std::pow(t, T(0.9)) * std::sqrt(std::fabs(J) / T(100))) * q.chroma_k : T(0);
+    const T M = C * q.f_l_qrt;
+    out[1] = std::log(T(1) + T(0.0228) * M) / T(0.0228);
+    out[0] = (T(1.7) * J) / (T(1) + T(0.007) * J);
+    out[2] = h;
+}
+
+template <class T>
+void cam16_inverse(T Jp, T Mp, T h, const Cam<T>& q, T out[3]) {
+    const T M = (std::exp(T(0.0228) * Mp) - T(1)) / T(0.0228);
+    const T J = Jp / (T(1.7) - T(0.007) * Jp);
+    const T C = M / q.f_l_qrt;
+    const T A = (q.a_w * sgn(J)) * std::pow(std::fabs(J / T(100)), q.inv_cz);
+    const T sq = std::sqrt(std::fabs(J) / T(100));
+    const T t = (sq > T(0) && C > T(0)) ? std::pow(C / (sq * q.chroma_k), T(1) / T(0.9)) : T(0);
+    const T et = T(0.25) * (std::cos(h + T(2)) + T(3.8));
+    const T p2 = A / q.n_bb + T(0.305);
+    const T p3 = T(21) / T(20);
+    T a = 0, b = 0;
+    if (t != T(0)) {
+        const T p1 = ((((T(50000) / T(13)) * q.n_c) * q.n_cb) * et) / t;
+        const T sh = std::sin(h), ch = std::cos(h);
+        if (std::fabs(sh) >= std::fabs(ch)) {
+            const T p4 = p1 / sh;
+            b = ((p2 * (T(2) + p3)) * (T(460) / T(1403))) /
+                (((p4 + ((T(2) + p3) * (T(220) / T(1403))) * (ch / sh)) - T(27) / T(1403)) + p3 * (T(6300) / T(1403)));
+            a = (b * ch) / sh;
+        } else {
+            const T p5 = p1 / ch;
+            a = ((p2 * (T(2) + p3)) * (T(460) / T(1403))) /
+                ((p5 + (T(2) + p3) * (T(220) / T(1403))) - (T(27) / T(1403) - p3 * (T(6300) / T(1403))) * (sh / ch));
+            b = (a * sh) / ch;
+        }
+    }
+    const T xa[3] = {((T(460) * p2 + T(451) * a) + T(288) * b) / T(1403), ((T(460) * p2 - T(891) * a) - T(261) * b) / T(1403),
+                     ((T(460) * p2 - T(220) * a) - T(6300) * b) / T(1403)};
+    T rc[3];
+    for (int i = 0; i < 3; ++i) {
+        const T v = xa[i] - T(0.1);
+        const T av = std::fabs(v);
+        const T beta = av >= T(400) ? T(0) : (T(27.13) * av) / (T(400) - av);
+        rc[i] = ((q.hundred_over_f_l * sgn(v)) * std::pow(beta, T(1) / T(0.42))) / q.d_rgb[i];
+    }
+    T x100[3];
+    mat_apply(q.m16_inv, rc, x100);
+    out[0] = x100[0] / T(100);
+    out[1] = x100[1] / T(100);
+    out[2] = x100[2] / T(100);
+}
+
+template <class T>
+T cctf_encode(T v, uint32_t mode) {
+    switch (mode) {
+        case 0: return v <= T(0.0031308) ? T(12.92) * v : T(1.055) * (sgn(v) * std::pow(std::fabs(v), T(0.41666666666666669))) - T(0.055);
+        case 1: return v < T(0.001953125) ? T(16) * v : std::pow(v, T(0.55555555555555558));
+        case 2: return std::pow(v, T(0.45470692717584371));
+        case 3: return v < T(0.018) ? T(4.5) * v : T(1.099) * (sgn(v) * std::pow(std::fabs(v), T(0.45))) - T(0.099);
+        default: return v;
+    }
+}
+
+template <class T>
+uint16_t encode16(T v) {
+    if (std::isnan(v)) return 0;
+    const T c = std::fmin(std::fmax(v, T(0)), T(1));
+    return uint16_t(std::floor(c * T(65535) + T(0.5)));
+}
+
+template <class T>
+T paper_lookup(const spk_plan& p, uint32_t c, T v) {
+    const uint32_t K = p.paper_k;
+    if (std::isnan(v)) return v;
+    auto X = [&](uint32_t i) { return sizeof(T) == 4 ? T(p.paper[i]) : T(p.paper_x64[i]); };
+    auto Y = [&](uint32_t i, uint32_t ch) { return sizeof(T) == 4 ? T(p.paper[2 * K + 3 * i + ch]) : T(p.paper_y64[3 * i + ch]); };
+    auto R = [&](uint32_t i) {
+        if (sizeof(T) == 4) return T(p.paper[K + i]);
+        const double d = p.paper_x64[i + 1] - p.paper_x64[i];
+        return T(d == 0.0 ? 0.0 : 1.0 / d);
+    };
+    if (v <= X(0)) return Y(0, c);
+    if (v >= X(K - 1)) return Y(K - 1, c);
+    uint32_t lo = 0, hi = K;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (X(mid) <= v) lo = mid + 1;
+        else hi = mid;
+    }
+    const uint32_t j = lo > 0 ? lo - 1 : 0;
+    const T y0 = Y(j, c);
+    return y0 + ((v - X(j)) * R(j)) * (Y(j + 1, c) - y0);
+}
+
+template <class T>
+void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
+    const int w = int(p.width), h = int(p.height);
+    const size_t np = size_t(w) * h;
+    const bool f32 = sizeof(T) == 4;
+    auto pick = [&](float a, double b) { return f32 ? T(a) : T(b); };
+    T eps[SPK_NL3], beta[SPK_NL], wt[SPK_NL3], g[3], o[3], seps[SPK_NL3], sbeta[SPK_NL], swt[SPK_NL3], to_rgb[9], m_nid[9];
+    for (int i = 0; i < SPK_NL3; ++i) {
+        eps[i] = pick(p.enl_eps[i], p.enl_eps64[i]);
+        wt[i] = pick(p.enl_wt[i], p.enl_wt64[i]);
+        seps[i] = pick(p.scan_eps[i], p.scan_eps64[i]);
+        swt[i] = pick(p.scan_wt[i], p.scan_wt64[i]);
+    }
+    for (int i = 0; i < SPK_NL; ++i) {
+        beta[i] = pick(p.enl_beta[i], p.enl_beta64[i]);
+        sbeta[i] = pick(p.scan_beta[i], p.scan_beta64[i]);
+    }
+    for (int i = 0; i < 3; ++i) {
+        g[i] = pick(p.enl_g[i], p.enl_g64[i]);
+        o[i] = pick(p.enl_o[i], p.enl_o64[i]);
+    }
+    for (int i = 0; i < 9; ++i) {
+        to_rgb[i] = pick(p.to_rgb[i], p.to_rgb64[i]);
+        m_nid[i] = pick(p.m_nid[i], p.m_nid64[i]);
+    }
+    const T kg = pick(p.k_gain, p.k_gain64);
+    Cam<T> q{};
+    for (int i = 0; i < 9; ++i) {
+        q.m_to_xyz[i] = pick(p.m_to_xyz[i], p.m_to_xyz64[i]);
+        q.m_to_rgb[i] = pick(p.m_to_rgb[i], p.m_to_rgb64[i]);
+        q.m16[i] = pick(p.m16[i], p.m16_64[i]);
+        q.m16_inv[i] = pick(p.m16_inv[i], p.m16_inv64[i]);
+    }
+    const float k32[22] = {p.f_l, p.n_bb, p.n_cb, p.n_, p.z, p.a_w, p.c, p.n_c, p.j_first, p.j_last, p.h_first,
+                           p.h_step, p.knee[0], p.knee[1], p.knee[2], p.lc[0], p.lc[1], p.lc[2], p.j_white,
+                           p.d_rgb[0], p.d_rgb[1], p.d_rgb[2]};
+    T kk[22];
+    for (int i = 0; i < 22; ++i) kk[i] = pick(k32[i], p.cam64[i]);
+    q.f_l = kk[0];
+    q.n_bb = kk[1];
+    q.n_cb = kk[2];
+    q.n = kk[3];
+    q.z = kk[4];
+    q.a_w = kk[5];
+    q.c = kk[6];
+    q.n_c = kk[7];
+    q.j_first = kk[8];
+    q.j_last = kk[9];
+    q.h_first = kk[10];
+    q.h_step = kk[11];
+    for (int i = 0; i < 3; ++i) {
+        q.knee[i] = kk[12 + i];
+        q.lc[i] = kk[15 + i];
+        q.d_rgb[i] = kk[19 + i];
+    }
+    q.j_white = kk[18];
+    q.cz = q.c * q.z;
+    q.inv_cz = T(1) / q.cz;
+    q.f_l_qrt = std::pow(q.f_l, T(0.25));
+    q.chroma_k = std::pow(T(1.64) - std::pow(T(0.29), q.n), T(0.73));
+    q.hundred_over_f_l = T(100) / q.f_l;
+    const int rows = SPK_CMAX_ROWS, cols = SPK_CMAX_COLS;
+    const T L2 = K<T>::log2_10, tiny = K<T>::tiny;
+    std::vector<T> rgb(3 * np);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < int64_t(np); ++i) {
+        T d0 = neg[i], d1 = neg[np + i], d2 = neg[2 * np + i];
+        {
+            T r0 = 0, r1 = 0, r2 = 0;
+            for (int l = 0; l < SPK_NL; ++l) {
+                const T D = ((d0 * eps[3 * l] + d1 * eps[3 * l + 1]) + d2 * eps[3 * l + 2]) + beta[l];
+                const T Tl = p_exp2(-D * L2);
+                r0 += Tl * wt[3 * l];
+                r1 += Tl * wt[3 * l + 1];
+                r2 += Tl * wt[3 * l + 2];
+            }
+            T l0 = p_log10(std::fmax(r0 * g[0] + o[0], T(0)) + tiny);
+            T l1 = p_log10(std::fmax(r1 * g[1] + o[1], T(0)) + tiny);
+            T l2 = p_log10(std::fmax(r2 * g[2] + o[2], T(0)) + tiny);
+            l0 = p_log10(std::fmax(p_exp2(l0 * L2) * kg, T(0)) + tiny);
+            l1 = p_log10(std::fmax(p_exp2(l1 * L2) * kg, T(0)) + tiny);
+            l2 = p_log10(std::fmax(p_exp2(l2 * L2) * kg, T(0)) + tiny);
+            d0 = paper_lookup<T>(p, 0, l0);
+            d1 = paper_lookup<T>(p, 1, l1);
+            d2 = paper_lookup<T>(p, 2, l2);
+        }
+        T xyz[3];
+        {
+            T r0 = 0, r1 = 0, r2 = 0;
+            for (int l = 0; l < SPK_NL; ++l) {
+                const T D = ((d0 * seps[3 * l] + d1 * seps[3 * l + 1]) + d2 * seps[3 * l + 2]) + sbeta[l];
+                const T Tl = p_exp2(-D * L2);
+                r0 += Tl * swt[3 * l];
+                r1 += Tl * swt[3 * l + 1];
+                r2 += Tl * swt[3 * l + 2];
+            }
+            xyz[0] = std::fmax(r0, T(0)) + tiny;
+            xyz[1] = std::fmax(r1, T(0)) + tiny;
+            xyz[2] = std::fmax(r2, T(0)) + tiny;
+        }
+        T c3[3];
+        mat_apply(to_rgb, xyz, c3);
+        if (p.gamut_active) {
+            T gx[3], f[3];
+            mat_apply(q.m_to_xyz, c3, gx);
+            cam16_forward(gx, q, f);
+            T Jp = f[0];
+            const T Mp = f[1], hh = f[2];
+            if (p.lc_active) {
+                T Ln = Jp / q.j_white;
+                if (Ln > q.lc[0]) {
+                    Ln = knee(Ln, q.lc[0], q.lc[1], q.lc[2]);
+                    Jp = Ln * q.j_white;
+                }
+            }
+            const T Jc = std::fmin(std::fmax(Jp, q.j_first), q.j_last);
+            const T u = ((Jc - q.j_first) / (q.j_last - q.j_first)) * T(rows - 1);
+            const int l0 = std::min(std::max(int(std::floor(u)), 0), rows - 2);
+            const T fl = u - T(l0);
+            const T v = (hh - q.h_first) / q.h_step;
+            const T vf = std::floor(v);
+            int h0 = int(vf) % cols;
+            if (h0 < 0) h0 += cols;
+            const int h1 = h0 == cols - 1 ? 0 : h0 + 1;
+            const T fh = v - vf;
+            auto cm = [&](int idx) { return f32 ? T(p.cmax[idx]) : T(p.cmax64[idx]); };
+            const T t00 = cm(l0 * cols + h0), t01 = cm(l0 * cols + h1), t10 = cm((l0 + 1) * cols + h0),
+                    t11 = cm((l0 + 1) * cols + h1);
+            const T cmv = (T(1) - fl) * ((T(1) - fh) * t00 + fh * t01) + fl * ((T(1) - fh) * t10 + fh * t11);
+            const T cs = std::fmax(cmv, T(1e-9));
+            T dd = Mp / cs;
+            if (dd > T(0)) dd = knee(dd, q.knee[0], q.knee[1], q.knee[2]);
+            T back[3];
+            cam16_inverse(Jp, dd * cs, hh, q, back);
+            mat_apply(q.m_to_rgb, back, c3);
+        }
+        rgb[i] = c3[0];
+        rgb[np + i] = c3[1];
+        rgb[2 * np + i] = c3[2];
+    }
+    const double one[3] = {1, 1, 1};
+    if (p.scanner_lens_blur_px > 0.0) {
+        const double sb[3] = {p.scanner_lens_blur_px, p.scanner_lens_blur_px, p.scanner_lens_blur_px};
+        std::vector<T> t(rgb);
+        blur3<T>(t.data(), rgb.data(), sb, 0, nullptr, one, w, h, p.fir_crossover);
+    }
+    const bool unsharp = p.unsharp_sigma > 0.0 && p.unsharp_amount > 0.0;
+    if (unsharp) {
+        const double su[3] = {p.unsharp_sigma, p.unsharp_sigma, p.unsharp_sigma};
+        std::vector<T> gb(3 * np);
+        blur3<T>(rgb.data(), gb.data(), su, 0, nullptr, one, w, h, p.fir_crossover);
+        const T ua = pick(float(1.0 + p.unsharp_amount), 1.0 + p.unsharp_amount);
+        const T ub = pick(float(-p.unsharp_amount), -p.unsharp_amount);
+        for (size_t i = 0; i < 3 * np; ++i) rgb[i] = ua * rgb[i] + ub * gb[i];
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < int64_t(np); ++i) {
+        T v[3] = {rgb[i], rgb[np + i], rgb[2 * np + i]};
+        if (rgb_out)
+            for (int c = 0; c < 3; ++c) rgb_out[c * np + i] = v[c];
+        if (p.cctf_active) {
+            T m[3];
+            mat_apply(m_nid, v, m);
+            for (int c = 0; c < 3; ++c) v[c] = cctf_encode(m[c], p.cctf_mode);
+        }
+        for (int c = 0; c < 3; ++c) out[4 * i + c] = encode16(v[c]);
+        out[4 * i + 3] = 65535;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// negative: 3 planar planes (w*h*3).
+void oracle_develop_f32(const spk_plan* p, const float* rgb, float* negative) { develop<float>(*p, rgb, negative); }
+void oracle_develop_f64(const spk_plan* p, const float* rgb, double* negative) { develop<double>(*p, rgb, negative); }
+// out: RGBA16, row pitch = width; rgb_linear (optional): the planar linear RGB before encoding.
+void oracle_print_f32(const spk_plan* p, const float* negative, uint16_t* out, float* rgb_linear) {
+    print<float>(*p, negative, out, rgb_linear);
+}
+void oracle_print_f64(const spk_plan* p, const double* negative, uint16_t* out, double* rgb_linear) {
+    print<double>(*p, negative, out, rgb_linear);
+}
+// A per-channel Gaussian of an interleaved 3-channel image with the reference branch rules.
+void oracle_gaussian_f32(const float* in, float* out, int w, int h, const double* sigma3, double crossover) {
+    const size_t np = size_t(w) * h;
+    std::vector<float> a(3 * np), b(3 * np);
+    for (size_t i = 0; i < np; ++i)
+        for (int c = 0; c < 3; ++c) a[c * np + i] = in[3 * i + c];
+    const double one[3] = {1, 1, 1};
+    blur3<float>(a.data(), b.data(), sigma3, 0, nullptr, one, w, h, crossover);
+    for (size_t i = 0; i < np; ++i)
+        for (int c = 0; c < 3; ++c) out[3 * i + c] = b[c * np + i];
+}
+void oracle_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t c[4] = {ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3]};
+        philox(c, key[2 * i], key[2 * i + 1]);
+        std::memcpy(out + 4 * i, c, 16);
+    }
+}
+void oracle_poisson(const float* lam, uint32_t n, uint32_t stream, uint32_t seed, float* out) {
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < int64_t(n); ++i) out[i] = poisson(lam[i], uint32_t(i), stream, seed);
+}
+float oracle_uniform(uint32_t word) { return (float(word >> 8) + 0.5f) * 0x1p-24f; }
+double oracle_fir_tap(double sigma, int i) {
+    const std::vector<double> t = fir_taps64(sigma);
+    return (i >= 0 && size_t(i) < t.size()) ? t[size_t(i)] : 0.0;
+}
+int oracle_fir_radius(double sigma) { return fir_radius(sigma); }
+void oracle_yvv(double sigma, double* out4) {
+    const Yvv k = yvv(sigma);
+    out4[0] = k.B;
+    out4[1] = k.B1;
+    out4[2] = k.B2;
+    out4[3] = k.B3;
+}
+int oracle_reflect(int i, int n) { return reflect_index(i, n); }
+float oracle_log10f(float x) { return spkm::log10f_(x); }
+float oracle_logf(float x) { return spkm::logf_(x); }
+float oracle_expf(float x) { return spkm::expf_(x); }
+int oracle_threads(void) { return omp_get_max_threads(); }
+
+// Byte offset of a named spk_plan member (for reading plans from Python); -1 when unknown.
+long oracle_plan_offset(const char* name) {
+#define F(m) \
+    if (std::strcmp(name, #m) == 0) return long(offsetof(spk_plan, m));
+    F(width) F(height) F(pitch_um) F(rho_tier) F(fir_crossover) F(L) F(K) F(a) F(a64) F(ec_gain)
+    F(scatter_active) F(n_scatter) F(scatter) F(bounce_active) F(n_bounce) F(bounce) F(mix_a) F(mix_b)
+    F(grain_active) F(grain_mode) F(grain_seed) F(grain_k) F(dmin_l) F(sat_d) F(count) F(uniformity) F(dmin_c)
+    F(grain_final_blur) F(grain_amount) F(printed) F(paper_k) F(enl_eps64) F(enl_beta64) F(enl_wt64) F(enl_g64)
+    F(enl_o64) F(k_gain) F(k_gain64) F(scan_eps64) F(scan_beta64) F(scan_wt64) F(to_rgb64) F(cam64) F(m16_inv64)
+    F(m_to_xyz64) F(m_to_rgb64) F(unsharp_sigma) F(unsharp_amount) F(cctf_active) F(cctf_mode) F(m_nid64)
+    F(table) F(table64) F(curves) F(curves_x64) F(curves_y64) F(paper) F(paper_x64) F(paper_y64) F(cmax) F(cmax64)
+#undef F
+    return -1;
+}
+}

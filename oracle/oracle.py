@@ -1,0 +1,153 @@
+"""ORACLE -- test infrastructure only: ctypes loader of oracle/liboracle.so (oracle/chain.cpp).
+
+Used by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg to check or time the CPU
+restatement of the render chain.  The product library never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_lib = None
+
+
+def build():
+    r = subprocess.run(["make", "-C", str(HERE)], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout + r.stderr)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        l = C.CDLL(str(LIB))
+        F, D, U, U16 = C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.POINTER(C.c_uint16)
+        V = C.c_void_p
+        l.oracle_develop_f32.argtypes = [V, F, F]
+        l.oracle_develop_f64.argtypes = [V, F, D]
+        l.oracle_print_f32.argtypes = [V, F, U16, F]
+        l.oracle_print_f64.argtypes = [V, D, U16, D]
+        l.oracle_gaussian_f32.argtypes = [F, F, C.c_int, C.c_int, D, C.c_double]
+        l.oracle_philox.argtypes = [U, U, U, C.c_uint32]
+        l.oracle_poisson.argtypes = [F, C.c_uint32, C.c_uint32, C.c_uint32, F]
+        l.oracle_uniform.argtypes = [C.c_uint32]
+        l.oracle_uniform.restype = C.c_float
+        l.oracle_fir_tap.argtypes = [C.c_double, C.c_int]
+        l.oracle_fir_tap.restype = C.c_double
+        l.oracle_fir_radius.argtypes = [C.c_double]
+        l.oracle_yvv.argtypes = [C.c_double, D]
+        l.oracle_reflect.argtypes = [C.c_int, C.c_int]
+        l.oracle_plan_offset.argtypes = [C.c_char_p]
+        l.oracle_plan_offset.restype = C.c_long
+        for n in ("oracle_log10f", "oracle_logf", "oracle_expf"):
+            getattr(l, n).argtypes = [C.c_float]
+            getattr(l, n).restype = C.c_float
+        for fn in (l.oracle_develop_f32, l.oracle_develop_f64, l.oracle_print_f32, l.oracle_print_f64,
+                   l.oracle_gaussian_f32, l.oracle_philox, l.oracle_poisson):
+            fn.restype = None
+        _lib = l
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+_DT = {"width": "u4", "height": "u4", "L": "u4", "K": "u4", "scatter_active": "u4", "n_scatter": "u4",
+       "bounce_active": "u4", "n_bounce": "u4", "grain_active": "u4", "grain_mode": "u4", "grain_seed": "u4",
+       "grain_k": "u4", "printed": "u4", "paper_k": "u4", "cctf_active": "u4", "cctf_mode": "u4",
+       "a": "f4", "ec_gain": "f4", "dmin_l": "f4", "sat_d": "f4", "count": "f4", "uniformity": "f4",
+       "dmin_c": "f4", "k_gain": "f4", "table": "f4", "curves": "f4", "paper": "f4", "cmax": "f4"}
+
+
+def field(plan, name, count=1):
+    """A member of an spk_plan buffer as a numpy array (count elements)."""
+    off = lib().oracle_plan_offset(name.encode())
+    if off < 0:
+        raise KeyError(name)
+    dt = np.dtype(_DT.get(name, "f8"))
+    return np.frombuffer(bytes(plan[off:off + dt.itemsize * count]), dtype=dt)
+
+
+def plan_size(plan) -> tuple[int, int]:
+    """(width, height) of a plan buffer (spk_plan: magic, version, width, height, ...)."""
+    head = np.frombuffer(bytes(plan[:16]), dtype=np.uint32)
+    assert head[0] == 0x504B5053, "not an spk_plan"
+    return int(head[2]), int(head[3])
+
+
+def develop(plan, rgb, f64=False):
+    """rgb: H x W x 3 float32 (the session's source).  Returns the negative, 3 x H x W."""
+    w, h = plan_size(plan)
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    assert rgb.shape == (h, w, 3)
+    out = np.zeros(3 * w * h, dtype=np.float64 if f64 else np.float32)
+    if f64:
+        lib().oracle_develop_f64(C.cast(plan, C.c_void_p), _p(rgb, C.c_float), _p(out, C.c_double))
+    else:
+        lib().oracle_develop_f32(C.cast(plan, C.c_void_p), _p(rgb, C.c_float), _p(out, C.c_float))
+    return out.reshape(3, h, w)
+
+
+def print_(plan, negative, f64=False, want_rgb=False):
+    """negative: 3 x H x W.  Returns the RGBA16 image H x W x 4 (and the linear RGB)."""
+    w, h = plan_size(plan)
+    out = np.zeros(4 * w * h, dtype=np.uint16)
+    if f64:
+        neg = np.ascontiguousarray(negative, dtype=np.float64).ravel()
+        rgb = np.zeros(3 * w * h, dtype=np.float64) if want_rgb else None
+        lib().oracle_print_f64(C.cast(plan, C.c_void_p), _p(neg, C.c_double), _p(out, C.c_uint16),
+                               _p(rgb, C.c_double) if want_rgb else None)
+    else:
+        neg = np.ascontiguousarray(negative, dtype=np.float32).ravel()
+        rgb = np.zeros(3 * w * h, dtype=np.float32) if want_rgb else None
+        lib().oracle_print_f32(C.cast(plan, C.c_void_p), _p(neg, C.c_float), _p(out, C.c_uint16),
+                               _p(rgb, C.c_float) if want_rgb else None)
+    img = out.reshape(h, w, 4)
+    return (img, rgb.reshape(3, h, w)) if want_rgb else img
+
+
+def render(plan, rgb, f64=False):
+    return print_(plan, develop(plan, rgb, f64), f64)
+
+
+def gaussian(image, sigma3, crossover=3.0):
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    h, w, _ = img.shape
+    out = np.zeros_like(img)
+    s = (C.c_double * 3)(*[float(v) for v in sigma3])
+    lib().oracle_gaussian_f32(_p(img, C.c_float), _p(out, C.c_float), w, h, s, crossover)
+    return out
+
+
+def philox(ctr, key):
+    ctr = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 4)
+    key = np.ascontiguousarray(key, dtype=np.uint32).reshape(-1, 2)
+    out = np.zeros_like(ctr)
+    lib().oracle_philox(_p(ctr, C.c_uint32), _p(key, C.c_uint32), _p(out, C.c_uint32), len(ctr))
+    return out
+
+
+def poisson(lam, stream, seed):
+    lam = np.ascontiguousarray(lam, dtype=np.float32).ravel()
+    out = np.zeros_like(lam)
+    lib().oracle_poisson(_p(lam, C.c_float), len(lam), stream, seed, _p(out, C.c_float))
+    return out
+
+
+def fir_taps(sigma):
+    r = lib().oracle_fir_radius(sigma)
+    return np.array([lib().oracle_fir_tap(sigma, i) for i in range(2 * r + 1)])
+
+
+def yvv(sigma):
+    out = (C.c_double * 4)()
+    lib().oracle_yvv(sigma, out)
+    return tuple(out)

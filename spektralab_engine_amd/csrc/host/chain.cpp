@@ -1,0 +1,373 @@
+// The render chain on the device; stage order and branch rules stated in chain.h.
+#include "chain.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "../kernels/kernels.h"
+#include "setup.h"
+#include "util.h"
+
+namespace spk::chain {
+
+namespace {
+
+gpu::DevBuf upload(const void* src, size_t bytes, gpu::Pool& pool) {
+    gpu::DevBuf b = pool.acquire(bytes);
+    gpu::check(hipMemcpy(b->ptr, src, bytes, hipMemcpyHostToDevice), "hipMemcpy(table)");
+    return b;
+}
+
+using M3 = std::array<double, 9>;
+
+M3 mat_mul(const M3& a, const M3& b) {
+    M3 r{};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += a[3 * i + k] * b[3 * k + j];
+            r[3 * i + j] = s;
+        }
+    return r;
+}
+
+M3 mat_pow(M3 a, uint64_t e) {
+    M3 r{1, 0, 0, 0, 1, 0, 0, 0, 1};
+    while (e) {
+        if (e & 1) r = mat_mul(r, a);
+        a = mat_mul(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+
+// Recursive-pass arguments for lines of length n with nch chunks of m (last chunk n - (nch-1) m).
+k::IirArgs iir_args(double sigma, uint32_t w, uint32_t h, uint32_t n, uint32_t m, uint32_t nch) {
+    const setup::IirCoefficients c = setup::iir_coefficients(sigma);
+    k::IirArgs a{};
+    a.w = w;
+    a.h = h;
+    a.n = n;
+    a.m = m;
+    a.nch = nch;
+    a.B = c.B;
+    a.B1 = c.B1;
+    a.B2 = c.B2;
+    a.B3 = c.B3;
+    const M3 A{c.B1, c.B2, c.B3, 1, 0, 0, 0, 1, 0};
+    for (int d = 0; d < 7; ++d) {
+        const M3 p = mat_pow(A, uint64_t(m) << d);
+        std::memcpy(a.pw[d], p.data(), sizeof p);
+    }
+    const M3 l = mat_pow(A, uint64_t(n) - uint64_t(nch - 1) * m);
+    std::memcpy(a.last, l.data(), sizeof l);
+    return a;
+}
+
+void fir_channel(const float* in, float* out, double sigma, uint32_t mode, const float* acc, double weight, uint32_t w,
+                 uint32_t h, float* tmp, hipStream_t s) {
+    const std::vector<float> taps = setup::fir_taps(sigma);
+    const int r = int(taps.size() / 2);
+    if (r > k::kFirMaxR) fail_internal("FIR radius above the kernel maximum");
+    k::FirArgs a{};
+    a.w = w;
+    a.h = h;
+    a.r = uint32_t(r);
+    for (size_t i = 0; i < taps.size(); ++i) a.taps[i] = taps[i];
+    a.acc_mode = 0;
+    k::fir_v(in, tmp, a, s);
+    a.acc_mode = mode;
+    a.weight = num::f32(weight);
+    k::fir_h(tmp, out, acc, a, s);
+}
+
+void iir_channel(const float* in, float* out, double sigma, uint32_t mode, const float* acc, double weight, uint32_t w,
+                 uint32_t h, float* tmp, float* tmp2, hipStream_t s) {
+    // rows: 128 threads, chunks of m = ceil(w / 128)
+    {
+        const uint32_t m = (w + 127) / 128;
+        const uint32_t nch = (w + m - 1) / m;
+        k::IirArgs a = iir_args(sigma, w, h, w, m, nch);
+        k::iir_row(in, tmp, a, s);
+    }
+    // columns: 32 chunks of a compile-time length; frames taller than 32 * 128 run transposed
+    uint32_t mc = 0;
+    for (uint32_t c : {16u, 32u, 64u, 128u})
+        if ((h + c - 1) / c <= 32) {
+            mc = c;
+            break;
+        }
+    if (mc) {
+        k::IirArgs a = iir_args(sigma, w, h, h, mc, (h + mc - 1) / mc);
+        a.acc_mode = mode;
+        a.weight = num::f32(weight);
+        k::iir_col(tmp, out, acc, a, s);
+    } else {
+        k::transpose(tmp, tmp2, nullptr, w, h, 0, 1.0f, s);  // tmp2: h wide, w tall
+        const uint32_t m = (h + 127) / 128;
+        const uint32_t nch = (h + m - 1) / m;
+        k::IirArgs a = iir_args(sigma, h, w, h, m, nch);
+        k::iir_row(tmp2, tmp2, a, s);
+        k::transpose(tmp2, out, acc, h, w, mode, num::f32(weight), s);
+    }
+}
+
+}  // namespace
+
+void blur3(const float* in, float* out, const double sigma[3], uint32_t mode, const float* acc, const double weight[3],
+           uint32_t w, uint32_t h, double crossover, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep) {
+    const size_t np = size_t(w) * h;
+    gpu::DevBuf tmp = pool.acquire(np * sizeof(float));
+    gpu::DevBuf tmp2;
+    keep.push_back(tmp);
+    for (int c = 0; c < 3; ++c) {
+        const float* ic = in + size_t(c) * np;
+        float* oc = out + size_t(c) * np;
+        const float* ac = acc ? acc + size_t(c) * np : nullptr;
+        const double sg = sigma[c];
+        const int r = setup::fir_radius(sg);
+        if (!(sg > 0.0) || (sg < crossover && r == 0)) {  // identity
+            if (mode == 0) k::copy(ic, oc, np, s);
+            else if (mode == 1) k::lincomb(ic, nullptr, oc, uint32_t(np), num::f32(weight[c]), 0.0f, s);
+            else k::lincomb(ac, ic, oc, uint32_t(np), 1.0f, num::f32(weight[c]), s);
+        } else if (sg < crossover) {
+            fir_channel(ic, oc, sg, mode, ac, weight[c], w, h, gpu::as<float>(tmp), s);
+        } else {
+            if (!tmp2 && h > 32 * 128) {
+                tmp2 = pool.acquire(np * sizeof(float));
+                keep.push_back(tmp2);
+            }
+            iir_channel(ic, oc, sg, mode, ac, weight[c], w, h, gpu::as<float>(tmp), gpu::as<float>(tmp2), s);
+        }
+    }
+}
+
+namespace {
+
+// M(in) accumulated in list order into out (src/fx/gaussian.cpp:254-273), out != in.
+void mixture(const float* in, float* out, const spk_mix_component* comps, uint32_t n, uint32_t w, uint32_t h,
+             double crossover, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep) {
+    for (uint32_t k = 0; k < n; ++k) {
+        blur3(in, out, comps[k].sigma, k == 0 ? 1u : 2u, out, comps[k].weight, w, h, crossover, pool, s, keep);
+    }
+}
+
+}  // namespace
+
+void Profiler::mark(const char* label, hipStream_t s) {
+    hipEvent_t e;
+    if (pool.empty()) {
+        gpu::check(hipEventCreate(&e), "hipEventCreate");
+    } else {
+        e = pool.back();
+        pool.pop_back();
+    }
+    gpu::check(hipEventRecord(e, s), "hipEventRecord");
+    marks.emplace_back(label, e);
+}
+
+void Profiler::collect() {
+    for (size_t i = 1; i < marks.size(); ++i) {
+        if (marks[i].first == "start") continue;
+        float ms = 0.0f;
+        gpu::check(hipEventElapsedTime(&ms, marks[i - 1].second, marks[i].second), "hipEventElapsedTime");
+        auto it = std::find_if(acc.begin(), acc.end(), [&](const auto& a) { return a.first == marks[i].first; });
+        if (it == acc.end()) acc.push_back({marks[i].first, {double(ms), 1}});
+        else {
+            it->second.first += ms;
+            it->second.second += 1;
+        }
+    }
+    for (auto& m : marks) pool.push_back(m.second);
+    marks.clear();
+}
+
+Profiler::~Profiler() {
+    for (auto& m : marks) (void)hipEventDestroy(m.second);
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+Tables upload_tables(const spk_plan& p, gpu::Pool& pool) {
+    Tables t;
+    t.L = p.L;
+    t.K = p.K;
+    t.table = upload(p.table, size_t(p.L) * p.L * 3 * sizeof(float), pool);
+    t.curves = upload(p.curves, 8 * size_t(p.K) * sizeof(float), pool);
+    const size_t gk = std::max<size_t>(p.grain_k, 1);
+    t.gaxis = upload(p.grain_axis, 3 * gk * sizeof(float), pool);
+    t.ginv = upload(p.grain_inv, 3 * gk * sizeof(float), pool);
+    t.glayers = upload(p.grain_layers, 9 * gk * sizeof(float), pool);
+    t.paper = upload(p.paper, 5 * size_t(std::max<uint32_t>(p.paper_k, 1)) * sizeof(float), pool);
+    t.cmax = upload(p.cmax, SPK_CMAX_N * sizeof(float), pool);
+
+    k::PrintConsts q{};
+    q.paper_k = p.paper_k;
+    std::memcpy(q.scan_eps, p.scan_eps, sizeof q.scan_eps);
+    std::memcpy(q.scan_beta, p.scan_beta, sizeof q.scan_beta);
+    std::memcpy(q.scan_wt, p.scan_wt, sizeof q.scan_wt);
+    std::memcpy(q.to_rgb, p.to_rgb, sizeof q.to_rgb);
+    std::memcpy(q.m_to_xyz, p.m_to_xyz, sizeof q.m_to_xyz);
+    std::memcpy(q.m_to_rgb, p.m_to_rgb, sizeof q.m_to_rgb);
+    std::memcpy(q.m16, p.m16, sizeof q.m16);
+    std::memcpy(q.m16_inv, p.m16_inv, sizeof q.m16_inv);
+    std::memcpy(q.d_rgb, p.d_rgb, sizeof q.d_rgb);
+    q.f_l = p.f_l;
+    q.n_bb = p.n_bb;
+    q.n_cb = p.n_cb;
+    q.n_ = p.n_;
+    q.z = p.z;
+    q.a_w = p.a_w;
+    q.c = p.c;
+    q.n_c = p.n_c;
+    q.j_first = p.j_first;
+    q.j_last = p.j_last;
+    q.h_first = p.h_first;
+    q.h_step = p.h_step;
+    std::memcpy(q.knee, p.knee, sizeof q.knee);
+    std::memcpy(q.lc, p.lc, sizeof q.lc);
+    q.j_white = p.j_white;
+    q.lc_active = p.lc_active;
+    q.gamut_active = p.gamut_active;
+    q.rows = SPK_CMAX_ROWS;
+    q.cols = SPK_CMAX_COLS;
+    q.cctf_active = p.cctf_active;
+    q.cctf_mode = p.cctf_mode;
+    std::memcpy(q.m_nid, p.m_nid, sizeof q.m_nid);
+    q.unsharp_active = (p.unsharp_sigma > 0.0 && p.unsharp_amount > 0.0) ? 1u : 0u;
+    q.unsharp_a = num::f32(1.0 + p.unsharp_amount);
+    q.unsharp_b = num::f32(-p.unsharp_amount);
+    t.print_consts = upload(&q, sizeof q, pool);
+    return t;
+}
+
+void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
+             std::vector<gpu::DevBuf>& keep, Profiler* prof) {
+    auto mark = [&](const char* l) {
+        if (prof && prof->on) prof->mark(l, s);
+    };
+    mark("start");
+    const uint32_t w = p.width, h = p.height;
+    const size_t np = size_t(w) * h;
+    gpu::DevBuf Hb = pool.acquire(3 * np * sizeof(float));
+    keep.push_back(Hb);
+    float* H = gpu::as<float>(Hb);
+
+    k::UpsampleArgs ua{};
+    ua.n = uint32_t(np);
+    ua.L = p.L;
+    std::memcpy(ua.a, p.a, sizeof ua.a);
+    ua.ec_gain = p.ec_gain;
+    ua.ec_active = p.ec_gain != 1.0f ? 1u : 0u;
+    k::upsample(rgb, H, gpu::as<float>(t.table), ua, s);
+    mark("upsample");
+
+    gpu::DevBuf accb;
+    if (p.scatter_active || p.bounce_active) {
+        accb = pool.acquire(3 * np * sizeof(float));
+        keep.push_back(accb);
+    }
+    float* acc = gpu::as<float>(accb);
+    if (p.scatter_active) {
+        mixture(H, acc, p.scatter, p.n_scatter, w, h, p.fir_crossover, pool, s, keep);
+        for (int c = 0; c < 3; ++c)
+            k::lincomb(H + c * np, acc + c * np, H + c * np, uint32_t(np), num::f32(p.scatter_a[c]),
+                       num::f32(p.scatter_b[c]), s);
+    }
+    if (p.bounce_active) mixture(H, acc, p.bounce, p.n_bounce, w, h, p.fir_crossover, pool, s, keep);
+    if (p.scatter_active || p.bounce_active) mark("halation");
+
+    k::DevelopArgs da{};
+    da.w = w;
+    da.h = h;
+    da.K = p.K;
+    da.mix_active = p.bounce_active;
+    for (int c = 0; c < 3; ++c) {
+        da.mix_a[c] = num::f32(p.mix_a[c]);
+        da.mix_b[c] = num::f32(p.mix_b[c]);
+    }
+    da.grain_active = p.grain_active;
+    const bool blend = p.grain_active && p.grain_amount != 1.0;
+    da.write_density = blend ? 1u : 0u;
+    da.seed = p.grain_seed;
+    da.positive = p.grain_positive;
+    da.gk = p.grain_k;
+    std::memcpy(da.dmin_l, p.dmin_l, sizeof da.dmin_l);
+    std::memcpy(da.sat_d, p.sat_d, sizeof da.sat_d);
+    std::memcpy(da.count, p.count, sizeof da.count);
+    std::memcpy(da.uniformity, p.uniformity, sizeof da.uniformity);
+    std::memcpy(da.dmin_c, p.dmin_c, sizeof da.dmin_c);
+
+    if (!p.grain_active) {
+        k::develop(H, acc, negative, nullptr, gpu::as<float>(t.curves), nullptr, nullptr, nullptr, da, s);
+        mark("develop");
+        return;
+    }
+    gpu::DevBuf Gb = pool.acquire(3 * np * sizeof(float));
+    keep.push_back(Gb);
+    float* G = gpu::as<float>(Gb);
+    gpu::DevBuf Db;
+    if (blend) {
+        Db = pool.acquire(3 * np * sizeof(float));
+        keep.push_back(Db);
+    }
+    k::develop(H, acc, G, gpu::as<float>(Db), gpu::as<float>(t.curves), gpu::as<float>(t.gaxis), gpu::as<float>(t.ginv),
+               gpu::as<float>(t.glayers), da, s);
+    mark("develop");
+    float* Gout = blend ? G : negative;
+    if (p.grain_final_blur > 0.0) {
+        const double sg[3] = {p.grain_final_blur, p.grain_final_blur, p.grain_final_blur};
+        const double one[3] = {1.0, 1.0, 1.0};
+        blur3(G, Gout, sg, 0, nullptr, one, w, h, p.fir_crossover, pool, s, keep);
+    } else if (Gout != G) {
+        k::copy(G, Gout, 3 * np, s);
+    }
+    if (blend) {
+        const float a = num::f32(1.0 - p.grain_amount), b = num::f32(p.grain_amount);
+        k::lincomb(gpu::as<float>(Db), G, negative, uint32_t(3 * np), a, b, s);
+    }
+    mark("grain_blur");
+}
+
+void print(const spk_plan& p, const Tables& t, const float* negative, uint16_t* out, uint32_t stride, gpu::Pool& pool,
+           hipStream_t s, std::vector<gpu::DevBuf>& keep, Profiler* prof) {
+    auto mark = [&](const char* l) {
+        if (prof && prof->on) prof->mark(l, s);
+    };
+    mark("start");
+    const uint32_t w = p.width, h = p.height;
+    const size_t np = size_t(w) * h;
+    if (!p.printed) fail_internal("print: the scanned-film path is not built");
+    k::EnlargerArgs ea{};
+    std::memcpy(ea.eps, p.enl_eps, sizeof ea.eps);
+    std::memcpy(ea.beta, p.enl_beta, sizeof ea.beta);
+    std::memcpy(ea.wt, p.enl_wt, sizeof ea.wt);
+    std::memcpy(ea.g, p.enl_g, sizeof ea.g);
+    std::memcpy(ea.o, p.enl_o, sizeof ea.o);
+    ea.k_gain = p.k_gain;
+    const k::PrintConsts* dq = gpu::as<const k::PrintConsts>(t.print_consts);
+
+    gpu::DevBuf rgbb = pool.acquire(3 * np * sizeof(float));
+    keep.push_back(rgbb);
+    float* rgb = gpu::as<float>(rgbb);
+    k::print_pointwise(negative, rgb, dq, ea, gpu::as<float>(t.paper), gpu::as<float>(t.cmax), w, h, s);
+    mark("print");
+    const double one[3] = {1.0, 1.0, 1.0};
+    if (p.scanner_lens_blur_px > 0.0) {
+        const double sb[3] = {p.scanner_lens_blur_px, p.scanner_lens_blur_px, p.scanner_lens_blur_px};
+        blur3(rgb, rgb, sb, 0, nullptr, one, w, h, p.fir_crossover, pool, s, keep);
+    }
+    gpu::DevBuf blurb;
+    const double su = p.unsharp_sigma;
+    if (su > 0.0 && p.unsharp_amount > 0.0) {
+        blurb = pool.acquire(3 * np * sizeof(float));
+        keep.push_back(blurb);
+        const double sg[3] = {su, su, su};
+        blur3(rgb, gpu::as<float>(blurb), sg, 0, nullptr, one, w, h, p.fir_crossover, pool, s, keep);
+    }
+    mark("unsharp_blur");
+    k::unsharp_encode(rgb, gpu::as<float>(blurb), out, stride, dq, w, h, s);
+    mark("encode");
+}
+
+}  // namespace spk::chain

@@ -1,0 +1,56 @@
+// The render chain on the device: film side (develop) and print side (print, scan, encode),
+// enqueued on one stream from an spk_plan.
+//
+// Stage order follows the reference's film::develop (src/film/develop.cpp:101-236) and
+// print::print_scan_encode (src/print/run.cpp:118-188), with the pointwise runs fused:
+//   develop  upsample+EC | [scatter mixture + lincomb] | [bounce mixture] |
+//            mix+log10+curves+grain draw | [grain final blur] -> negative
+//   print    enlarger+gain+paper+scan+to_rgb+gamut | [scanner lens blur] | [unsharp blur] |
+//            unsharp combine+matrix+transfer+16-bit encode -> RGBA16
+// The blur machinery mirrors src/fx/gaussian.cpp:201-273 per channel: sigma <= 0 identity,
+// 0 < sigma < crossover FIR (vertical then horizontal, accumulation fused into the horizontal
+// pass), otherwise Young-van Vliet (rows then columns, accumulation fused into the column pass).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "gpu.h"
+#include "spektralab_plan.h"
+
+namespace spk::chain {
+
+// Optional stage timing with HIP events recorded on the render stream: mark(label) closes the
+// stage that began at the previous mark; collect() (after the stream has drained) accumulates
+// the elapsed milliseconds per label.
+struct Profiler {
+    bool on = false;
+    std::vector<std::pair<std::string, hipEvent_t>> marks;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<std::string, std::pair<double, int>>> acc;
+    void mark(const char* label, hipStream_t s);
+    void collect();
+    ~Profiler();
+};
+
+// Device copies of a plan's pipeline-constant tables and print constants (uploaded once per
+// pipeline; the per-run enlarger constants travel as kernel arguments).
+struct Tables {
+    gpu::DevBuf table, curves, gaxis, ginv, glayers, paper, cmax, print_consts;
+    uint32_t L = 0, K = 0;
+};
+Tables upload_tables(const spk_plan& p, gpu::Pool& pool);
+
+// rgb: interleaved w x h x 3 on the device; negative: 3 planar planes (w*h*3 floats).
+void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
+             std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr);
+// out: RGBA16 with a row pitch of `stride` pixels.
+void print(const spk_plan& p, const Tables& t, const float* negative, uint16_t* out, uint32_t stride, gpu::Pool& pool,
+           hipStream_t s, std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr);
+
+// One per-channel separable Gaussian on 3 planar planes (exposed for the blur tests):
+// out_c = fuse(mode, acc_c, weight_c, G_{sigma_c}(in_c)); mode 0 none, 1 zero, 2 plane.
+void blur3(const float* in, float* out, const double sigma[3], uint32_t mode, const float* acc, const double weight[3],
+           uint32_t w, uint32_t h, double crossover, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep);
+
+}  // namespace spk::chain

@@ -1,0 +1,112 @@
+// The HIP runtime layer; behaviour stated in gpu.h.
+#include "gpu.h"
+
+#include "util.h"
+
+namespace spk::gpu {
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail_gpu(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void release_block(Block* b) {
+    if (b->pool) b->pool->give_back(b);
+    delete b;
+}
+
+Pool::~Pool() { trim(); }
+
+DevBuf Pool::acquire(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    bytes = (bytes + 255) & ~size_t(255);
+    {
+        std::lock_guard<std::mutex> g(m_);
+        auto it = idle_.lower_bound(bytes);
+        if (it != idle_.end() && it->first < 2 * bytes) {
+            Block* b = new Block{it->second, it->first, this};
+            idle_bytes_ -= it->first;
+            live_ += it->first;
+            high_ = std::max(high_, live_);
+            idle_.erase(it);
+            return DevBuf(b, release_block);
+        }
+    }
+    DeviceScope ds(device_);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        trim();
+        check(hipMalloc(&p, bytes), "hipMalloc");
+    }
+    std::lock_guard<std::mutex> g(m_);
+    live_ += bytes;
+    high_ = std::max(high_, live_);
+    return DevBuf(new Block{p, bytes, this}, release_block);
+}
+
+void Pool::give_back(Block* b) {
+    std::lock_guard<std::mutex> g(m_);
+    live_ -= b->bytes;
+    idle_bytes_ += b->bytes;
+    idle_.emplace(b->bytes, b->ptr);
+}
+
+size_t Pool::trim() {
+    std::multimap<size_t, void*> drop;
+    {
+        std::lock_guard<std::mutex> g(m_);
+        drop.swap(idle_);
+        idle_bytes_ = 0;
+    }
+    size_t freed = 0;
+    if (!drop.empty()) {
+        DeviceScope ds(device_);
+        (void)hipDeviceSynchronize();
+        for (auto& [n, p] : drop) {
+            (void)hipFree(p);
+            freed += n;
+        }
+    }
+    return freed;
+}
+
+size_t Pool::live_bytes() const {
+    std::lock_guard<std::mutex> g(m_);
+    return live_;
+}
+size_t Pool::idle_bytes() const {
+    std::lock_guard<std::mutex> g(m_);
+    return idle_bytes_;
+}
+size_t Pool::high_water() const {
+    std::lock_guard<std::mutex> g(m_);
+    return high_;
+}
+
+PinnedBuf::PinnedBuf(size_t n) : bytes(n) { check(hipHostMalloc(&ptr, n ? n : 1, hipHostMallocDefault), "hipHostMalloc"); }
+PinnedBuf::~PinnedBuf() {
+    if (ptr) (void)hipHostFree(ptr);
+}
+
+Stream::Stream(int device) {
+    DeviceScope ds(device);
+    check(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking), "hipStreamCreate");
+}
+Stream::~Stream() {
+    if (s_) (void)hipStreamDestroy(s_);
+}
+void Stream::sync() const { check(hipStreamSynchronize(s_), "hipStreamSynchronize"); }
+
+DeviceScope::DeviceScope(int device) { check(hipSetDevice(device), "hipSetDevice"); }
+
+int device_count() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+}  // namespace spk::gpu

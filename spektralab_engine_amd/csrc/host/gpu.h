@@ -1,0 +1,86 @@
+// The HIP runtime layer: device selection, a pooled device allocator and pinned host buffers.
+//
+// Replaces the reference's Metal bridge (src/gpu/gpu.hpp:39-240): shared-storage buffers become
+// device allocations (hipMalloc) reused through a best-fit pool (a free block of [b, 2b) bytes
+// serves a request of b, as gpu.mm's Pool does), and host-visible results are pinned host buffers
+// (hipHostMalloc).  Every session renders on its own non-blocking stream of default priority.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+
+namespace spk::gpu {
+
+void check(hipError_t e, const char* what);  // throws spk::Error(SPK_ERR_GPU)
+
+class Pool;
+
+struct Block {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    Pool* pool = nullptr;
+};
+
+// A device allocation returned to its pool when the last reference goes.
+using DevBuf = std::shared_ptr<Block>;
+
+class Pool : public std::enable_shared_from_this<Pool> {
+public:
+    explicit Pool(int device) : device_(device) {}
+    ~Pool();
+    DevBuf acquire(size_t bytes);
+    size_t trim();  // free every idle block; returns bytes freed
+    size_t live_bytes() const;
+    size_t idle_bytes() const;
+    size_t high_water() const;
+    int device() const { return device_; }
+
+private:
+    friend void release_block(Block*);
+    void give_back(Block* b);
+    int device_;
+    mutable std::mutex m_;
+    std::multimap<size_t, void*> idle_;
+    size_t live_ = 0, idle_bytes_ = 0, high_ = 0;
+};
+
+template <class T>
+T* as(const DevBuf& b) {
+    return b ? static_cast<T*>(b->ptr) : nullptr;
+}
+
+// Pinned host memory (hipHostMalloc), freed on destruction.
+struct PinnedBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    explicit PinnedBuf(size_t n);
+    ~PinnedBuf();
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+};
+
+class Stream {
+public:
+    explicit Stream(int device);
+    ~Stream();
+    hipStream_t get() const { return s_; }
+    void sync() const;
+
+private:
+    hipStream_t s_ = nullptr;
+};
+
+// Makes `device` current on this thread for the scope.
+struct DeviceScope {
+    explicit DeviceScope(int device);
+};
+
+int device_count();
+
+}  // namespace spk::gpu

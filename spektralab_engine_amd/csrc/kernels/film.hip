@@ -1,0 +1,355 @@
+// Film-side pointwise kernels for gfx950: spectral upsampling (+ exposure compensation) and the
+// fused develop kernel (halation mix, log10 exposure, density curves, Poisson grain draw).
+//
+// This file is compiled with -ffp-contract=off: every float operation is rounded as written, in
+// the reference kernels' bracketed order, so the cached negative is bit-identical to the CPU
+// oracle's.  log / exp / log10 come from spk_math.h (binary64 evaluation, one final rounding).
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "spk_math.h"
+
+#pragma clang fp contract(off)
+
+namespace spk::k {
+namespace {
+
+// ---- Mitchell-Netravali (B = C = 1/3) and the table lookup (kernels/film.metal:58-101) --------
+__device__ inline float mitchell(float u) {
+    const float a = fabsf(u);
+    if (a < 1.0f) return (1.0f / 6.0f) * (7.0f * a * a * a - 12.0f * a * a + 16.0f / 3.0f);
+    if (a < 2.0f) return (1.0f / 6.0f) * ((-7.0f / 3.0f) * a * a * a + 12.0f * a * a - 20.0f * a + 32.0f / 3.0f);
+    return 0.0f;
+}
+
+__device__ inline void mitchell_axis(float t, uint32_t L, int* idx, float* w) {
+    const float last = float(L - 1);
+    float u = t * last;
+    int base;
+    float frac;
+    if (u <= 0.0f) u = 0.0f;
+    if (u >= last) {
+        base = int(L) - 2;
+        frac = 1.0f;
+    } else {
+        base = int(floorf(u));
+        frac = u - float(base);
+    }
+    const int n = int(L);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int i = base - 1 + k;
+        if (i < 0) i = -i;
+        if (i >= n) i = 2 * (n - 1) - i;
+        idx[k] = i;
+    }
+    w[0] = mitchell(frac + 1.0f);
+    w[1] = mitchell(frac);
+    w[2] = mitchell(frac - 1.0f);
+    w[3] = mitchell(frac - 2.0f);
+}
+
+// F(v; x, inv, y): clamped piecewise-linear with a right bisection (kernels/film.metal:88-101).
+__device__ inline float curve_lookup(const float* __restrict__ x, const float* __restrict__ inv,
+                                     const float* __restrict__ y, uint32_t K, float v) {
+    if (v <= x[0]) return y[0];
+    if (v >= x[K - 1]) return y[K - 1];
+    uint32_t lo = 0, hi = K;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (x[mid] <= v) lo = mid + 1;
+        else hi = mid;
+    }
+    const uint32_t l = lo == 0 ? 0 : lo - 1;
+    const float t = (v - x[l]) * inv[l];
+    return y[l] + t * (y[l + 1] - y[l]);
+}
+
+// ---- Philox4x32-10 (Salmon et al., SC 2011) and the per-(pixel, stream) word stream ---------------
+struct Philox {
+    uint32_t pixel, stream, seed, j, used;
+    uint32_t b0, b1, b2, b3;
+    __device__ void init(uint32_t p, uint32_t s, uint32_t sd) {
+        pixel = p;
+        stream = s;
+        seed = sd;
+        j = 0;
+        used = 4;
+    }
+    __device__ void block() {
+        uint32_t c0 = pixel, c1 = stream, c2 = j, c3 = 0u;
+        uint32_t k0 = seed, k1 = 0u;
+#pragma unroll
+        for (int r = 0; r < 10; ++r) {
+            const uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+            const uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+            const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0;
+            const uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
+            c1 = uint32_t(p1);
+            c3 = uint32_t(p0);
+            c0 = n0;
+            c2 = n2;
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        b0 = c0;
+        b1 = c1;
+        b2 = c2;
+        b3 = c3;
+        ++j;
+        used = 0;
+    }
+    __device__ float uniform() {
+        if (used == 4) block();
+        const uint32_t w = used == 0 ? b0 : used == 1 ? b1 : used == 2 ? b2 : b3;
+        ++used;
+        return (float(w >> 8) + 0.5f) * 0x1p-24f;
+    }
+};
+
+// lgamma(k + 1) by Stirling's series with upward recurrence below 7 (kernels/fx_random.h:19-41).
+__device__ inline float lgamma_k(float x) {
+    if (x == 1.0f || x == 2.0f) return 0.0f;
+    float x0 = x;
+    int n = 0;
+    if (x <= 7.0f) {
+        n = int(7.0f - x);
+        x0 = x + float(n);
+    }
+    const float x2 = 1.0f / (x0 * x0);
+    float g = 8.417508417508418e-04f;
+    g = g * x2 + -5.952380952380952e-04f;
+    g = g * x2 + 7.936507936507937e-04f;
+    g = g * x2 + -2.777777777777778e-03f;
+    g = g * x2 + 8.333333333333333e-02f;
+    float gl = ((g / x0 + 0.5f * spkm::logf_(6.283185307179586f)) + (x0 - 0.5f) * spkm::logf_(x0)) - x0;
+    if (x <= 7.0f) {
+        for (int i = 0; i < n; ++i) {
+            gl = gl - spkm::logf_(x0 - 1.0f);
+            x0 = x0 - 1.0f;
+        }
+    }
+    return gl;
+}
+
+// Poisson(lam): Hormann's transformed rejection (PTRS, 1993) for lam >= 10, at most 64 tries;
+// Knuth's multiplication method below, at most 512 steps (kernels/fx_random.h:48-83).
+__device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t seed) {
+    Philox g;
+    g.init(pixel, stream, seed);
+    if (lam >= 10.0f) {
+        const float slam = sqrtf(lam);
+        const float loglam = spkm::logf_(lam);
+        const float b = 0.931f + (2.53f * slam);
+        const float a = -0.059f + (0.02483f * b);
+        const float invalpha = 1.1239f + (1.1328f / (b - 3.4f));
+        const float vr = 0.9277f - (3.6224f / (b - 2.0f));
+        for (int it = 0; it < 64; ++it) {
+            const float U = g.uniform() - 0.5f;
+            const float V = g.uniform();
+            const float us = 0.5f - fabsf(U);
+            const float k = floorf(((((2.0f * a) / us + b) * U + lam) + 0.43f));
+            if (us >= 0.07f && V <= vr) return k;
+            if (k < 0.0f || (us < 0.013f && V > us)) continue;
+            const float Lh = ((spkm::logf_(V) + spkm::logf_(invalpha)) - spkm::logf_(a / (us * us) + b));
+            const float Rt = ((-lam + k * loglam) - lgamma_k(k + 1.0f));
+            if (Lh <= Rt) return k;
+        }
+        return floorf(lam);
+    }
+    const float E = spkm::expf_(-lam);
+    float X = 0.0f, prod = 1.0f;
+    for (int it = 0; it < 512; ++it) {
+        prod = prod * g.uniform();
+        if (prod > E) X = X + 1.0f;
+        else return X;
+    }
+    return X;
+}
+
+// q := clamp(d / D, 1e-6, 0.999999) (NaN stays); sat := 1 - q u 0.999999; lam := (n / sat) q;
+// draw := Poisson(lam) (D / n) sat (kernels/fx_random.h:91-99).
+__device__ inline float sublayer_draw(float d, float D, float n, float u, uint32_t pixel, uint32_t stream,
+                                      uint32_t seed) {
+    float q = d / D;
+    q = q < 1e-6f ? 1e-6f : (q > 0.999999f ? 0.999999f : q);
+    const float sat = 1.0f - ((q * u) * 0.999999f);
+    const float lam = (n / sat) * q;
+    return poisson(lam, pixel, stream, seed) * ((D / n) * sat);
+}
+
+// ---- kernels ---------------------------------------------------------------------------------
+
+// RGB -> layer exposure H (kernels/film.metal:189-227), then x f32(2^EC) (film.metal:176-180).
+__global__ __launch_bounds__(256) void k_upsample(const float* __restrict__ rgb, float* __restrict__ H,
+                                                  const float* __restrict__ table, UpsampleArgs q) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= q.n) return;
+    const float r = rgb[3 * size_t(i)], g = rgb[3 * size_t(i) + 1], b = rgb[3 * size_t(i) + 2];
+    const float X = (q.a[0] * r + q.a[1] * g) + q.a[2] * b;
+    const float Y = (q.a[3] * r + q.a[4] * g) + q.a[5] * b;
+    const float Z = (q.a[6] * r + q.a[7] * g) + q.a[8] * b;
+    const float bs = (X + Y) + Z;
+    const float d = bs > 1e-10f ? bs : 1e-10f;
+    const float x = X / d, y = Y / d, o = 1.0f - x;
+    const float t0 = fminf(fmaxf(o * o, 0.0f), 1.0f);
+    const float t1 = fminf(fmaxf(y / (o > 1e-10f ? o : 1e-10f), 0.0f), 1.0f);
+    const float bp = isnan(bs) ? 0.0f : bs;
+    int ia[4], ib[4];
+    float wa[4], wb[4];
+    mitchell_axis(t0, q.L, ia, wa);
+    mitchell_axis(t1, q.L, ib, wb);
+    float s = 0.0f, a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+    for (int ii = 0; ii < 4; ++ii) {
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const float w = wa[ii] * wb[jj];
+            const float* t = table + 3 * (size_t(ia[ii]) * q.L + size_t(ib[jj]));
+            s += w;
+            a0 += w * t[0];
+            a1 += w * t[1];
+            a2 += w * t[2];
+        }
+    }
+    if (s != 0.0f) {
+        a0 = a0 / s;
+        a1 = a1 / s;
+        a2 = a2 / s;
+    }
+    float h0 = bp * a0, h1 = bp * a1, h2 = bp * a2;
+    if (q.ec_active) {
+        h0 = h0 * q.ec_gain;
+        h1 = h1 * q.ec_gain;
+        h2 = h2 * q.ec_gain;
+    }
+    H[i] = h0;
+    H[size_t(q.n) + i] = h1;
+    H[2 * size_t(q.n) + i] = h2;
+}
+
+// Per pixel and channel: [raw := mix_a raw + mix_b acc] -> L := log10(max(raw, 0) + 1e-10) ->
+// D := F(L) -> [G := sum of the three sub-layer draws - D_min,c]
+// (halation.cpp:321-324 lincomb, film.metal:242-261, fx_grain.metal:39-97 SUM mode with shift).
+__global__ __launch_bounds__(256) void k_develop(const float* __restrict__ H, const float* __restrict__ acc,
+                                                 float* __restrict__ out, float* __restrict__ density,
+                                                 const float* __restrict__ curves, const float* __restrict__ gaxis,
+                                                 const float* __restrict__ ginv, const float* __restrict__ glayers,
+                                                 DevelopArgs q) {
+    const uint32_t n = q.w * q.h;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t K = q.K;
+    const float* x = curves;
+    const float* inv = curves + K;
+    const float* y = curves + 2 * K;
+#pragma unroll 1
+    for (uint32_t c = 0; c < 3; ++c) {
+        const size_t o = size_t(c) * n + i;
+        float v = H[o];
+        if (q.mix_active) v = q.mix_a[c] * v + q.mix_b[c] * acc[o];
+        const float L = spkm::log10f_(fmaxf(v, 0.0f) + 1e-10f);
+        const float D = curve_lookup(x, inv, y + c * K, K, L);
+        if (q.write_density) density[o] = D;
+        if (!q.grain_active) {
+            out[o] = D;
+            continue;
+        }
+        const float vv = q.positive ? -D : D;
+        const uint32_t GK = q.gk;
+        float d[3];
+        if (vv <= gaxis[c]) {
+            for (uint32_t l = 0; l < 3; ++l) d[l] = glayers[l * 3 + c];
+        } else if (vv >= gaxis[3 * (GK - 1) + c]) {
+            for (uint32_t l = 0; l < 3; ++l) d[l] = glayers[((GK - 1) * 3 + l) * 3 + c];
+        } else {
+            uint32_t lo = 0, hi = GK;
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo) / 2;
+                if (gaxis[3 * mid + c] <= vv) lo = mid + 1;
+                else hi = mid;
+            }
+            const uint32_t l0 = lo > 0 ? lo - 1 : 0;  // lo == 0 only for a NaN or a non-monotone axis
+            const float t = (vv - gaxis[3 * l0 + c]) * ginv[3 * l0 + c];
+            for (uint32_t l = 0; l < 3; ++l) {
+                const float y0 = glayers[(l0 * 3 + l) * 3 + c];
+                const float y1 = glayers[((l0 + 1) * 3 + l) * 3 + c];
+                d[l] = y0 + t * (y1 - y0);
+            }
+        }
+        float val[3];
+        for (uint32_t l = 0; l < 3; ++l) {
+            const uint32_t lc = l * 3 + c;
+            const float dl = d[l] + q.dmin_l[lc];
+            val[l] = sublayer_draw(dl, q.sat_d[lc], q.count[lc], q.uniformity[c], i, c + 10u * l, q.seed);
+        }
+        float G = ((0.0f + val[0]) + val[1]) + val[2];
+        G = G * 1.0f + (-q.dmin_c[c]);
+        out[o] = G;
+    }
+}
+
+// Self-test kernels: the generator and the sampler exactly as the grain draw uses them.
+__global__ void k_philox_test(const uint32_t* __restrict__ ctr, const uint32_t* __restrict__ key,
+                              uint32_t* __restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Philox g;
+    g.init(ctr[4 * i], ctr[4 * i + 1], key[2 * i]);
+    g.j = ctr[4 * i + 2];
+    // The counter's fourth word and the key's second word are fixed at 0 by the grain streams;
+    // the test covers arbitrary values through the generic rounds below.
+    uint32_t c0 = ctr[4 * i], c1 = ctr[4 * i + 1], c2 = ctr[4 * i + 2], c3 = ctr[4 * i + 3];
+    uint32_t k0 = key[2 * i], k1 = key[2 * i + 1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+        const uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+        const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
+        c1 = uint32_t(p1);
+        c3 = uint32_t(p0);
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    if (ctr[4 * i + 3] == 0u && key[2 * i + 1] == 0u) {
+        g.block();  // the grain stream's own block must agree with the generic rounds
+        if (g.b0 != c0 || g.b1 != c1 || g.b2 != c2 || g.b3 != c3) c0 = c1 = c2 = c3 = 0xDEADBEEFu;
+    }
+    out[4 * i] = c0;
+    out[4 * i + 1] = c1;
+    out[4 * i + 2] = c2;
+    out[4 * i + 3] = c3;
+}
+
+__global__ void k_poisson_test(const float* __restrict__ lam, float* __restrict__ out, uint32_t n, uint32_t stream,
+                               uint32_t seed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = poisson(lam[i], i, stream, seed);
+}
+
+}  // namespace
+
+void philox_test(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_philox_test, dim3((n + 255) / 256), dim3(256), 0, s, ctr, key, out, n);
+}
+
+void poisson_test(const float* lam, float* out, uint32_t n, uint32_t stream, uint32_t seed, hipStream_t s) {
+    hipLaunchKernelGGL(k_poisson_test, dim3((n + 255) / 256), dim3(256), 0, s, lam, out, n, stream, seed);
+}
+
+void upsample(const float* rgb, float* H, const float* table, const UpsampleArgs& a, hipStream_t s) {
+    const uint32_t blocks = (a.n + 255) / 256;
+    hipLaunchKernelGGL(k_upsample, dim3(blocks), dim3(256), 0, s, rgb, H, table, a);
+}
+
+void develop(const float* H, const float* acc, float* out, float* density, const float* curves, const float* gaxis,
+             const float* ginv, const float* glayers, const DevelopArgs& a, hipStream_t s) {
+    const uint32_t n = a.w * a.h;
+    hipLaunchKernelGGL(k_develop, dim3((n + 255) / 256), dim3(256), 0, s, H, acc, out, density, curves, gaxis, ginv,
+                       glayers, a);
+}
+
+}  // namespace spk::k

@@ -1,0 +1,105 @@
+// Host-side declarations of the HIP kernels' launchers and their by-value argument structs.
+//
+// Memory layout on the device (one frame of w x h):
+//   rgb input     interleaved float32 x 3 (the ABI's layout), 12 B/px
+//   film planes   planar float32: channel c of pixel i at p[c * w * h + i]  (exposure, mixture
+//                 accumulators, grain, the cached negative)
+//   print planes  planar float32 (the scanned RGB before and after the unsharp blur)
+//   output        RGBA16, row pitch `stride` pixels, 8 B/px
+// Planar planes make every line of every channel a contiguous (rows) or w-strided (columns)
+// float sequence, which is what the blur passes stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "spektralab_plan.h"
+
+namespace spk::k {
+
+inline constexpr int kFirMaxR = 64;                 // sigma < 21.5 at truncate 3
+inline constexpr int kFirMaxTaps = 2 * kFirMaxR + 1;
+
+// One channel of a FIR pass (taps -R..R).
+struct FirArgs {
+    uint32_t w, h, r;
+    uint32_t acc_mode;   // 0 NONE: out = v; 1 ZERO: out = 0 + wt v; 2 PLANE: out = acc + wt v
+    float weight;
+    float taps[kFirMaxTaps];
+};
+
+// One channel of a Young-van Vliet pass along lines of length n, processed in nch chunks of m.
+// The carries between chunks use powers of the companion matrix A (row-major 3x3, state
+// (w_i, w_{i-1}, w_{i-2})): pw[d] = A^(m 2^d), last = A^(n - (nch-1) m).
+struct IirArgs {
+    uint32_t w, h;       // plane size
+    uint32_t n, m, nch;  // line length, chunk length, chunk count
+    uint32_t acc_mode;   // column pass only (rows: NONE)
+    float weight;
+    double B, B1, B2, B3;
+    double pw[7][9];
+    double last[9];
+};
+
+struct UpsampleArgs {
+    uint32_t n, L;
+    float a[9];
+    float ec_gain;
+    uint32_t ec_active;
+};
+
+// The fused develop kernel: halation mix, log10, density curves, grain draw.
+struct DevelopArgs {
+    uint32_t w, h, K;
+    uint32_t mix_active;      // raw := mix_a raw + mix_b acc
+    float mix_a[3], mix_b[3];
+    uint32_t grain_active;    // SUM draw into the output, minus D_min,c
+    uint32_t write_density;   // also write the pre-grain density (grain_amount != 1)
+    uint32_t seed, positive, gk;
+    float dmin_l[9], sat_d[9], count[9], uniformity[3], dmin_c[3];
+};
+
+// Per-run constants of the enlarger (they follow the live print fields), passed by value.
+struct EnlargerArgs {
+    float eps[SPK_NL3], beta[SPK_NL], wt[SPK_NL3], g[3], o[3];
+    float k_gain;
+};
+
+// Pipeline constants of the print side, device-resident (uploaded with the pipeline's tables).
+struct PrintConsts {
+    uint32_t paper_k, pad0;
+    float scan_eps[SPK_NL3], scan_beta[SPK_NL], scan_wt[SPK_NL3];
+    float to_rgb[9];
+    float m_to_xyz[9], m_to_rgb[9], m16[9], m16_inv[9], d_rgb[3];
+    float f_l, n_bb, n_cb, n_, z, a_w, c, n_c, j_first, j_last, h_first, h_step, knee[3], lc[3], j_white;
+    uint32_t lc_active, gamut_active, rows, cols;
+    uint32_t cctf_active, cctf_mode;
+    float m_nid[9];
+    float unsharp_a;          // (1 + a), the unsharp combine's first weight; second is -a
+    float unsharp_b;
+    uint32_t unsharp_active;
+};
+
+// Launchers (stream-ordered, no synchronisation). Planes as documented above.
+void upsample(const float* rgb, float* H, const float* table, const UpsampleArgs& a, hipStream_t s);
+void fir_v(const float* in, float* out, const FirArgs& a, hipStream_t s);
+void fir_h(const float* in, float* out, const float* acc, const FirArgs& a, hipStream_t s);
+void iir_row(const float* in, float* out, const IirArgs& a, hipStream_t s);
+void iir_col(const float* in, float* out, const float* acc, const IirArgs& a, hipStream_t s);
+// Transposes a w x h single-channel plane into h x w; with acc_mode, the transposed value v is
+// combined as out = (mode ZERO ? 0 : acc) + weight v.
+void transpose(const float* in, float* out, const float* acc, uint32_t w, uint32_t h, uint32_t acc_mode, float weight,
+               hipStream_t s);
+// out_c = a_c x_c + b_c y_c on n-element single-channel planes (y == nullptr: y reads as 0).
+void lincomb(const float* x, const float* y, float* out, uint32_t n, float a, float b, hipStream_t s);
+void copy(const float* x, float* out, size_t n, hipStream_t s);
+void develop(const float* H, const float* acc, float* out, float* density, const float* curves, const float* gaxis,
+             const float* ginv, const float* glayers, const DevelopArgs& a, hipStream_t s);
+void print_pointwise(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea, const float* paper,
+                     const float* cmax, uint32_t w, uint32_t h, hipStream_t s);
+void philox_test(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n, hipStream_t s);
+void poisson_test(const float* lam, float* out, uint32_t n, uint32_t stream, uint32_t seed, hipStream_t s);
+void unsharp_encode(const float* rgb, const float* blurred, uint16_t* out, uint32_t stride, const PrintConsts* pc,
+                    uint32_t w, uint32_t h, hipStream_t s);
+
+}  // namespace spk::k

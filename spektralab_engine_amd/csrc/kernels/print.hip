@@ -1,0 +1,275 @@
+// Print-side kernels for gfx950.
+//
+//   k_print            per pixel, fused: enlarger spectral integral -> log10 exposure on paper
+//                      (kernels/print.metal:165-189), print gain (:192-198), paper curves
+//                      (:201-232), scan spectral integral (:165-189, SCAN mode), XYZ -> output RGB
+//                      (:249-258), CAM16-UCS gamut compression (:267-317).  The 81-bin loops run
+//                      over wavelength ascending with the tables read through the scalar cache
+//                      (uniform addresses) and the exponentials on the hardware exp2.
+//   k_unsharp_encode   rgb := (1 + a) rgb - a G(rgb) (src/fx/scanner.cpp:16-33) fused with the
+//                      near-identity matrix, transfer function and 16-bit encode (:350-366).
+// Float contraction is allowed here (the output is compared to the CPU oracle within 1 LSB).
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace spk::k {
+namespace {
+
+constexpr float kLog2Of10 = 3.321928094887362f;
+constexpr float kTiny = 1e-10f;
+
+__device__ inline float sgn(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
+
+struct F3 {
+    float x, y, z;
+};
+
+__device__ inline F3 mat_apply(const float* m, F3 v) {
+    return {(m[0] * v.x + m[1] * v.y) + m[2] * v.z, (m[3] * v.x + m[4] * v.y) + m[5] * v.z,
+            (m[6] * v.x + m[7] * v.y) + m[8] * v.z};
+}
+
+__device__ inline float knee(float d, float tau, float lim, float p) {
+    if (d <= tau) return d;
+    const float x = (d - tau) / (lim - tau);
+    return tau + (lim - tau) * (x / powf(1.0f + powf(x, p), 1.0f / p));
+}
+
+__device__ inline float post_adapt(float v, float f_l) {
+    const float f = powf((f_l * fabsf(v)) / 100.0f, 0.42f);
+    return ((400.0f * sgn(v)) * f) / (27.13f + f) + 0.1f;
+}
+
+struct Cam16Derived {
+    float cz, inv_cz, f_l_qrt, chroma_k, hundred_over_f_l;
+};
+
+__device__ inline F3 cam16_forward(F3 xyz, const PrintConsts& q, const Cam16Derived& k) {
+    const F3 x100{100.0f * xyz.x, 100.0f * xyz.y, 100.0f * xyz.z};
+    const F3 rgb = mat_apply(q.m16, x100);
+    const float ra = post_adapt(q.d_rgb[0] * rgb.x, q.f_l);
+    const float ga = post_adapt(q.d_rgb[1] * rgb.y, q.f_l);
+    const float ba = post_adapt(q.d_rgb[2] * rgb.z, q.f_l);
+    const float a = (ra - (12.0f * ga) / 11.0f) + ba / 11.0f;
+    const float b = ((ra + ga) - 2.0f * ba) / 9.0f;
+    const float h = atan2f(b, a);
+    const float et = 0.25f * (cosf(h + 2.0f) + 3.8f);
+    const float A = (((2.0f * ra + ga) + ba / 20.0f) - 0.305f) * q.n_bb;
+    const float ratio = A / q.a_w;
+    const float J = (100.0f * sgn(ratio)) * powf(fabsf(ratio), k.cz);
+    const float den = (ra + ga) + (21.0f * ba) / 20.0f;
+    const float t = den != 0.0f ? (((((50000.0f / 13.0f) * q.n_c) * q.n_cb) * et) * sqrtf(a * a + b * b)) / den : 0.0f;
+    const float C = t > 0.0f ? (powf(t, 0.9f) * sqrtf(fabsf(J) / 100.0f)) * k.chroma_k : 0.0f;
+    const float M = C * k.f_l_qrt;
+    const float Mp = logf(1.0f + 0.0228f * M) / 0.0228f;
+    const float Jp = (1.7f * J) / (1.0f + 0.007f * J);
+    return {Jp, Mp, h};
+}
+
+__device__ inline F3 cam16_inverse(float Jp, float Mp, float h, const PrintConsts& q, const Cam16Derived& k) {
+    const float M = (expf(0.0228f * Mp) - 1.0f) / 0.0228f;
+    const float J = Jp / (1.7f - 0.007f * Jp);
+    const float C = M / k.f_l_qrt;
+    const float A = (q.a_w * sgn(J)) * powf(fabsf(J / 100.0f), k.inv_cz);
+    const float sq = sqrtf(fabsf(J) / 100.0f);
+    const float t = (sq > 0.0f && C > 0.0f) ? powf(C / (sq * k.chroma_k), 1.0f / 0.9f) : 0.0f;
+    const float et = 0.25f * (cosf(h + 2.0f) + 3.8f);
+    const float p2 = A / q.n_bb + 0.305f;
+    const float p3 = 21.0f / 20.0f;
+    float a = 0.0f, b = 0.0f;
+    if (t != 0.0f) {
+        const float p1 = ((((50000.0f / 13.0f) * q.n_c) * q.n_cb) * et) / t;
+        const float sh = sinf(h), ch = cosf(h);
+        if (fabsf(sh) >= fabsf(ch)) {
+            const float p4 = p1 / sh;
+            b = ((p2 * (2.0f + p3)) * (460.0f / 1403.0f)) /
+                (((p4 + ((2.0f + p3) * (220.0f / 1403.0f)) * (ch / sh)) - 27.0f / 1403.0f) + p3 * (6300.0f / 1403.0f));
+            a = (b * ch) / sh;
+        } else {
+            const float p5 = p1 / ch;
+            a = ((p2 * (2.0f + p3)) * (460.0f / 1403.0f)) /
+                ((p5 + (2.0f + p3) * (220.0f / 1403.0f)) - (27.0f / 1403.0f - p3 * (6300.0f / 1403.0f)) * (sh / ch));
+            b = (a * sh) / ch;
+        }
+    }
+    const float xa[3] = {((460.0f * p2 + 451.0f * a) + 288.0f * b) / 1403.0f,
+                         ((460.0f * p2 - 891.0f * a) - 261.0f * b) / 1403.0f,
+                         ((460.0f * p2 - 220.0f * a) - 6300.0f * b) / 1403.0f};
+    float rc[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float v = xa[i] - 0.1f;
+        const float av = fabsf(v);
+        const float beta = av >= 400.0f ? 0.0f : (27.13f * av) / (400.0f - av);
+        rc[i] = ((k.hundred_over_f_l * sgn(v)) * powf(beta, 1.0f / 0.42f)) / q.d_rgb[i];
+    }
+    const F3 x100 = mat_apply(q.m16_inv, F3{rc[0], rc[1], rc[2]});
+    return {x100.x / 100.0f, x100.y / 100.0f, x100.z / 100.0f};
+}
+
+// Clamped paper-curve lookup with a right bisection; NaN passes (print.metal:201-232).
+__device__ inline float paper_lookup(const float* __restrict__ tab, uint32_t K, uint32_t c, float v) {
+    const float* x = tab;
+    const float* r = tab + K;
+    const float* y = tab + 2 * K;
+    if (isnan(v)) return v;
+    if (v <= x[0]) return y[c];
+    if (v >= x[K - 1]) return y[3 * (K - 1) + c];
+    uint32_t lo = 0, hi = K;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (x[mid] <= v) lo = mid + 1;
+        else hi = mid;
+    }
+    const uint32_t j = lo > 0 ? lo - 1 : 0;  // lo == 0 only on a non-monotone axis
+    const float y0 = y[3 * j + c];
+    return y0 + ((v - x[j]) * r[j]) * (y[3 * (j + 1) + c] - y0);
+}
+
+__device__ inline float cctf_encode(float v, uint32_t mode) {
+    switch (mode) {
+        case 0: return v <= 0.0031308f ? 12.92f * v : 1.055f * (sgn(v) * powf(fabsf(v), 0.41666666666666669f)) - 0.055f;
+        case 1: return v < 0.001953125f ? 16.0f * v : powf(v, 0.55555555555555558f);
+        case 2: return powf(v, 0.45470692717584371f);
+        case 3: return v < 0.018f ? 4.5f * v : 1.099f * (sgn(v) * powf(fabsf(v), 0.45f)) - 0.099f;
+        default: return v;
+    }
+}
+
+__device__ inline uint16_t encode16(float v) {
+    if (isnan(v)) return 0;
+    const float c = fminf(fmaxf(v, 0.0f), 1.0f);
+    return uint16_t(floorf(c * 65535.0f + 0.5f));
+}
+
+__global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, float* __restrict__ rgb,
+                                               const PrintConsts* __restrict__ pcp, EnlargerArgs e,
+                                               const float* __restrict__ paper, const float* __restrict__ cmax,
+                                               uint32_t w, uint32_t h) {
+    const PrintConsts& q = *pcp;
+    const uint32_t n = w * h;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float d0 = neg[i], d1 = neg[size_t(n) + i], d2 = neg[2 * size_t(n) + i];
+
+    // enlarger: log10 exposure on the paper, then the print gain
+    {
+        float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+#pragma unroll 9
+        for (uint32_t l = 0; l < SPK_NL; ++l) {
+            const float D = ((d0 * e.eps[3 * l] + d1 * e.eps[3 * l + 1]) + d2 * e.eps[3 * l + 2]) + e.beta[l];
+            const float T = __builtin_amdgcn_exp2f(-D * kLog2Of10);
+            r0 += T * e.wt[3 * l];
+            r1 += T * e.wt[3 * l + 1];
+            r2 += T * e.wt[3 * l + 2];
+        }
+        float l0 = log10f(fmaxf(r0 * e.g[0] + e.o[0], 0.0f) + kTiny);
+        float l1 = log10f(fmaxf(r1 * e.g[1] + e.o[1], 0.0f) + kTiny);
+        float l2 = log10f(fmaxf(r2 * e.g[2] + e.o[2], 0.0f) + kTiny);
+        l0 = log10f(fmaxf(exp2f(l0 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
+        l1 = log10f(fmaxf(exp2f(l1 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
+        l2 = log10f(fmaxf(exp2f(l2 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
+        d0 = paper_lookup(paper, q.paper_k, 0, l0);
+        d1 = paper_lookup(paper, q.paper_k, 1, l1);
+        d2 = paper_lookup(paper, q.paper_k, 2, l2);
+    }
+    // scan: XYZ of the print
+    F3 xyz;
+    {
+        float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+#pragma unroll 9
+        for (uint32_t l = 0; l < SPK_NL; ++l) {
+            const float D = ((d0 * q.scan_eps[3 * l] + d1 * q.scan_eps[3 * l + 1]) + d2 * q.scan_eps[3 * l + 2]) + q.scan_beta[l];
+            const float T = __builtin_amdgcn_exp2f(-D * kLog2Of10);
+            r0 += T * q.scan_wt[3 * l];
+            r1 += T * q.scan_wt[3 * l + 1];
+            r2 += T * q.scan_wt[3 * l + 2];
+        }
+        xyz = {fmaxf(r0, 0.0f) + kTiny, fmaxf(r1, 0.0f) + kTiny, fmaxf(r2, 0.0f) + kTiny};
+    }
+    F3 o = mat_apply(q.to_rgb, xyz);
+
+    if (q.gamut_active) {
+        const F3 gx = mat_apply(q.m_to_xyz, o);
+        Cam16Derived k;
+        k.cz = q.c * q.z;
+        k.inv_cz = 1.0f / k.cz;
+        k.f_l_qrt = powf(q.f_l, 0.25f);
+        k.chroma_k = powf(1.64f - powf(0.29f, q.n_), 0.73f);
+        k.hundred_over_f_l = 100.0f / q.f_l;
+        const F3 f = cam16_forward(gx, q, k);
+        float Jp = f.x;
+        const float Mp = f.y, hh = f.z;
+        if (q.lc_active) {
+            float Ln = Jp / q.j_white;
+            if (Ln > q.lc[0]) {
+                Ln = knee(Ln, q.lc[0], q.lc[1], q.lc[2]);
+                Jp = Ln * q.j_white;
+            }
+        }
+        const float Jc = fminf(fmaxf(Jp, q.j_first), q.j_last);
+        const float u = ((Jc - q.j_first) / (q.j_last - q.j_first)) * float(q.rows - 1);
+        const int l0 = min(max(int(floorf(u)), 0), int(q.rows) - 2);
+        const float fl = u - float(l0);
+        const float v = (hh - q.h_first) / q.h_step;
+        const float vf = floorf(v);
+        const int cols = int(q.cols);
+        int h0 = int(vf) % cols;
+        if (h0 < 0) h0 += cols;
+        const int h1 = h0 == cols - 1 ? 0 : h0 + 1;
+        const float fh = v - vf;
+        const float t00 = cmax[l0 * cols + h0], t01 = cmax[l0 * cols + h1];
+        const float t10 = cmax[(l0 + 1) * cols + h0], t11 = cmax[(l0 + 1) * cols + h1];
+        const float cm = (1.0f - fl) * ((1.0f - fh) * t00 + fh * t01) + fl * ((1.0f - fh) * t10 + fh * t11);
+        const float cs = fmaxf(cm, 1e-9f);
+        float dd = Mp / cs;
+        if (dd > 0.0f) dd = knee(dd, q.knee[0], q.knee[1], q.knee[2]);
+        const F3 back = cam16_inverse(Jp, dd * cs, hh, q, k);
+        o = mat_apply(q.m_to_rgb, back);
+    }
+    rgb[i] = o.x;
+    rgb[size_t(n) + i] = o.y;
+    rgb[2 * size_t(n) + i] = o.z;
+}
+
+__global__ __launch_bounds__(256) void k_unsharp_encode(const float* __restrict__ rgb, const float* __restrict__ blurred,
+                                                        uint16_t* __restrict__ out, uint32_t stride,
+                                                        const PrintConsts* __restrict__ pcp, uint32_t w, uint32_t h) {
+    const PrintConsts& q = *pcp;
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const uint32_t n = w * h;
+    const size_t i = size_t(y) * w + x;
+    F3 v{rgb[i], rgb[size_t(n) + i], rgb[2 * size_t(n) + i]};
+    if (q.unsharp_active) {
+        v.x = q.unsharp_a * v.x + q.unsharp_b * blurred[i];
+        v.y = q.unsharp_a * v.y + q.unsharp_b * blurred[size_t(n) + i];
+        v.z = q.unsharp_a * v.z + q.unsharp_b * blurred[2 * size_t(n) + i];
+    }
+    if (q.cctf_active) {
+        const F3 m = mat_apply(q.m_nid, v);
+        v = {cctf_encode(m.x, q.cctf_mode), cctf_encode(m.y, q.cctf_mode), cctf_encode(m.z, q.cctf_mode)};
+    }
+    ushort4 px;
+    px.x = encode16(v.x);
+    px.y = encode16(v.y);
+    px.z = encode16(v.z);
+    px.w = 65535;
+    reinterpret_cast<ushort4*>(out)[size_t(y) * stride + x] = px;
+}
+
+}  // namespace
+
+void print_pointwise(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea, const float* paper,
+                     const float* cmax, uint32_t w, uint32_t h, hipStream_t s) {
+    const uint32_t n = w * h;
+    hipLaunchKernelGGL(k_print, dim3((n + 255) / 256), dim3(256), 0, s, neg, rgb, pc, ea, paper, cmax, w, h);
+}
+
+void unsharp_encode(const float* rgb, const float* blurred, uint16_t* out, uint32_t stride, const PrintConsts* pc,
+                    uint32_t w, uint32_t h, hipStream_t s) {
+    hipLaunchKernelGGL(k_unsharp_encode, dim3((w + 255) / 256, h), dim3(256), 0, s, rgb, blurred, out, stride, pc, w, h);
+}
+
+}  // namespace spk::k

@@ -1,0 +1,135 @@
+// Reproducible float32 transcendentals for the film side and the grain sampler.
+//
+// The reference evaluates log10 (film log, kernels/film.metal:242-246) and the Poisson sampler's
+// log / exp (kernels/fx_random.h:19-83) with Metal's precise float32 library, whose exact results
+// are not reproducible off Apple GPUs.  Here each function is evaluated in IEEE binary64 with a
+// fixed sequence of +, -, *, / (no contraction: compiled with -ffp-contract=off on the host and
+// `#pragma clang fp contract(off)` on the device) and rounded once to binary32.  The host and the
+// GPU therefore return bit-identical floats, which is what makes the Poisson counts of the grain
+// stage bit-exact between the HIP kernels and the CPU oracle.
+//
+// Methods (published, coefficients derived here from their series):
+//   log:  x = m 2^e with m in [sqrt(1/2), sqrt(2)); ln x = e ln2 + 2 atanh(s), s = (m-1)/(m+1),
+//         |s| <= 0.1716; atanh by its odd Taylor series through s^17 (relative error < 1e-15).
+//   exp:  Cody-Waite reduction y = k ln2 + r, |r| <= ln2/2, ln2 split into a 32-bit-exact high part
+//         and a low part; exp(r) by its Taylor series through r^13 (relative error < 1e-15).
+// Accuracy of the float results: the binary64 value is within ~2e-15 relative of the true
+// value, so the rounded float is the correctly rounded result except within 2e-15 of a rounding
+// midpoint (checked against glibc in tests/test_oracle_math.py).
+#pragma once
+
+#include <stdint.h>
+#include <string.h>
+
+#if defined(__HIP__) || defined(__HIPCC__)
+#define SPKM_FN __host__ __device__ inline
+#else
+#define SPKM_FN inline
+#endif
+
+#if defined(__clang__)
+#define SPKM_NO_CONTRACT _Pragma("clang fp contract(off)")
+#else
+#define SPKM_NO_CONTRACT
+#endif
+
+namespace spkm {
+
+SPKM_FN uint64_t bits_of(double x) {
+    uint64_t u;
+    memcpy(&u, &x, 8);
+    return u;
+}
+SPKM_FN double from_bits(uint64_t u) {
+    double x;
+    memcpy(&x, &u, 8);
+    return x;
+}
+
+// ln(x) in binary64 for finite x > 0; NaN for x < 0 or NaN; -inf for 0; +inf for +inf.
+SPKM_FN double log_d(double x) {
+    SPKM_NO_CONTRACT
+    if (!(x > 0.0)) return x == 0.0 ? -__builtin_inf() : __builtin_nan("");
+    uint64_t u = bits_of(x);
+    int e = int((u >> 52) & 0x7FF);
+    if (e == 0x7FF) return x;  // +inf
+    int adj = 0;
+    if (e == 0) {  // subnormal: scale by 2^54
+        x = x * 18014398509481984.0;
+        u = bits_of(x);
+        e = int((u >> 52) & 0x7FF);
+        adj = -54;
+    }
+    // mantissa in [1, 2), then fold to [sqrt(1/2), sqrt(2))
+    double m = from_bits((u & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
+    int k = e - 1023 + adj;
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        k = k + 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    // 2 atanh(s) = 2 (s + s^3/3 + ... + s^17/17)
+    double p = 1.0 / 17.0;
+    p = p * s2 + 1.0 / 15.0;
+    p = p * s2 + 1.0 / 13.0;
+    p = p * s2 + 1.0 / 11.0;
+    p = p * s2 + 1.0 / 9.0;
+    p = p * s2 + 1.0 / 7.0;
+    p = p * s2 + 1.0 / 5.0;
+    p = p * s2 + 1.0 / 3.0;
+    const double at = s + (s * s2) * p;
+    const double ln2_hi = 0.693147180369123816490;   // 0x3FE62E42FEE00000: k * ln2_hi is exact for |k| < 2^11
+    const double ln2_lo = 1.90821492927058770002e-10;
+    const double kd = double(k);
+    return (kd * ln2_hi) + ((kd * ln2_lo) + 2.0 * at);
+}
+
+// exp(y) in binary64; overflow to +inf above ~709.78, 0 below ~-745.
+SPKM_FN double exp_d(double y) {
+    SPKM_NO_CONTRACT
+    if (y != y) return y;
+    if (y > 709.78) return __builtin_inf();
+    if (y < -745.2) return 0.0;
+    const double inv_ln2 = 1.44269504088896338700;
+    const double t = y * inv_ln2;
+    const double kd = t >= 0.0 ? double(int64_t(t + 0.5)) : -double(int64_t(-t + 0.5));
+    const double ln2_hi = 0.693147180369123816490;
+    const double ln2_lo = 1.90821492927058770002e-10;
+    const double r = (y - kd * ln2_hi) - kd * ln2_lo;
+    // Taylor series of exp(r) through r^13, Horner form
+    double p = 1.0 / 6227020800.0;
+    p = p * r + 1.0 / 479001600.0;
+    p = p * r + 1.0 / 39916800.0;
+    p = p * r + 1.0 / 3628800.0;
+    p = p * r + 1.0 / 362880.0;
+    p = p * r + 1.0 / 40320.0;
+    p = p * r + 1.0 / 5040.0;
+    p = p * r + 1.0 / 720.0;
+    p = p * r + 1.0 / 120.0;
+    p = p * r + 1.0 / 24.0;
+    p = p * r + 1.0 / 6.0;
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    // scale by 2^k in two steps so that subnormal results are formed by one final rounding
+    int k = int(kd);
+    if (k > 1000) {
+        p = p * from_bits(uint64_t(1023 + 1000) << 52);
+        k -= 1000;
+    } else if (k < -1000) {
+        p = p * from_bits(uint64_t(1023 - 1000) << 52);
+        k += 1000;
+    }
+    return p * from_bits(uint64_t(1023 + k) << 52);
+}
+
+// float32 results, rounded once from binary64.
+SPKM_FN float logf_(float x) { return float(log_d(double(x))); }
+SPKM_FN float expf_(float x) { return float(exp_d(double(x))); }
+SPKM_FN float log10f_(float x) {
+    SPKM_NO_CONTRACT
+    return float(log_d(double(x)) * 0.43429448190325182765);  // 1 / ln 10
+}
+
+}  // namespace spkm

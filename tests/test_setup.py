@@ -1,0 +1,146 @@
+"""The engine's float64 set-up (host logic, no GPU) through spkx_plan.
+
+Pinned by the reference's known answers (CAM16 constants, tests/test_print.cpp:79-104), by closed
+forms (scan normalisation, fog split, renormalised halation mix, FIR/IIR crossover) and by the
+CAM16 round trip and C_max in-gamut property (test_print.cpp:106-146) with an independent numpy
+statement of the published CAM16 equations (Li et al., Color Research and Application, 2017).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import spektralab_engine_amd as spk
+from conftest import CONFIG1, CONFIG2, CONFIG2_OVERRIDES, delta
+
+
+@pytest.fixture(scope="module")
+def plan1(resources):
+    return spk.host_plan(resources, CONFIG1, 512, 512)
+
+
+def test_cam16_prophoto_constants(oracle, plan1):
+    k = oracle.field(plan1, "cam64", 22)
+    assert abs(k[0] - 0.6839903845696502) < 1e-15   # F_L
+    assert abs(k[3] - 0.2) == 0.0                  # n
+    assert abs(k[1] - 1.0003040045593807) < 1e-15   # N_bb
+    assert abs(k[4] - 1.9272135954999579) < 1e-15   # z
+    assert abs(k[5] - 37.17349142011997) < 1e-13    # A_w
+    np.testing.assert_allclose(k[19:22], [1.0048858995308805, 0.9991651597854243, 1.18239029867048], atol=1e-15)
+    assert abs(k[18] - 100.0) < 1e-12               # J' of the adopted white
+    assert k[8] == 1.0 and k[9] == 110.0 and abs(k[10] + math.pi) < 1e-15 and abs(k[11] - 2 * math.pi / 720) < 1e-15
+    np.testing.assert_allclose(k[12:18], [0, 1, 6, 0.7, 1.0, 2.2])
+
+
+# ---- an independent numpy statement of CAM16-UCS (Li et al. 2017) for the ProPhoto white ----------
+M16 = np.array([[0.401288, 0.650173, -0.051461], [-0.250268, 1.204414, 0.045854], [-0.002079, 0.048952, 0.953127]])
+
+
+def cam16_consts(k):
+    return dict(F_L=k[0], N_bb=k[1], N_cb=k[2], n=k[3], z=k[4], A_w=k[5], c=k[6], N_c=k[7], D=np.array(k[19:22]))
+
+
+def cam16_forward(q, xyz):
+    rgb = M16 @ (100.0 * np.asarray(xyz))
+    v = q["D"] * rgb
+    f = (q["F_L"] * np.abs(v) / 100.0) ** 0.42
+    ra, ga, ba = 400.0 * np.sign(v) * f / (27.13 + f) + 0.1
+    a = ra - 12.0 * ga / 11.0 + ba / 11.0
+    b = (ra + ga - 2.0 * ba) / 9.0
+    h = math.atan2(b, a)
+    et = 0.25 * (math.cos(h + 2.0) + 3.8)
+    A = (2.0 * ra + ga + ba / 20.0 - 0.305) * q["N_bb"]
+    J = 100.0 * (A / q["A_w"]) ** (q["c"] * q["z"])
+    t = (50000.0 / 13.0 * q["N_c"] * q["N_cb"] * et * math.hypot(a, b)) / (ra + ga + 21.0 * ba / 20.0)
+    C = t ** 0.9 * math.sqrt(J / 100.0) * (1.64 - 0.29 ** q["n"]) ** 0.73
+    M = C * q["F_L"] ** 0.25
+    return 1.7 * J / (1.0 + 0.007 * J), math.log(1.0 + 0.0228 * M) / 0.0228, h
+
+
+def cam16_inverse(q, Jp, Mp, h):
+    M = (math.exp(0.0228 * Mp) - 1.0) / 0.0228
+    J = Jp / (1.7 - 0.007 * Jp)
+    C = M / q["F_L"] ** 0.25
+    t = (C / (math.sqrt(J / 100.0) * (1.64 - 0.29 ** q["n"]) ** 0.73)) ** (1.0 / 0.9)
+    et = 0.25 * (math.cos(h + 2.0) + 3.8)
+    A = q["A_w"] * (J / 100.0) ** (1.0 / (q["c"] * q["z"]))
+    p1 = 50000.0 / 13.0 * q["N_c"] * q["N_cb"] * et
+    p2 = A / q["N_bb"] + 0.305
+    gamma = 23.0 * p2 * t / (23.0 * p1 + 11.0 * t * math.cos(h) + 108.0 * t * math.sin(h))
+    a, b = gamma * math.cos(h), gamma * math.sin(h)
+    ra = (460.0 * p2 + 451.0 * a + 288.0 * b) / 1403.0
+    ga = (460.0 * p2 - 891.0 * a - 261.0 * b) / 1403.0
+    ba = (460.0 * p2 - 220.0 * a - 6300.0 * b) / 1403.0
+    out = []
+    for x, d in zip((ra, ga, ba), q["D"]):
+        v = x - 0.1
+        out.append(np.sign(v) * 100.0 / q["F_L"] * (27.13 * abs(v) / (400.0 - abs(v))) ** (1.0 / 0.42) / d)
+    return np.linalg.solve(M16, np.array(out)) / 100.0
+
+
+def test_cam16_round_trip(oracle, plan1):
+    q = cam16_consts(oracle.field(plan1, "cam64", 22))
+    for xyz in ([0.2, 0.18, 0.1], [0.05, 0.1, 0.4], [0.6, 0.5, 0.3], [0.9, 1.0, 0.8]):
+        Jp, Mp, h = cam16_forward(q, xyz)
+        np.testing.assert_allclose(cam16_inverse(q, Jp, Mp, h), xyz, atol=1e-10)
+
+
+def test_cmax_cells_in_gamut(oracle, plan1):
+    """A C_max cell is in gamut; one bisection step (x4) beyond it is not, for most cells."""
+    q = cam16_consts(oracle.field(plan1, "cam64", 22))
+    cmax = oracle.field(plan1, "cmax64", 64 * 720).reshape(64, 720)
+    to_rgb = oracle.field(plan1, "m_to_rgb64", 9).reshape(3, 3)
+    assert cmax.min() >= 0.0 and 20.0 < cmax.max() < 150.0
+    step = 150.0 / 2 ** 18
+
+    def in_gamut(Jp, M, h):
+        rgb = to_rgb @ cam16_inverse(q, Jp, M, h)
+        return bool(np.all((rgb >= -1e-6) & (rgb <= 1 + 1e-6)))
+
+    bad, outside = 0, 0
+    cells = [(i, j) for i in range(3, 64, 7) for j in range(0, 720, 37)]
+    for i, j in cells:
+        Jp, h = 1.0 + 109.0 * i / 63.0, -math.pi + 2.0 * math.pi * j / 720.0
+        m = cmax[i, j]
+        if m > 0 and not in_gamut(Jp, m, h):
+            bad += 1
+        if m < 149.0 and not in_gamut(Jp, m + 4 * step, h):
+            outside += 1
+    assert bad == 0
+    assert outside >= 0.9 * len(cells)
+
+
+def test_scan_normalised_to_white(oracle, plan1):
+    """wt_Y sums to 1: a zero-density, base-free print scans to Y = 1 (setup.cpp:236-256)."""
+    wt = oracle.field(plan1, "scan_wt64", 243).reshape(81, 3)
+    assert abs(wt[:, 1].sum() - 1.0) < 1e-12
+
+
+def test_grain_constants(resources, oracle):
+    plan = spk.host_plan(resources, CONFIG2, 600, 400, film_seed=7)
+    assert oracle.field(plan, "grain_active")[0] == 1 and oracle.field(plan, "grain_seed")[0] == 7
+    dmin_l = oracle.field(plan, "dmin_l", 9).reshape(3, 3)
+    np.testing.assert_allclose(dmin_l.sum(axis=0), 0.03, rtol=1e-6)   # the fog split by layer fractions
+    assert oracle.field(plan, "grain_final_blur")[0] == 0.65
+    count = oracle.field(plan, "count", 9).reshape(3, 3)
+    # count ~ p^2 f / (0.2 s_c s_l): the slow layer (s_l 0.5) holds 4x the fast one (s_l 2) per unit fraction
+    frac_ratio = (dmin_l[2] / dmin_l[0])
+    np.testing.assert_allclose(count[2] / count[0], 4.0 * frac_ratio, rtol=1e-5)
+
+
+def test_halation_mixture_override(resources, oracle):
+    plan = spk.host_plan(resources, CONFIG2, 600, 400, overrides=CONFIG2_OVERRIDES)
+    assert oracle.field(plan, "scatter_active")[0] == 0
+    assert oracle.field(plan, "bounce_active")[0] == 1 and oracle.field(plan, "n_bounce")[0] == 3
+    comps = oracle.field(plan, "bounce", 3 * 6).reshape(3, 2, 3)
+    np.testing.assert_array_equal(comps[:, 1, 0], [2.0, 8.0, 32.0])
+    np.testing.assert_array_equal(comps[:, 0, 0], [0.6, 0.3, 0.1])
+    A = np.float32(0.08) * np.float32(1.0)  # 'weak' antihalation, red
+    assert abs(oracle.field(plan, "mix_a", 3)[0] - 1.0 / (1.0 + float(A))) < 1e-15
+
+
+def test_refuses_stages_it_does_not_build(resources):
+    with pytest.raises(spk.SpkError, match="auto_exposure"):
+        spk.host_plan(resources, delta(auto_exposure=True), 64, 64)
+    with pytest.raises(spk.SpkError, match="glare"):
+        spk.host_plan(resources, delta(glare_active=True), 64, 64)
