@@ -4,6 +4,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "../kernels/kernels.h"
 #include "setup.h"
@@ -19,30 +22,49 @@ gpu::DevBuf upload(const void* src, size_t bytes, gpu::Pool& pool) {
     return b;
 }
 
-using M3 = std::array<double, 9>;
+// Powers A^k of the recursion's companion matrix A = [[B1 B2 B3] [1 0 0] [0 1 0]] for the chunk
+// carries: pw[d] = A^(m 2^d) (d < 7) and last = A^len_last.  For wide Gaussians the three poles
+// nearly coincide near 1 and A is far from normal, so repeated squaring loses most digits (at
+// sigma 100 the entries of A^288 come out ~5e-4 wrong in float64).  Column i of A^k is instead the
+// state after k zero-input steps of the recursion itself from unit state e_i, stepped in long
+// double: the same well-conditioned arithmetic as the serial recursion.
+struct IirPowers {
+    double pw[7][9];
+    double last[9];
+};
 
-M3 mat_mul(const M3& a, const M3& b) {
-    M3 r{};
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            double s = 0.0;
-            for (int k = 0; k < 3; ++k) s += a[3 * i + k] * b[3 * k + j];
-            r[3 * i + j] = s;
+IirPowers iir_powers(const setup::IirCoefficients& c, uint32_t m, uint32_t len_last) {
+    IirPowers r{};
+    const long double b1 = c.B1, b2 = c.B2, b3 = c.B3;
+    const uint64_t kmax = uint64_t(m) << 6;
+    for (int col = 0; col < 3; ++col) {
+        long double s0 = col == 0, s1 = col == 1, s2 = col == 2;
+        uint64_t next = m;
+        int d = 0;
+        for (uint64_t k = 1; k <= kmax; ++k) {
+            const long double t = b1 * s0 + b2 * s1 + b3 * s2;
+            s2 = s1;
+            s1 = s0;
+            s0 = t;
+            if (k == len_last) {
+                r.last[col] = double(s0);
+                r.last[3 + col] = double(s1);
+                r.last[6 + col] = double(s2);
+            }
+            if (k == next) {
+                r.pw[d][col] = double(s0);
+                r.pw[d][3 + col] = double(s1);
+                r.pw[d][6 + col] = double(s2);
+                ++d;
+                next <<= 1;
+            }
         }
-    return r;
-}
-
-M3 mat_pow(M3 a, uint64_t e) {
-    M3 r{1, 0, 0, 0, 1, 0, 0, 0, 1};
-    while (e) {
-        if (e & 1) r = mat_mul(r, a);
-        a = mat_mul(a, a);
-        e >>= 1;
     }
     return r;
 }
 
 // Recursive-pass arguments for lines of length n with nch chunks of m (last chunk n - (nch-1) m).
+// The powers depend only on (sigma, m, len_last) and are cached: a render reuses them every frame.
 k::IirArgs iir_args(double sigma, uint32_t w, uint32_t h, uint32_t n, uint32_t m, uint32_t nch) {
     const setup::IirCoefficients c = setup::iir_coefficients(sigma);
     k::IirArgs a{};
@@ -55,13 +77,25 @@ k::IirArgs iir_args(double sigma, uint32_t w, uint32_t h, uint32_t n, uint32_t m
     a.B1 = c.B1;
     a.B2 = c.B2;
     a.B3 = c.B3;
-    const M3 A{c.B1, c.B2, c.B3, 1, 0, 0, 0, 1, 0};
-    for (int d = 0; d < 7; ++d) {
-        const M3 p = mat_pow(A, uint64_t(m) << d);
-        std::memcpy(a.pw[d], p.data(), sizeof p);
+    const uint32_t len_last = n - (nch - 1) * m;
+    using Key = std::tuple<double, uint32_t, uint32_t>;
+    static std::mutex mu;
+    static std::map<Key, IirPowers> cache;
+    const Key key{sigma, m, len_last};
+    IirPowers p;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            p = it->second;
+        } else {
+            if (cache.size() > 4096) cache.clear();
+            p = iir_powers(c, m, len_last);
+            cache.emplace(key, p);
+        }
     }
-    const M3 l = mat_pow(A, uint64_t(n) - uint64_t(nch - 1) * m);
-    std::memcpy(a.last, l.data(), sizeof l);
+    std::memcpy(a.pw, p.pw, sizeof a.pw);
+    std::memcpy(a.last, p.last, sizeof a.last);
     return a;
 }
 

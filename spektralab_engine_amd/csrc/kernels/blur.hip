@@ -197,6 +197,14 @@ __global__ __launch_bounds__(128) void k_iir_row(const float* __restrict__ in, f
 
 constexpr uint32_t kColW = 16, kColChunks = 32;
 
+// Raw buffer access moves 32-bit patterns: reinterpret them, never convert.
+__device__ inline float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ inline void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+
 // A strip of 16 columns of one channel per block; thread (tx, j) holds rows [j MC, j MC + MC) of
 // column x in registers.  Every chunk has the compile-time length MC: the last chunk's rows past
 // the frame are padded with the replicated edge value u[n-1], which the backward recursion (unit
@@ -214,15 +222,20 @@ __global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, f
     const uint32_t start = j * uint32_t(MC);
     const uint32_t len = has ? min(uint32_t(MC), n - start) : 0u;
     const uint32_t last = nch - 1;
-    // Buffer addressing: one per-lane byte offset plus a wave-uniform row offset per element, so
-    // the MC loads need no per-element address registers; rows past the plane read as 0.
+    // Buffer addressing with a per-lane base offset.  Loads clamp the row to the chunk's last
+    // valid row (lanes past the frame edge read column w-1; lanes without a chunk read row 0), so
+    // every load stays inside the plane; stores use a wave-uniform row offset and only rows < n.
+    const uint32_t rowb = w * 4u;
     const uint32_t plane_bytes = q.w * q.h * 4u;
     const __amdgpu_buffer_rsrc_t rin =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(in), (short)0, plane_bytes, 0x00020000);
+    const uint32_t xc = colok ? x : w - 1u;
+    const uint32_t kmax = has ? len - 1u : 0u;
+    const uint32_t lbase = ((has ? start : 0u) * w + xc) * 4u;
     const uint32_t voff = (start * w + x) * 4u;
     float v[MC];
 #pragma unroll
-    for (int k = 0; k < MC; ++k) v[k] = __builtin_amdgcn_raw_buffer_load_b32(rin, voff, uint32_t(k) * w * 4u, 0);
+    for (int k = 0; k < MC; ++k) v[k] = buf_ld(rin, lbase + min(uint32_t(k), kmax) * rowb, 0u);
 
     // ---- forward: zero-state chunk pass, carry scan, true-state pass ----
     St z{0.0, 0.0, 0.0};
@@ -235,7 +248,7 @@ __global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, f
     // chunk live across the two passes.
 #pragma unroll
     for (int k = 0; k < MC; ++k) asm volatile("" : "+v"(v[k]));
-    const double x0 = double(__builtin_amdgcn_raw_buffer_load_b32(rin, x * 4u, 0, 0));
+    const double x0 = double(buf_ld(rin, xc * 4u, 0u));
     const St e0{x0, x0, x0};
     if (j == 0) z = st_add(mat_apply(q.pw[0], e0), z);
     V[j][tx][0] = z.a;
@@ -320,15 +333,15 @@ __global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, f
 #pragma unroll
         for (int k = 0; k < MC; ++k)
             if (uint32_t(k) < len) {
-                const float a = __builtin_amdgcn_raw_buffer_load_b32(racc, voff, uint32_t(k) * w * 4u, 0);
-                __builtin_amdgcn_raw_buffer_store_b32(a + q.weight * v[k], rout, voff, uint32_t(k) * w * 4u, 0);
+                const float a = buf_ld(racc, voff, uint32_t(k) * rowb);
+                buf_st(a + q.weight * v[k], rout, voff, uint32_t(k) * rowb);
             }
     } else {
 #pragma unroll
         for (int k = 0; k < MC; ++k)
             if (uint32_t(k) < len) {
                 const float r = q.acc_mode == 1u ? 0.0f + q.weight * v[k] : v[k];
-                __builtin_amdgcn_raw_buffer_store_b32(r, rout, voff, uint32_t(k) * w * 4u, 0);
+                buf_st(r, rout, voff, uint32_t(k) * rowb);
             }
     }
 }

@@ -122,12 +122,17 @@ def test_config2_small_matches_oracle(engine, oracle, crossover):
     img, plan, out, neg, neg_o, out_o = render_and_compare(engine, oracle, CONFIG2, 600, 400, ov, seed=1)
     assert oracle.field(plan, "grain_active")[0] == 1
     assert oracle.field(plan, "grain_seed")[0] == 0x4ABEA221  # the session's first film seed
-    mism = np.count_nonzero(neg != neg_o)
-    # the recursive passes may round 1 ulp apart (a chunked vs a serial float64 recursion); every
-    # other value, the Poisson counts included, is identical
-    assert mism <= neg.size * 1e-4, f"{mism} of {neg.size} negative values differ"
+    diff = np.abs(neg.astype(np.float64) - neg_o)
+    mism = int(np.count_nonzero(diff))
+    print(f"crossover {crossover}: {mism}/{neg.size} negative values differ, max {diff.max():.3g}, "
+          f"p99.99 {np.quantile(diff, 0.9999):.3g}")
+    # The recursive passes (sigma 32, and 8 under crossover 3) may round 1 ulp apart from the
+    # serial recursion; such a value then feeds the grain draw of its pixel.  Everything else,
+    # the Poisson counts included, is identical.
+    assert mism <= neg.size * 1e-3, f"{mism} of {neg.size} negative values differ"
+    # The print side is compared on the engine's own negative: +-1 LSB everywhere.
     d = np.abs(out.astype(np.int32) - out_o.astype(np.int32))
-    assert np.mean(d <= 1) >= 1.0 - 1e-4 and d.max() <= 1 or mism > 0
+    assert d.max() <= 1, f"max {d.max()} LSB"
 
 
 def test_reprint_reuses_negative_and_live_params(engine):
