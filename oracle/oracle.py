@@ -64,7 +64,8 @@ _DT = {"width": "u4", "height": "u4", "L": "u4", "K": "u4", "scatter_active": "u
        "bounce_active": "u4", "n_bounce": "u4", "grain_active": "u4", "grain_mode": "u4", "grain_seed": "u4",
        "grain_k": "u4", "printed": "u4", "paper_k": "u4", "cctf_active": "u4", "cctf_mode": "u4",
        "a": "f4", "ec_gain": "f4", "dmin_l": "f4", "sat_d": "f4", "count": "f4", "uniformity": "f4",
-       "dmin_c": "f4", "k_gain": "f4", "table": "f4", "curves": "f4", "paper": "f4", "cmax": "f4"}
+       "dmin_c": "f4", "k_gain": "f4", "table": "f4", "curves": "f4", "paper": "f4", "cmax": "f4",
+       "grain_axis": "f4", "grain_inv": "f4", "grain_layers": "f4"}
 
 
 def field(plan, name, count=1):

@@ -108,7 +108,8 @@ struct Philox {
 };
 
 // lgamma(k + 1) by Stirling's series with upward recurrence below 7 (kernels/fx_random.h:19-41).
-__device__ inline float lgamma_k(float x) {
+// hl2p = 0.5f * ln(f32(2 pi)), computed once per thread by the caller.
+__device__ inline float lgamma_k(float x, float hl2p) {
     if (x == 1.0f || x == 2.0f) return 0.0f;
     float x0 = x;
     int n = 0;
@@ -122,7 +123,7 @@ __device__ inline float lgamma_k(float x) {
     g = g * x2 + 7.936507936507937e-04f;
     g = g * x2 + -2.777777777777778e-03f;
     g = g * x2 + 8.333333333333333e-02f;
-    float gl = ((g / x0 + 0.5f * spkm::logf_(6.283185307179586f)) + (x0 - 0.5f) * spkm::logf_(x0)) - x0;
+    float gl = ((g / x0 + hl2p) + (x0 - 0.5f) * spkm::logf_(x0)) - x0;
     if (x <= 7.0f) {
         for (int i = 0; i < n; ++i) {
             gl = gl - spkm::logf_(x0 - 1.0f);
@@ -134,7 +135,7 @@ __device__ inline float lgamma_k(float x) {
 
 // Poisson(lam): Hormann's transformed rejection (PTRS, 1993) for lam >= 10, at most 64 tries;
 // Knuth's multiplication method below, at most 512 steps (kernels/fx_random.h:48-83).
-__device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t seed) {
+__device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t seed, float hl2p) {
     Philox g;
     g.init(pixel, stream, seed);
     if (lam >= 10.0f) {
@@ -144,6 +145,7 @@ __device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t se
         const float a = -0.059f + (0.02483f * b);
         const float invalpha = 1.1239f + (1.1328f / (b - 3.4f));
         const float vr = 0.9277f - (3.6224f / (b - 2.0f));
+        const float loginv = spkm::logf_(invalpha);
         for (int it = 0; it < 64; ++it) {
             const float U = g.uniform() - 0.5f;
             const float V = g.uniform();
@@ -151,8 +153,8 @@ __device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t se
             const float k = floorf(((((2.0f * a) / us + b) * U + lam) + 0.43f));
             if (us >= 0.07f && V <= vr) return k;
             if (k < 0.0f || (us < 0.013f && V > us)) continue;
-            const float Lh = ((spkm::logf_(V) + spkm::logf_(invalpha)) - spkm::logf_(a / (us * us) + b));
-            const float Rt = ((-lam + k * loglam) - lgamma_k(k + 1.0f));
+            const float Lh = ((spkm::logf_(V) + loginv) - spkm::logf_(a / (us * us) + b));
+            const float Rt = ((-lam + k * loglam) - lgamma_k(k + 1.0f, hl2p));
             if (Lh <= Rt) return k;
         }
         return floorf(lam);
@@ -170,12 +172,12 @@ __device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t se
 // q := clamp(d / D, 1e-6, 0.999999) (NaN stays); sat := 1 - q u 0.999999; lam := (n / sat) q;
 // draw := Poisson(lam) (D / n) sat (kernels/fx_random.h:91-99).
 __device__ inline float sublayer_draw(float d, float D, float n, float u, uint32_t pixel, uint32_t stream,
-                                      uint32_t seed) {
+                                      uint32_t seed, float hl2p) {
     float q = d / D;
     q = q < 1e-6f ? 1e-6f : (q > 0.999999f ? 0.999999f : q);
     const float sat = 1.0f - ((q * u) * 0.999999f);
     const float lam = (n / sat) * q;
-    return poisson(lam, pixel, stream, seed) * ((D / n) * sat);
+    return poisson(lam, pixel, stream, seed, hl2p) * ((D / n) * sat);
 }
 
 // ---- kernels ---------------------------------------------------------------------------------
@@ -243,6 +245,7 @@ __global__ __launch_bounds__(256) void k_develop(const float* __restrict__ H, co
     const float* x = curves;
     const float* inv = curves + K;
     const float* y = curves + 2 * K;
+    const float hl2p = 0.5f * spkm::logf_(6.283185307179586f);
 #pragma unroll 1
     for (uint32_t c = 0; c < 3; ++c) {
         const size_t o = size_t(c) * n + i;
@@ -281,7 +284,7 @@ __global__ __launch_bounds__(256) void k_develop(const float* __restrict__ H, co
         for (uint32_t l = 0; l < 3; ++l) {
             const uint32_t lc = l * 3 + c;
             const float dl = d[l] + q.dmin_l[lc];
-            val[l] = sublayer_draw(dl, q.sat_d[lc], q.count[lc], q.uniformity[c], i, c + 10u * l, q.seed);
+            val[l] = sublayer_draw(dl, q.sat_d[lc], q.count[lc], q.uniformity[c], i, c + 10u * l, q.seed, hl2p);
         }
         float G = ((0.0f + val[0]) + val[1]) + val[2];
         G = G * 1.0f + (-q.dmin_c[c]);
@@ -327,7 +330,7 @@ __global__ void k_poisson_test(const float* __restrict__ lam, float* __restrict_
                                uint32_t seed) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    out[i] = poisson(lam[i], i, stream, seed);
+    out[i] = poisson(lam[i], i, stream, seed, 0.5f * spkm::logf_(6.283185307179586f));
 }
 
 }  // namespace

@@ -9,17 +9,21 @@
 // stage bit-exact between the HIP kernels and the CPU oracle.
 //
 // Methods (published, coefficients derived here from their series):
-//   log:  x = m 2^e with m in [sqrt(1/2), sqrt(2)); ln x = e ln2 + 2 atanh(s), s = (m-1)/(m+1),
-//         |s| <= 0.1716; atanh by its odd Taylor series through s^17 (relative error < 1e-15).
-//   exp:  Cody-Waite reduction y = k ln2 + r, |r| <= ln2/2, ln2 split into a 32-bit-exact high part
-//         and a low part; exp(r) by its Taylor series through r^13 (relative error < 1e-15).
+//   log_d: x = m 2^e with m in [sqrt(1/2), sqrt(2)); ln x = e ln2 + 2 atanh(s), s = (m-1)/(m+1),
+//          |s| <= 0.1716; atanh by its odd Taylor series through s^17 (relative error < 1e-15).
+//          Used for log10 of the film log.
+//   logf_: the sampler's log, table-driven without division (described at the function).
+//   exp:   Cody-Waite reduction y = k ln2 + r, |r| <= ln2/2, ln2 split into a 32-bit-exact high
+//          part and a low part; exp(r) by its Taylor series through r^13 (relative error < 1e-15).
 // Accuracy of the float results: the binary64 value is within ~2e-15 relative of the true
 // value, so the rounded float is the correctly rounded result except within 2e-15 of a rounding
-// midpoint (checked against glibc in tests/test_oracle_math.py).
+// midpoint (checked against glibc in tests/test_oracle_kats.py).
 #pragma once
 
 #include <stdint.h>
 #include <string.h>
+
+#include "spk_log_table.h"
 
 #if defined(__HIP__) || defined(__HIPCC__)
 #define SPKM_FN __host__ __device__ inline
@@ -34,6 +38,14 @@
 #endif
 
 namespace spkm {
+
+// {1/c_i, -ln(1/c_i)} pairs of the sampler's logarithm; device code reads them from constant
+// memory with a per-lane index.
+#if defined(__HIP_DEVICE_COMPILE__)
+__constant__ static const double kLogTable[256] = {SPKM_LOG_TABLE_ENTRIES};
+#else
+static constexpr double kLogTable[256] = {SPKM_LOG_TABLE_ENTRIES};
+#endif
 
 SPKM_FN uint64_t bits_of(double x) {
     uint64_t u;
@@ -124,8 +136,34 @@ SPKM_FN double exp_d(double y) {
     return p * from_bits(uint64_t(1023 + k) << 52);
 }
 
+SPKM_FN double fma_d(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// ln(x) for the grain sampler, the function it calls most (up to 6 times per rejection step).
+// Table-driven (Tang's method, ACM TOMS 15(2), 1989) in binary64 with fused multiply-adds, so it
+// needs no division: x = 2^e m with m in [1, 2); the top 7 bits of m pick c_i = 1 + (i + 1/2)/128
+// with invc ~ 1/c_i and logc = -ln(invc) from spk_log_table.h; r = m invc - 1 (one rounding,
+// |r| <= 2^-8); ln x = (e ln2 + logc) + log1p(r), log1p by its Taylor series through r^5
+// (truncation < 2^-48 / 6).  Relative error before the final rounding < 2^-50: the float result
+// is the correctly rounded one except within that distance of a rounding midpoint.
+SPKM_FN float logf_(float x) {
+    SPKM_NO_CONTRACT
+    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_inff() : __builtin_nanf("");
+    if (x == __builtin_inff()) return x;
+    const uint64_t u = bits_of(double(x));  // float subnormals are normal in binary64
+    const int e = int((u >> 52) & 0x7FF) - 1023;
+    const uint32_t i = uint32_t(u >> 45) & 0x7Fu;
+    const double m = from_bits((u & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
+    const double invc = kLogTable[2 * i], logc = kLogTable[2 * i + 1];
+    const double r = fma_d(m, invc, -1.0);
+    double p = fma_d(r, 0.2, -0.25);
+    p = fma_d(r, p, 1.0 / 3.0);
+    p = fma_d(r, p, -0.5);
+    const double lp = fma_d(r * r, p, r);
+    const double t = fma_d(double(e), 0x1.62e42fefa39efp-1, logc);
+    return float(t + lp);
+}
+
 // float32 results, rounded once from binary64.
-SPKM_FN float logf_(float x) { return float(log_d(double(x))); }
 SPKM_FN float expf_(float x) { return float(exp_d(double(x))); }
 SPKM_FN float log10f_(float x) {
     SPKM_NO_CONTRACT

@@ -64,8 +64,25 @@ def test_reflect_folds(oracle):
     assert lib.oracle_reflect(5, 1) == 0
 
 
-@pytest.mark.parametrize("name,ref", [("oracle_logf", math.log), ("oracle_expf", math.exp),
-                                      ("oracle_log10f", math.log10)])
+def test_sampler_log_is_faithful(oracle):
+    """The grain sampler's table-driven log: within 1 ulp everywhere, and equal to the correctly
+    rounded result except on the rare inputs within 2^-50 of a rounding midpoint."""
+    fn = oracle.lib().oracle_logf
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([
+        (rng.uniform(1.0, 2.0, 200000) * 2.0 ** rng.integers(-149, 128, 200000)),
+        1.0 + rng.uniform(-1e-3, 1e-3, 50000),  # near 1, where e ln2 + logc cancels
+        rng.uniform(0.0, 1.0, 50000), rng.uniform(10.0, 700.0, 50000)]).astype(np.float32)
+    xs = xs[(xs > 0) & np.isfinite(xs)]
+    got = np.array([fn(float(x)) for x in xs], np.float32)
+    want = np.array([math.log(float(x)) for x in xs]).astype(np.float32)
+    ulp = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32))
+    assert ulp.max() <= 1 and np.mean(ulp != 0) < 1e-4, (ulp.max(), np.mean(ulp != 0))
+    assert fn(1.0) == 0.0 and fn(0.0) == -math.inf and math.isnan(fn(-1.0)) and math.isnan(fn(math.nan))
+    assert fn(math.inf) == math.inf and fn(1e-45) == np.float32(math.log(1.401298464324817e-45))
+
+
+@pytest.mark.parametrize("name,ref", [("oracle_expf", math.exp), ("oracle_log10f", math.log10)])
 def test_shared_math_correctly_rounded(oracle, name, ref):
     """The reproducible float transcendentals round like the correctly rounded result."""
     fn = getattr(oracle.lib(), name)
