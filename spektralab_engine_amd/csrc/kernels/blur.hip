@@ -1,8 +1,8 @@
 // Separable Gaussian passes on planar float32 planes for gfx950.
 //
-//   FIR vertical    register-tiled: each thread produces TY outputs down a column from TY + 2R
+//   FIR vertical    register-windowed: each thread produces 16 outputs down a column from 16 + 2R
 //                   coalesced loads (64 consecutive columns per wave)
-//   FIR horizontal  LDS-tiled: a block stages 256 + 2R samples of one row, one output per thread
+//   FIR horizontal  LDS-staged row segments of 1024 outputs, 16 adjacent outputs per thread
 //   recursive rows  Young-van Vliet 3rd order (reference kernels/fx_blur.metal:82-204) as a chunked
 //                   scan: the row lives in LDS; each of 128 threads runs the recursion over its
 //                   chunk from a zero state, the chunk-end states are combined by a Hillis-Steele
@@ -41,46 +41,114 @@ __device__ inline float fuse(uint32_t mode, const float* __restrict__ acc, size_
 }
 
 // ---- FIR ---------------------------------------------------------------------------------------
-constexpr int kTY = 8;
+// Both passes give each thread kT adjacent outputs and stream the 2R + kT inputs they need through a
+// register window of 2 kT values, unrolled kT taps at a time so that every window index is a
+// compile-time constant: output j takes tap i from window slot i + j.  Each output accumulates
+// its taps in ascending order from 0.  The tap loop's trip count is the only runtime bound, and
+// it is wave-uniform.
+constexpr int kT = 16;
 
+// Vertical: a wave is 64 adjacent columns of one run of kT rows; loads are coalesced rows, and
+// rows outside the frame (wave-uniform) are folded by half-sample reflection.
 __global__ __launch_bounds__(256) void k_fir_v(const float* __restrict__ in, float* __restrict__ out, FirArgs q) {
-    const int x = int(blockIdx.x * 64 + threadIdx.x);
-    const int y0 = int((blockIdx.y * blockDim.y + threadIdx.y) * kTY);
     const int w = int(q.w), h = int(q.h), R = int(q.r);
-    if (x >= w || y0 >= h) return;
-    float acc[kTY];
+    const int x = int(blockIdx.x * 64 + threadIdx.x);
+    const int y0 = int((blockIdx.y * 4 + threadIdx.y) * kT);
+    if (y0 >= h) return;
+    const int xc = x < w ? x : w - 1;
+    const float* col = in + xc;
+    auto load = [&](int t) {
+        int yy = y0 - R + t;
+        if (yy < 0 || yy >= h) yy = reflect_index(yy, h);
+        return col[size_t(yy) * w];
+    };
+    const int n = 2 * R + 1;
+    float acc[kT], win[2 * kT];
 #pragma unroll
-    for (int j = 0; j < kTY; ++j) acc[j] = 0.0f;
-    const int span = kTY + 2 * R;
-    for (int t = 0; t < span; ++t) {
-        const int yy = reflect_index(y0 - R + t, h);
-        const float v = in[size_t(yy) * w + x];
-#pragma unroll
-        for (int j = 0; j < kTY; ++j) {
-            const int i = t - j;
-            if (i >= 0 && i <= 2 * R) acc[j] = fmaf(q.taps[i], v, acc[j]);
-        }
+    for (int j = 0; j < kT; ++j) {
+        acc[j] = 0.0f;
+        win[j] = load(j);
     }
+    for (int i0 = 0; i0 < n; i0 += kT) {
 #pragma unroll
-    for (int j = 0; j < kTY; ++j)
+        for (int k = 0; k < kT; ++k) win[kT + k] = load(i0 + kT + k);
+#pragma unroll
+        for (int ii = 0; ii < kT; ++ii) {
+            if (i0 + ii < n) {
+                const float tap = q.taps[i0 + ii];
+#pragma unroll
+                for (int j = 0; j < kT; ++j) acc[j] = fmaf(tap, win[ii + j], acc[j]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kT; ++k) win[k] = win[kT + k];
+    }
+    if (x >= w) return;
+#pragma unroll
+    for (int j = 0; j < kT; ++j)
         if (y0 + j < h) out[size_t(y0 + j) * w + x] = acc[j];
 }
 
+// Horizontal: a wave is one row segment of 64 kT outputs staged in LDS with one pad word per 16
+// (lane l's window starts at word 17 l: conflict-free); results go back through the same LDS
+// so that the stores, and the accumulation plane's loads, are coalesced.
+constexpr int kHSeg = 64 * kT;
+
+__host__ __device__ inline int hpad(int t) { return t + (t >> 4); }
+
 __global__ __launch_bounds__(256) void k_fir_h(const float* __restrict__ in, float* __restrict__ out,
                                                const float* __restrict__ accp, FirArgs q) {
-    extern __shared__ float tile[];
-    const int w = int(q.w), R = int(q.r);
-    const int y = int(blockIdx.y);
-    const int x0 = int(blockIdx.x * 256);
-    const float* row = in + size_t(y) * w;
-    for (int t = int(threadIdx.x); t < 256 + 2 * R; t += 256) tile[t] = row[reflect_index(x0 - R + t, w)];
+    extern __shared__ float lds_h[];
+    const int w = int(q.w), h = int(q.h), R = int(q.r);
+    const int y = int(blockIdx.y * 4 + threadIdx.y);
+    const int x0 = int(blockIdx.x) * kHSeg;
+    const int span = kHSeg + 2 * R + kT;
+    const int stride = hpad(span) + 1;
+    float* tile = lds_h + threadIdx.y * stride;
+    const int lane = int(threadIdx.x);
+    const bool rowok = y < h;
+    if (rowok) {
+        const float* row = in + size_t(y) * w;
+        for (int t = lane; t < span; t += 64) {
+            int xx = x0 - R + t;
+            if (xx < 0 || xx >= w) xx = reflect_index(xx, w);
+            tile[hpad(t)] = row[xx];
+        }
+    }
     __syncthreads();
-    const int x = x0 + int(threadIdx.x);
-    if (x >= w) return;
-    float a = 0.0f;
-    for (int i = 0; i <= 2 * R; ++i) a = fmaf(q.taps[i], tile[int(threadIdx.x) + i], a);
-    const size_t o = size_t(y) * w + x;
-    out[o] = fuse(q.acc_mode, accp, o, q.weight, a);
+    const int n = 2 * R + 1;
+    const int base = lane * kT;
+    float acc[kT], win[2 * kT];
+#pragma unroll
+    for (int j = 0; j < kT; ++j) {
+        acc[j] = 0.0f;
+        win[j] = tile[hpad(base + j)];
+    }
+    for (int i0 = 0; i0 < n; i0 += kT) {
+#pragma unroll
+        for (int k = 0; k < kT; ++k) win[kT + k] = tile[hpad(base + i0 + kT + k)];
+#pragma unroll
+        for (int ii = 0; ii < kT; ++ii) {
+            if (i0 + ii < n) {
+                const float tap = q.taps[i0 + ii];
+#pragma unroll
+                for (int j = 0; j < kT; ++j) acc[j] = fmaf(tap, win[ii + j], acc[j]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kT; ++k) win[k] = win[kT + k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kT; ++j) tile[hpad(base + j)] = acc[j];
+    __syncthreads();
+    if (!rowok) return;
+    for (int t = lane; t < kHSeg; t += 64) {
+        const int x = x0 + t;
+        if (x >= w) break;
+        const size_t o = size_t(y) * w + x;
+        out[o] = fuse(q.acc_mode, accp, o, q.weight, tile[hpad(t)]);
+    }
 }
 
 // ---- recursive Gaussian ----------------------------------------------------------------------------
@@ -377,14 +445,15 @@ __global__ __launch_bounds__(256) void k_lincomb(const float* __restrict__ x, co
 }  // namespace
 
 void fir_v(const float* in, float* out, const FirArgs& a, hipStream_t s) {
-    const uint32_t rows_per_block = 4 * kTY;
+    const uint32_t rows_per_block = 4 * kT;
     hipLaunchKernelGGL(k_fir_v, dim3((a.w + 63) / 64, (a.h + rows_per_block - 1) / rows_per_block), dim3(64, 4), 0, s, in,
                        out, a);
 }
 
 void fir_h(const float* in, float* out, const float* acc, const FirArgs& a, hipStream_t s) {
-    const size_t lds = (256 + 2 * size_t(a.r)) * sizeof(float);
-    hipLaunchKernelGGL(k_fir_h, dim3((a.w + 255) / 256, a.h), dim3(256), lds, s, in, out, acc, a);
+    const int span = kHSeg + 2 * int(a.r) + kT;
+    const size_t lds = 4 * size_t(hpad(span) + 1) * sizeof(float);
+    hipLaunchKernelGGL(k_fir_h, dim3((a.w + kHSeg - 1) / kHSeg, (a.h + 3) / 4), dim3(64, 4), lds, s, in, out, acc, a);
 }
 
 void iir_row(const float* in, float* out, const IirArgs& a, hipStream_t s) {
