@@ -16,6 +16,9 @@ namespace spk::chain {
 
 namespace {
 
+// The kernels take transmittance as exp2 of a dot product: density tables arrive scaled by -log2(10).
+constexpr double kLog2Of10 = 3.32192809488736234787;
+
 gpu::DevBuf upload(const void* src, size_t bytes, gpu::Pool& pool) {
     gpu::DevBuf b = pool.acquire(bytes);
     gpu::check(hipMemcpy(b->ptr, src, bytes, hipMemcpyHostToDevice), "hipMemcpy(table)");
@@ -237,8 +240,8 @@ Tables upload_tables(const spk_plan& p, gpu::Pool& pool) {
 
     k::PrintConsts q{};
     q.paper_k = p.paper_k;
-    std::memcpy(q.scan_eps, p.scan_eps, sizeof q.scan_eps);
-    std::memcpy(q.scan_beta, p.scan_beta, sizeof q.scan_beta);
+    for (int i = 0; i < SPK_NL3; ++i) q.scan_eps[i] = num::f32(-p.scan_eps64[i] * kLog2Of10);
+    for (int i = 0; i < SPK_NL; ++i) q.scan_beta[i] = num::f32(-p.scan_beta64[i] * kLog2Of10);
     std::memcpy(q.scan_wt, p.scan_wt, sizeof q.scan_wt);
     std::memcpy(q.to_rgb, p.to_rgb, sizeof q.to_rgb);
     std::memcpy(q.m_to_xyz, p.m_to_xyz, sizeof q.m_to_xyz);
@@ -261,6 +264,14 @@ Tables upload_tables(const spk_plan& p, gpu::Pool& pool) {
     std::memcpy(q.knee, p.knee, sizeof q.knee);
     std::memcpy(q.lc, p.lc, sizeof q.lc);
     q.j_white = p.j_white;
+    {
+        const double f_l = p.cam64[0], n = p.cam64[3], z = p.cam64[4], c = p.cam64[6];
+        q.cz = num::f32(c * z);
+        q.inv_cz = num::f32(1.0 / (c * z));
+        q.f_l_qrt = num::f32(std::pow(f_l, 0.25));
+        q.chroma_k = num::f32(std::pow(1.64 - std::pow(0.29, n), 0.73));
+        q.hundred_over_f_l = num::f32(100.0 / f_l);
+    }
     q.lc_active = p.lc_active;
     q.gamut_active = p.gamut_active;
     q.rows = SPK_CMAX_ROWS;
@@ -373,8 +384,8 @@ void print(const spk_plan& p, const Tables& t, const float* negative, uint16_t* 
     const size_t np = size_t(w) * h;
     if (!p.printed) fail_internal("print: the scanned-film path is not built");
     k::EnlargerArgs ea{};
-    std::memcpy(ea.eps, p.enl_eps, sizeof ea.eps);
-    std::memcpy(ea.beta, p.enl_beta, sizeof ea.beta);
+    for (int i = 0; i < SPK_NL3; ++i) ea.eps[i] = num::f32(-p.enl_eps64[i] * kLog2Of10);
+    for (int i = 0; i < SPK_NL; ++i) ea.beta[i] = num::f32(-p.enl_beta64[i] * kLog2Of10);
     std::memcpy(ea.wt, p.enl_wt, sizeof ea.wt);
     std::memcpy(ea.g, p.enl_g, sizeof ea.g);
     std::memcpy(ea.o, p.enl_o, sizeof ea.o);

@@ -60,6 +60,7 @@ struct DevelopArgs {
 };
 
 // Per-run constants of the enlarger (they follow the live print fields), passed by value.
+// eps and beta (and the scan's) are pre-scaled by -log2(10): transmittance = exp2(eps . d + beta).
 struct EnlargerArgs {
     float eps[SPK_NL3], beta[SPK_NL], wt[SPK_NL3], g[3], o[3];
     float k_gain;
@@ -72,6 +73,7 @@ struct PrintConsts {
     float to_rgb[9];
     float m_to_xyz[9], m_to_rgb[9], m16[9], m16_inv[9], d_rgb[3];
     float f_l, n_bb, n_cb, n_, z, a_w, c, n_c, j_first, j_last, h_first, h_step, knee[3], lc[3], j_white;
+    float cz, inv_cz, f_l_qrt, chroma_k, hundred_over_f_l;  // derived on the host in float64
     uint32_t lc_active, gamut_active, rows, cols;
     uint32_t cctf_active, cctf_mode;
     float m_nid[9];

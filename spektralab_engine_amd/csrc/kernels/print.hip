@@ -5,7 +5,8 @@
 //                      (:201-232), scan spectral integral (:165-189, SCAN mode), XYZ -> output RGB
 //                      (:249-258), CAM16-UCS gamut compression (:267-317).  The 81-bin loops run
 //                      over wavelength ascending with the tables read through the scalar cache
-//                      (uniform addresses) and the exponentials on the hardware exp2.
+//                      (uniform addresses); the density tables arrive pre-scaled by -log2(10), so
+//                      each bin is one exp2 of a 3-term dot product on the hardware exp2.
 //   k_unsharp_encode   rgb := (1 + a) rgb - a G(rgb) (src/fx/scanner.cpp:16-33) fused with the
 //                      near-identity matrix, transfer function and 16-bit encode (:350-366).
 // Float contraction is allowed here (the output is compared to the CPU oracle within 1 LSB).
@@ -21,6 +22,18 @@ constexpr float kTiny = 1e-10f;
 
 __device__ inline float sgn(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
 
+// Hardware transcendentals: v_log_f32 (log2), v_exp_f32 (exp2), v_sin_f32 / v_cos_f32 (argument
+// in revolutions), each within a few ulp over the ranges met here; the libm versions' special-case
+// and wide-range reduction code is most of the kernel otherwise.  fpow is for x >= 0 (0 -> 0).
+__device__ inline float flog2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ inline float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ inline float fpow(float x, float y) { return fexp2(y * flog2(x)); }
+__device__ inline float flog10(float x) { return flog2(x) * 0.30102999566398120f; }
+__device__ inline float fln(float x) { return flog2(x) * 0.69314718055994531f; }
+__device__ inline float fexp(float x) { return fexp2(x * 1.4426950408889634f); }
+__device__ inline float fsin(float x) { return __builtin_amdgcn_sinf(x * 0.15915494309189535f); }
+__device__ inline float fcos(float x) { return __builtin_amdgcn_cosf(x * 0.15915494309189535f); }
+
 struct F3 {
     float x, y, z;
 };
@@ -33,11 +46,11 @@ __device__ inline F3 mat_apply(const float* m, F3 v) {
 __device__ inline float knee(float d, float tau, float lim, float p) {
     if (d <= tau) return d;
     const float x = (d - tau) / (lim - tau);
-    return tau + (lim - tau) * (x / powf(1.0f + powf(x, p), 1.0f / p));
+    return tau + (lim - tau) * (x / fpow(1.0f + fpow(x, p), 1.0f / p));
 }
 
 __device__ inline float post_adapt(float v, float f_l) {
-    const float f = powf((f_l * fabsf(v)) / 100.0f, 0.42f);
+    const float f = fpow((f_l * fabsf(v)) / 100.0f, 0.42f);
     return ((400.0f * sgn(v)) * f) / (27.13f + f) + 0.1f;
 }
 
@@ -54,33 +67,33 @@ __device__ inline F3 cam16_forward(F3 xyz, const PrintConsts& q, const Cam16Deri
     const float a = (ra - (12.0f * ga) / 11.0f) + ba / 11.0f;
     const float b = ((ra + ga) - 2.0f * ba) / 9.0f;
     const float h = atan2f(b, a);
-    const float et = 0.25f * (cosf(h + 2.0f) + 3.8f);
+    const float et = 0.25f * (fcos(h + 2.0f) + 3.8f);
     const float A = (((2.0f * ra + ga) + ba / 20.0f) - 0.305f) * q.n_bb;
     const float ratio = A / q.a_w;
-    const float J = (100.0f * sgn(ratio)) * powf(fabsf(ratio), k.cz);
+    const float J = (100.0f * sgn(ratio)) * fpow(fabsf(ratio), k.cz);
     const float den = (ra + ga) + (21.0f * ba) / 20.0f;
     const float t = den != 0.0f ? (((((50000.0f / 13.0f) * q.n_c) * q.n_cb) * et) * sqrtf(a * a + b * b)) / den : 0.0f;
-    const float C = t > 0.0f ? (powf(t, 0.9f) * sqrtf(fabsf(J) / 100.0f)) * k.chroma_k : 0.0f;
+    const float C = t > 0.0f ? (fpow(t, 0.9f) * sqrtf(fabsf(J) / 100.0f)) * k.chroma_k : 0.0f;
     const float M = C * k.f_l_qrt;
-    const float Mp = logf(1.0f + 0.0228f * M) / 0.0228f;
+    const float Mp = fln(1.0f + 0.0228f * M) * (1.0f / 0.0228f);
     const float Jp = (1.7f * J) / (1.0f + 0.007f * J);
     return {Jp, Mp, h};
 }
 
 __device__ inline F3 cam16_inverse(float Jp, float Mp, float h, const PrintConsts& q, const Cam16Derived& k) {
-    const float M = (expf(0.0228f * Mp) - 1.0f) / 0.0228f;
+    const float M = (fexp(0.0228f * Mp) - 1.0f) * (1.0f / 0.0228f);
     const float J = Jp / (1.7f - 0.007f * Jp);
     const float C = M / k.f_l_qrt;
-    const float A = (q.a_w * sgn(J)) * powf(fabsf(J / 100.0f), k.inv_cz);
+    const float A = (q.a_w * sgn(J)) * fpow(fabsf(J / 100.0f), k.inv_cz);
     const float sq = sqrtf(fabsf(J) / 100.0f);
-    const float t = (sq > 0.0f && C > 0.0f) ? powf(C / (sq * k.chroma_k), 1.0f / 0.9f) : 0.0f;
-    const float et = 0.25f * (cosf(h + 2.0f) + 3.8f);
+    const float t = (sq > 0.0f && C > 0.0f) ? fpow(C / (sq * k.chroma_k), 1.0f / 0.9f) : 0.0f;
+    const float et = 0.25f * (fcos(h + 2.0f) + 3.8f);
     const float p2 = A / q.n_bb + 0.305f;
     const float p3 = 21.0f / 20.0f;
     float a = 0.0f, b = 0.0f;
     if (t != 0.0f) {
         const float p1 = ((((50000.0f / 13.0f) * q.n_c) * q.n_cb) * et) / t;
-        const float sh = sinf(h), ch = cosf(h);
+        const float sh = fsin(h), ch = fcos(h);
         if (fabsf(sh) >= fabsf(ch)) {
             const float p4 = p1 / sh;
             b = ((p2 * (2.0f + p3)) * (460.0f / 1403.0f)) /
@@ -102,7 +115,7 @@ __device__ inline F3 cam16_inverse(float Jp, float Mp, float h, const PrintConst
         const float v = xa[i] - 0.1f;
         const float av = fabsf(v);
         const float beta = av >= 400.0f ? 0.0f : (27.13f * av) / (400.0f - av);
-        rc[i] = ((k.hundred_over_f_l * sgn(v)) * powf(beta, 1.0f / 0.42f)) / q.d_rgb[i];
+        rc[i] = ((k.hundred_over_f_l * sgn(v)) * fpow(beta, 1.0f / 0.42f)) / q.d_rgb[i];
     }
     const F3 x100 = mat_apply(q.m16_inv, F3{rc[0], rc[1], rc[2]});
     return {x100.x / 100.0f, x100.y / 100.0f, x100.z / 100.0f};
@@ -158,18 +171,17 @@ __global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, fl
         float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
 #pragma unroll 9
         for (uint32_t l = 0; l < SPK_NL; ++l) {
-            const float D = ((d0 * e.eps[3 * l] + d1 * e.eps[3 * l + 1]) + d2 * e.eps[3 * l + 2]) + e.beta[l];
-            const float T = __builtin_amdgcn_exp2f(-D * kLog2Of10);
+            const float T = fexp2(((d0 * e.eps[3 * l] + d1 * e.eps[3 * l + 1]) + d2 * e.eps[3 * l + 2]) + e.beta[l]);
             r0 += T * e.wt[3 * l];
             r1 += T * e.wt[3 * l + 1];
             r2 += T * e.wt[3 * l + 2];
         }
-        float l0 = log10f(fmaxf(r0 * e.g[0] + e.o[0], 0.0f) + kTiny);
-        float l1 = log10f(fmaxf(r1 * e.g[1] + e.o[1], 0.0f) + kTiny);
-        float l2 = log10f(fmaxf(r2 * e.g[2] + e.o[2], 0.0f) + kTiny);
-        l0 = log10f(fmaxf(exp2f(l0 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
-        l1 = log10f(fmaxf(exp2f(l1 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
-        l2 = log10f(fmaxf(exp2f(l2 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
+        float l0 = flog10(fmaxf(r0 * e.g[0] + e.o[0], 0.0f) + kTiny);
+        float l1 = flog10(fmaxf(r1 * e.g[1] + e.o[1], 0.0f) + kTiny);
+        float l2 = flog10(fmaxf(r2 * e.g[2] + e.o[2], 0.0f) + kTiny);
+        l0 = flog10(fmaxf(fexp2(l0 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
+        l1 = flog10(fmaxf(fexp2(l1 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
+        l2 = flog10(fmaxf(fexp2(l2 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
         d0 = paper_lookup(paper, q.paper_k, 0, l0);
         d1 = paper_lookup(paper, q.paper_k, 1, l1);
         d2 = paper_lookup(paper, q.paper_k, 2, l2);
@@ -180,8 +192,8 @@ __global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, fl
         float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
 #pragma unroll 9
         for (uint32_t l = 0; l < SPK_NL; ++l) {
-            const float D = ((d0 * q.scan_eps[3 * l] + d1 * q.scan_eps[3 * l + 1]) + d2 * q.scan_eps[3 * l + 2]) + q.scan_beta[l];
-            const float T = __builtin_amdgcn_exp2f(-D * kLog2Of10);
+            const float T = fexp2(((d0 * q.scan_eps[3 * l] + d1 * q.scan_eps[3 * l + 1]) + d2 * q.scan_eps[3 * l + 2]) +
+                                  q.scan_beta[l]);
             r0 += T * q.scan_wt[3 * l];
             r1 += T * q.scan_wt[3 * l + 1];
             r2 += T * q.scan_wt[3 * l + 2];
@@ -192,12 +204,7 @@ __global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, fl
 
     if (q.gamut_active) {
         const F3 gx = mat_apply(q.m_to_xyz, o);
-        Cam16Derived k;
-        k.cz = q.c * q.z;
-        k.inv_cz = 1.0f / k.cz;
-        k.f_l_qrt = powf(q.f_l, 0.25f);
-        k.chroma_k = powf(1.64f - powf(0.29f, q.n_), 0.73f);
-        k.hundred_over_f_l = 100.0f / q.f_l;
+        const Cam16Derived k{q.cz, q.inv_cz, q.f_l_qrt, q.chroma_k, q.hundred_over_f_l};
         const F3 f = cam16_forward(gx, q, k);
         float Jp = f.x;
         const float Mp = f.y, hh = f.z;
