@@ -57,6 +57,9 @@ def main():
                     help="cfg: FIR for sigma <= 16; ref: the reference's FIR for sigma < 3")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-sample", type=str, default="1536x1024")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one plain 1 GiB device-to-device copy (a reference dispatch of known "
+                         "traffic for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the render chain runs on a single "
@@ -81,6 +84,14 @@ def main():
     sess = eng.open(frame, delta)
     sess.set_overrides(overrides)
     out = torch.empty((H, W, 4), dtype=torch.int16, device="cuda")
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 1 GiB read, 1 GiB written")
+        del src, dst
     log("[bench] warm-up")
     for _ in range(args.warmup):
         sess.render_device(out.data_ptr(), W)

@@ -493,7 +493,11 @@ SPK_API spk_status spkx_poisson(spk_engine* engine, const float* lam, uint32_t n
         spk::gpu::DeviceScope ds(e.device());
         spk::gpu::DevBuf l = e.pool().acquire(size_t(n) * 4), o = e.pool().acquire(size_t(n) * 4);
         spk::gpu::check(hipMemcpy(l->ptr, lam, size_t(n) * 4, hipMemcpyHostToDevice), "hipMemcpy");
-        spk::k::poisson_test(static_cast<float*>(l->ptr), static_cast<float*>(o->ptr), n, stream, seed, nullptr);
+        const std::vector<float> lg = spk::chain::lgamma_table();
+        spk::gpu::DevBuf g = e.pool().acquire(lg.size() * 4);
+        spk::gpu::check(hipMemcpy(g->ptr, lg.data(), lg.size() * 4, hipMemcpyHostToDevice), "hipMemcpy");
+        spk::k::poisson_test(static_cast<float*>(l->ptr), static_cast<float*>(o->ptr), n, stream, seed,
+                             static_cast<float*>(g->ptr), nullptr);
         spk::gpu::check(hipGetLastError(), "poisson kernel");
         spk::gpu::check(hipMemcpy(out, o->ptr, size_t(n) * 4, hipMemcpyDeviceToHost), "hipMemcpy");
         return SPK_OK;

@@ -9,6 +9,7 @@
 #include <tuple>
 
 #include "../kernels/kernels.h"
+#include "../kernels/spk_math.h"
 #include "setup.h"
 #include "util.h"
 
@@ -192,6 +193,13 @@ void mixture(const float* in, float* out, const spk_mix_component* comps, uint32
 
 }  // namespace
 
+std::vector<float> lgamma_table() {
+    std::vector<float> t(spkm::kLgammaTable);
+    const float hl2p = spkm::half_log_2pi();
+    for (int k = 0; k < spkm::kLgammaTable; ++k) t[size_t(k)] = spkm::lgamma_k(float(k) + 1.0f, hl2p);
+    return t;
+}
+
 void Profiler::mark(const char* label, hipStream_t s) {
     hipEvent_t e;
     if (pool.empty()) {
@@ -235,6 +243,8 @@ Tables upload_tables(const spk_plan& p, gpu::Pool& pool) {
     t.gaxis = upload(p.grain_axis, 3 * gk * sizeof(float), pool);
     t.ginv = upload(p.grain_inv, 3 * gk * sizeof(float), pool);
     t.glayers = upload(p.grain_layers, 9 * gk * sizeof(float), pool);
+    const std::vector<float> lg = lgamma_table();
+    t.lgam = upload(lg.data(), lg.size() * sizeof(float), pool);
     t.paper = upload(p.paper, 5 * size_t(std::max<uint32_t>(p.paper_k, 1)) * sizeof(float), pool);
     t.cmax = upload(p.cmax, SPK_CMAX_N * sizeof(float), pool);
 
@@ -344,7 +354,7 @@ void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negati
     std::memcpy(da.dmin_c, p.dmin_c, sizeof da.dmin_c);
 
     if (!p.grain_active) {
-        k::develop(H, acc, negative, nullptr, gpu::as<float>(t.curves), nullptr, nullptr, nullptr, da, s);
+        k::develop(H, acc, negative, nullptr, gpu::as<float>(t.curves), nullptr, nullptr, nullptr, nullptr, da, s);
         mark("develop");
         return;
     }
@@ -357,7 +367,7 @@ void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negati
         keep.push_back(Db);
     }
     k::develop(H, acc, G, gpu::as<float>(Db), gpu::as<float>(t.curves), gpu::as<float>(t.gaxis), gpu::as<float>(t.ginv),
-               gpu::as<float>(t.glayers), da, s);
+               gpu::as<float>(t.glayers), gpu::as<float>(t.lgam), da, s);
     mark("develop");
     float* Gout = blend ? G : negative;
     if (p.grain_final_blur > 0.0) {

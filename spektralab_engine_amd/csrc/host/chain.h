@@ -35,8 +35,11 @@ struct Profiler {
 
 // Device copies of a plan's pipeline-constant tables and print constants (uploaded once per
 // pipeline; the per-run enlarger constants travel as kernel arguments).
+// lgamma(k + 1) for k < spkm::kLgammaTable, evaluated by the grain sampler's own function.
+std::vector<float> lgamma_table();
+
 struct Tables {
-    gpu::DevBuf table, curves, gaxis, ginv, glayers, paper, cmax, print_consts;
+    gpu::DevBuf table, curves, gaxis, ginv, glayers, lgam, paper, cmax, print_consts;
     uint32_t L = 0, K = 0;
 };
 Tables upload_tables(const spk_plan& p, gpu::Pool& pool);

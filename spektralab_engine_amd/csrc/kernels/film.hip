@@ -65,106 +65,96 @@ __device__ inline float curve_lookup(const float* __restrict__ x, const float* _
     return y[l] + t * (y[l + 1] - y[l]);
 }
 
-// ---- Philox4x32-10 (Salmon et al., SC 2011) and the per-(pixel, stream) word stream ---------------
-struct Philox {
-    uint32_t pixel, stream, seed, j, used;
-    uint32_t b0, b1, b2, b3;
-    __device__ void init(uint32_t p, uint32_t s, uint32_t sd) {
-        pixel = p;
-        stream = s;
-        seed = sd;
-        j = 0;
-        used = 4;
-    }
-    __device__ void block() {
-        uint32_t c0 = pixel, c1 = stream, c2 = j, c3 = 0u;
-        uint32_t k0 = seed, k1 = 0u;
+// ---- Philox4x32-10 (Salmon et al., SC 2011) ------------------------------------------------------
+// Ten rounds on counter (c0..c3) with key (k0, k1) (reference spk_common.h:23-34).
+__device__ inline uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
 #pragma unroll
-        for (int r = 0; r < 10; ++r) {
-            const uint64_t p0 = uint64_t(0xD2511F53u) * c0;
-            const uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
-            const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0;
-            const uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
-            c1 = uint32_t(p1);
-            c3 = uint32_t(p0);
-            c0 = n0;
-            c2 = n2;
-            k0 += 0x9E3779B9u;
-            k1 += 0xBB67AE85u;
-        }
-        b0 = c0;
-        b1 = c1;
-        b2 = c2;
-        b3 = c3;
-        ++j;
-        used = 0;
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+        const uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+        const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
+        c1 = uint32_t(p1);
+        c3 = uint32_t(p0);
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
     }
-    __device__ float uniform() {
-        if (used == 4) block();
-        const uint32_t w = used == 0 ? b0 : used == 1 ? b1 : used == 2 ? b2 : b3;
-        ++used;
-        return (float(w >> 8) + 0.5f) * 0x1p-24f;
-    }
-};
-
-// lgamma(k + 1) by Stirling's series with upward recurrence below 7 (kernels/fx_random.h:19-41).
-// hl2p = 0.5f * ln(f32(2 pi)), computed once per thread by the caller.
-__device__ inline float lgamma_k(float x, float hl2p) {
-    if (x == 1.0f || x == 2.0f) return 0.0f;
-    float x0 = x;
-    int n = 0;
-    if (x <= 7.0f) {
-        n = int(7.0f - x);
-        x0 = x + float(n);
-    }
-    const float x2 = 1.0f / (x0 * x0);
-    float g = 8.417508417508418e-04f;
-    g = g * x2 + -5.952380952380952e-04f;
-    g = g * x2 + 7.936507936507937e-04f;
-    g = g * x2 + -2.777777777777778e-03f;
-    g = g * x2 + 8.333333333333333e-02f;
-    float gl = ((g / x0 + hl2p) + (x0 - 0.5f) * spkm::logf_(x0)) - x0;
-    if (x <= 7.0f) {
-        for (int i = 0; i < n; ++i) {
-            gl = gl - spkm::logf_(x0 - 1.0f);
-            x0 = x0 - 1.0f;
-        }
-    }
-    return gl;
+    return make_uint4(c0, c1, c2, c3);
 }
 
-// Poisson(lam): Hormann's transformed rejection (PTRS, 1993) for lam >= 10, at most 64 tries;
-// Knuth's multiplication method below, at most 512 steps (kernels/fx_random.h:48-83).
-__device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t seed, float hl2p) {
-    Philox g;
-    g.init(pixel, stream, seed);
+// Block j of the (pixel, stream) generator: counter (pixel, stream, j, 0), key (seed, 0); its words
+// are consumed x, y, z, w before block j + 1 (reference spk_common.h:41-67).
+__device__ inline uint4 philox_block(uint32_t pixel, uint32_t stream, uint32_t j, uint32_t seed) {
+    return philox4x32_10(pixel, stream, j, 0u, seed, 0u);
+}
+
+// u = ((w >> 8) + 1/2) 2^-24 (reference spk_common.h:37); 1.0f is reachable.
+__device__ inline float u01(uint32_t w) { return (float(w >> 8) + 0.5f) * 0x1p-24f; }
+
+// Hormann's PTRS constants for one lam >= 10 (reference kernels/fx_random.h:52-58).
+struct Ptrs {
+    float lam, loglam, b, a, vr, loginv;
+};
+
+// One PTRS try from the uniform pair (u, v): 1 = accept *k, 0 = next try.  lgamma(k + 1) comes
+// from the LDS table for k < 1024 (the table holds spkm::lgamma_k exactly).
+__device__ inline bool ptrs_try(const Ptrs& P, float u, float v, const float* __restrict__ lgam, float hl2p,
+                                float* k_out) {
+    const float U = u - 0.5f;
+    const float V = v;
+    const float us = 0.5f - fabsf(U);
+    const float k = floorf(((((2.0f * P.a) / us + P.b) * U + P.lam) + 0.43f));
+    *k_out = k;
+    if (us >= 0.07f && V <= P.vr) return true;
+    if (k < 0.0f || (us < 0.013f && V > us)) return false;
+    const float Lh = ((spkm::logf_(V) + P.loginv) - spkm::logf_(P.a / (us * us) + P.b));
+    const float lg = k < float(spkm::kLgammaTable) ? lgam[int(k)] : spkm::lgamma_k(k + 1.0f, hl2p);
+    const float Rt = ((-P.lam + k * P.loglam) - lg);
+    return Lh <= Rt;
+}
+
+// Poisson(lam) of one (pixel, stream) generator (reference kernels/fx_random.h:48-83):
+// lam >= 10: Hormann's transformed rejection (PTRS, 1993), at most 64 tries, two per Philox
+// block, then floor(lam); lam < 10: multiply uniforms until the product falls to exp(-lam), at
+// most 512 steps, four per block.  NaN lam takes the second branch and returns 0.
+__device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t seed, float hl2p,
+                         const float* __restrict__ lgam) {
     if (lam >= 10.0f) {
+        Ptrs P;
+        P.lam = lam;
         const float slam = sqrtf(lam);
-        const float loglam = spkm::logf_(lam);
-        const float b = 0.931f + (2.53f * slam);
-        const float a = -0.059f + (0.02483f * b);
-        const float invalpha = 1.1239f + (1.1328f / (b - 3.4f));
-        const float vr = 0.9277f - (3.6224f / (b - 2.0f));
-        const float loginv = spkm::logf_(invalpha);
-        for (int it = 0; it < 64; ++it) {
-            const float U = g.uniform() - 0.5f;
-            const float V = g.uniform();
-            const float us = 0.5f - fabsf(U);
-            const float k = floorf(((((2.0f * a) / us + b) * U + lam) + 0.43f));
-            if (us >= 0.07f && V <= vr) return k;
-            if (k < 0.0f || (us < 0.013f && V > us)) continue;
-            const float Lh = ((spkm::logf_(V) + loginv) - spkm::logf_(a / (us * us) + b));
-            const float Rt = ((-lam + k * loglam) - lgamma_k(k + 1.0f, hl2p));
-            if (Lh <= Rt) return k;
+        P.loglam = spkm::logf_(lam);
+        P.b = 0.931f + (2.53f * slam);
+        P.a = -0.059f + (0.02483f * P.b);
+        const float invalpha = 1.1239f + (1.1328f / (P.b - 3.4f));
+        P.vr = 0.9277f - (3.6224f / (P.b - 2.0f));
+        P.loginv = spkm::logf_(invalpha);
+        for (uint32_t j = 0; j < 32; ++j) {
+            const uint4 w = philox_block(pixel, stream, j, seed);
+            float k;
+            if (ptrs_try(P, u01(w.x), u01(w.y), lgam, hl2p, &k)) return k;
+            if (ptrs_try(P, u01(w.z), u01(w.w), lgam, hl2p, &k)) return k;
         }
         return floorf(lam);
     }
     const float E = spkm::expf_(-lam);
     float X = 0.0f, prod = 1.0f;
-    for (int it = 0; it < 512; ++it) {
-        prod = prod * g.uniform();
-        if (prod > E) X = X + 1.0f;
-        else return X;
+    for (uint32_t j = 0; j < 128; ++j) {
+        const uint4 w = philox_block(pixel, stream, j, seed);
+        prod = prod * u01(w.x);
+        if (!(prod > E)) return X;
+        X = X + 1.0f;
+        prod = prod * u01(w.y);
+        if (!(prod > E)) return X;
+        X = X + 1.0f;
+        prod = prod * u01(w.z);
+        if (!(prod > E)) return X;
+        X = X + 1.0f;
+        prod = prod * u01(w.w);
+        if (!(prod > E)) return X;
+        X = X + 1.0f;
     }
     return X;
 }
@@ -172,12 +162,12 @@ __device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t se
 // q := clamp(d / D, 1e-6, 0.999999) (NaN stays); sat := 1 - q u 0.999999; lam := (n / sat) q;
 // draw := Poisson(lam) (D / n) sat (kernels/fx_random.h:91-99).
 __device__ inline float sublayer_draw(float d, float D, float n, float u, uint32_t pixel, uint32_t stream,
-                                      uint32_t seed, float hl2p) {
+                                      uint32_t seed, float hl2p, const float* __restrict__ lgam) {
     float q = d / D;
     q = q < 1e-6f ? 1e-6f : (q > 0.999999f ? 0.999999f : q);
     const float sat = 1.0f - ((q * u) * 0.999999f);
     const float lam = (n / sat) * q;
-    return poisson(lam, pixel, stream, seed, hl2p) * ((D / n) * sat);
+    return poisson(lam, pixel, stream, seed, hl2p, lgam) * ((D / n) * sat);
 }
 
 // ---- kernels ---------------------------------------------------------------------------------
@@ -237,15 +227,20 @@ __global__ __launch_bounds__(256) void k_develop(const float* __restrict__ H, co
                                                  float* __restrict__ out, float* __restrict__ density,
                                                  const float* __restrict__ curves, const float* __restrict__ gaxis,
                                                  const float* __restrict__ ginv, const float* __restrict__ glayers,
-                                                 DevelopArgs q) {
+                                                 const float* __restrict__ lgam, DevelopArgs q) {
+    __shared__ float s_lgam[spkm::kLgammaTable];
     const uint32_t n = q.w * q.h;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q.grain_active) {
+        for (uint32_t t = threadIdx.x; t < uint32_t(spkm::kLgammaTable); t += blockDim.x) s_lgam[t] = lgam[t];
+        __syncthreads();
+    }
     if (i >= n) return;
     const uint32_t K = q.K;
     const float* x = curves;
     const float* inv = curves + K;
     const float* y = curves + 2 * K;
-    const float hl2p = 0.5f * spkm::logf_(6.283185307179586f);
+    const float hl2p = spkm::half_log_2pi();
 #pragma unroll 1
     for (uint32_t c = 0; c < 3; ++c) {
         const size_t o = size_t(c) * n + i;
@@ -284,7 +279,8 @@ __global__ __launch_bounds__(256) void k_develop(const float* __restrict__ H, co
         for (uint32_t l = 0; l < 3; ++l) {
             const uint32_t lc = l * 3 + c;
             const float dl = d[l] + q.dmin_l[lc];
-            val[l] = sublayer_draw(dl, q.sat_d[lc], q.count[lc], q.uniformity[c], i, c + 10u * l, q.seed, hl2p);
+            val[l] = sublayer_draw(dl, q.sat_d[lc], q.count[lc], q.uniformity[c], i, c + 10u * l, q.seed, hl2p,
+                                   s_lgam);
         }
         float G = ((0.0f + val[0]) + val[1]) + val[2];
         G = G * 1.0f + (-q.dmin_c[c]);
@@ -297,40 +293,21 @@ __global__ void k_philox_test(const uint32_t* __restrict__ ctr, const uint32_t* 
                               uint32_t* __restrict__ out, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Philox g;
-    g.init(ctr[4 * i], ctr[4 * i + 1], key[2 * i]);
-    g.j = ctr[4 * i + 2];
-    // The counter's fourth word and the key's second word are fixed at 0 by the grain streams;
-    // the test covers arbitrary values through the generic rounds below.
-    uint32_t c0 = ctr[4 * i], c1 = ctr[4 * i + 1], c2 = ctr[4 * i + 2], c3 = ctr[4 * i + 3];
-    uint32_t k0 = key[2 * i], k1 = key[2 * i + 1];
-    for (int r = 0; r < 10; ++r) {
-        const uint64_t p0 = uint64_t(0xD2511F53u) * c0;
-        const uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
-        const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
-        c1 = uint32_t(p1);
-        c3 = uint32_t(p0);
-        c0 = n0;
-        c2 = n2;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    if (ctr[4 * i + 3] == 0u && key[2 * i + 1] == 0u) {
-        g.block();  // the grain stream's own block must agree with the generic rounds
-        if (g.b0 != c0 || g.b1 != c1 || g.b2 != c2 || g.b3 != c3) c0 = c1 = c2 = c3 = 0xDEADBEEFu;
-    }
-    out[4 * i] = c0;
-    out[4 * i + 1] = c1;
-    out[4 * i + 2] = c2;
-    out[4 * i + 3] = c3;
+    const uint4 r = philox4x32_10(ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3], key[2 * i], key[2 * i + 1]);
+    out[4 * i] = r.x;
+    out[4 * i + 1] = r.y;
+    out[4 * i + 2] = r.z;
+    out[4 * i + 3] = r.w;
 }
 
-__global__ void k_poisson_test(const float* __restrict__ lam, float* __restrict__ out, uint32_t n, uint32_t stream,
-                               uint32_t seed) {
+__global__ __launch_bounds__(256) void k_poisson_test(const float* __restrict__ lam, float* __restrict__ out, uint32_t n,
+                                                      uint32_t stream, uint32_t seed, const float* __restrict__ lgam) {
+    __shared__ float s_lgam[spkm::kLgammaTable];
+    for (uint32_t t = threadIdx.x; t < uint32_t(spkm::kLgammaTable); t += blockDim.x) s_lgam[t] = lgam[t];
+    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    out[i] = poisson(lam[i], i, stream, seed, 0.5f * spkm::logf_(6.283185307179586f));
+    out[i] = poisson(lam[i], i, stream, seed, spkm::half_log_2pi(), s_lgam);
 }
 
 }  // namespace
@@ -339,8 +316,9 @@ void philox_test(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32
     hipLaunchKernelGGL(k_philox_test, dim3((n + 255) / 256), dim3(256), 0, s, ctr, key, out, n);
 }
 
-void poisson_test(const float* lam, float* out, uint32_t n, uint32_t stream, uint32_t seed, hipStream_t s) {
-    hipLaunchKernelGGL(k_poisson_test, dim3((n + 255) / 256), dim3(256), 0, s, lam, out, n, stream, seed);
+void poisson_test(const float* lam, float* out, uint32_t n, uint32_t stream, uint32_t seed, const float* lgam,
+                  hipStream_t s) {
+    hipLaunchKernelGGL(k_poisson_test, dim3((n + 255) / 256), dim3(256), 0, s, lam, out, n, stream, seed, lgam);
 }
 
 void upsample(const float* rgb, float* H, const float* table, const UpsampleArgs& a, hipStream_t s) {
@@ -349,10 +327,10 @@ void upsample(const float* rgb, float* H, const float* table, const UpsampleArgs
 }
 
 void develop(const float* H, const float* acc, float* out, float* density, const float* curves, const float* gaxis,
-             const float* ginv, const float* glayers, const DevelopArgs& a, hipStream_t s) {
+             const float* ginv, const float* glayers, const float* lgam, const DevelopArgs& a, hipStream_t s) {
     const uint32_t n = a.w * a.h;
     hipLaunchKernelGGL(k_develop, dim3((n + 255) / 256), dim3(256), 0, s, H, acc, out, density, curves, gaxis, ginv,
-                       glayers, a);
+                       glayers, lgam, a);
 }
 
 }  // namespace spk::k

@@ -96,11 +96,12 @@ void transpose(const float* in, float* out, const float* acc, uint32_t w, uint32
 void lincomb(const float* x, const float* y, float* out, uint32_t n, float a, float b, hipStream_t s);
 void copy(const float* x, float* out, size_t n, hipStream_t s);
 void develop(const float* H, const float* acc, float* out, float* density, const float* curves, const float* gaxis,
-             const float* ginv, const float* glayers, const DevelopArgs& a, hipStream_t s);
+             const float* ginv, const float* glayers, const float* lgam, const DevelopArgs& a, hipStream_t s);
 void print_pointwise(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea, const float* paper,
                      const float* cmax, uint32_t w, uint32_t h, hipStream_t s);
 void philox_test(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n, hipStream_t s);
-void poisson_test(const float* lam, float* out, uint32_t n, uint32_t stream, uint32_t seed, hipStream_t s);
+void poisson_test(const float* lam, float* out, uint32_t n, uint32_t stream, uint32_t seed, const float* lgam,
+                  hipStream_t s);
 void unsharp_encode(const float* rgb, const float* blurred, uint16_t* out, uint32_t stride, const PrintConsts* pc,
                     uint32_t w, uint32_t h, hipStream_t s);
 

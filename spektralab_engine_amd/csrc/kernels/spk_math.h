@@ -165,6 +165,37 @@ SPKM_FN float logf_(float x) {
 
 // float32 results, rounded once from binary64.
 SPKM_FN float expf_(float x) { return float(exp_d(double(x))); }
+
+// lgamma(x) for x = k + 1 as the grain sampler evaluates it (reference kernels/fx_random.h:19-41):
+// Stirling's series in 1/x0^2 with an upward shift to x0 >= 7, binary32 in the written order;
+// hl2p = 0.5f * logf_(f32(2 pi)).  The device reads lgamma(k + 1), k < 1024, from a table this
+// same function filled on the host.
+SPKM_FN float lgamma_k(float x, float hl2p) {
+    SPKM_NO_CONTRACT
+    if (x == 1.0f || x == 2.0f) return 0.0f;
+    float x0 = x;
+    int n = 0;
+    if (x <= 7.0f) {
+        n = int(7.0f - x);
+        x0 = x + float(n);
+    }
+    const float x2 = 1.0f / (x0 * x0);
+    float g = 8.417508417508418e-04f;
+    g = g * x2 + -5.952380952380952e-04f;
+    g = g * x2 + 7.936507936507937e-04f;
+    g = g * x2 + -2.777777777777778e-03f;
+    g = g * x2 + 8.333333333333333e-02f;
+    float gl = ((g / x0 + hl2p) + (x0 - 0.5f) * logf_(x0)) - x0;
+    if (x <= 7.0f) {
+        for (int i = 0; i < n; ++i) {
+            gl = gl - logf_(x0 - 1.0f);
+            x0 = x0 - 1.0f;
+        }
+    }
+    return gl;
+}
+SPKM_FN float half_log_2pi() { return 0.5f * logf_(6.283185307179586f); }
+inline constexpr int kLgammaTable = 1024;
 SPKM_FN float log10f_(float x) {
     SPKM_NO_CONTRACT
     return float(log_d(double(x)) * 0.43429448190325182765);  // 1 / ln 10
