@@ -31,19 +31,20 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per pixel of each stage: a pointwise run reads its input planes once and
 # writes its output once; each separable pass of a Gaussian reads one 3-channel plane and writes
 # one, plus one read for an accumulating pass (12 B per 3-channel float plane, 8 B per RGBA16 px).
-STAGE_BYTES = {"upsample": 24, "develop": 36, "grain_blur": 48, "print": 24, "unsharp_blur": 48, "encode": 32}
+# The whole chain of the workload: upsample 24 + halation mixture of 3 components 168 + develop 36
+# + grain blur 48 + print 24 + unsharp blur 48 + unsharp combine and encode 32 = 380 B/px.  Stages
+# the engine runs as one kernel keep the bytes of the stages they fuse (unsharp_encode = 48 + 32).
+B_ALG = 380
+STAGE_BYTES = {"upsample": 24, "develop": 36, "grain_blur": 48, "print": 24, "unsharp_blur": 48, "encode": 32,
+               "unsharp_encode": 80, "halation": 168}
 STAGE_KERNEL = {"upsample": "k_upsample", "develop": "k_develop", "print": "k_print", "encode": "k_unsharp_encode",
-                "halation": "k_fir_v/k_fir_h/k_iir_row/k_iir_col", "grain_blur": "k_fir_v/k_fir_h",
-                "unsharp_blur": "k_fir_v/k_fir_h"}
+                "unsharp_encode": "k_unsharp_encode2d", "halation": "k_fir2d/k_fir_v/k_fir_h/k_iir_row/k_iir_col",
+                "grain_blur": "k_fir2d", "unsharp_blur": "k_fir2d"}
 METRIC = "Mpx/s full chain @24MP (frac of HBM peak); batched frames/s; fwd+bwd Mpx/s"
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
-
-
-def halation_bytes(n_comp):
-    return 48 + 60 * (n_comp - 1)
 
 
 def main():
@@ -109,17 +110,20 @@ def main():
     ms = (t1 - t0) * 1000.0 / args.steps
     px = W * H
     mpxs = px / (ms / 1000.0) / 1e6
-    b_alg = sum(STAGE_BYTES.values()) + halation_bytes(3)
+    b_alg = B_ALG
     chain_gbs = b_alg * px / (ms / 1000.0) / 1e9
     per_stage = {k: v["ms"] / v["count"] for k, v in stages.items()}
-    dom = max((k for k in per_stage if k in STAGE_BYTES), key=lambda k: per_stage[k])
+    # the dominant single-kernel stage
+    dom = max((k for k in per_stage if k in STAGE_BYTES and "/" not in STAGE_KERNEL[k]), key=lambda k: per_stage[k])
     dom_ms = per_stage[dom]
     dom_bytes = STAGE_BYTES[dom] * px
     achieved = dom_bytes / (dom_ms / 1000.0) / 1e9
     traffic = None
-    tf = ROOT / "profiles" / f"traffic_{STAGE_KERNEL[dom]}_24mp.json"
+    tf = ROOT / "profiles" / "traffic_chain_24mp.json"
     if tf.exists() and (W, H) == (6000, 4000):
-        traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+        entry = json.loads(tf.read_text()).get("kernels", {}).get(STAGE_KERNEL[dom])
+        if entry:
+            traffic = round(entry["hbm_bytes_per_launch"])
     log(f"[bench] {ms:.3f} ms/frame, {mpxs:.1f} Mpx/s; stages (ms): " +
         ", ".join(f"{k} {v:.3f}" for k, v in per_stage.items()))
 

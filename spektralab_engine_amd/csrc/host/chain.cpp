@@ -103,8 +103,9 @@ k::IirArgs iir_args(double sigma, uint32_t w, uint32_t h, uint32_t n, uint32_t m
     return a;
 }
 
-void fir_channel(const float* in, float* out, double sigma, uint32_t mode, const float* acc, double weight, uint32_t w,
-                 uint32_t h, float* tmp, hipStream_t s) {
+// One Gaussian on np contiguous planes sharing sigma and weight: FIR branch.
+void fir_planes(const float* in, float* out, double sigma, uint32_t mode, const float* acc, double weight, uint32_t w,
+                uint32_t h, uint32_t np, float* tmp, hipStream_t s) {
     const std::vector<float> taps = setup::fir_taps(sigma);
     const int r = int(taps.size() / 2);
     if (r > k::kFirMaxR) fail_internal("FIR radius above the kernel maximum");
@@ -113,71 +114,98 @@ void fir_channel(const float* in, float* out, double sigma, uint32_t mode, const
     a.h = h;
     a.r = uint32_t(r);
     for (size_t i = 0; i < taps.size(); ++i) a.taps[i] = taps[i];
-    a.acc_mode = 0;
-    k::fir_v(in, tmp, a, s);
-    a.acc_mode = mode;
-    a.weight = num::f32(weight);
-    k::fir_h(tmp, out, acc, a, s);
-}
-
-void iir_channel(const float* in, float* out, double sigma, uint32_t mode, const float* acc, double weight, uint32_t w,
-                 uint32_t h, float* tmp, float* tmp2, hipStream_t s) {
-    // rows: 128 threads, chunks of m = ceil(w / 128)
-    {
-        const uint32_t m = (w + 127) / 128;
-        const uint32_t nch = (w + m - 1) / m;
-        k::IirArgs a = iir_args(sigma, w, h, w, m, nch);
-        k::iir_row(in, tmp, a, s);
-    }
-    // columns: 32 chunks of a compile-time length; frames taller than 32 * 128 run transposed
-    uint32_t mc = 0;
-    for (uint32_t c : {16u, 32u, 64u, 128u})
-        if ((h + c - 1) / c <= 32) {
-            mc = c;
-            break;
-        }
-    if (mc) {
-        k::IirArgs a = iir_args(sigma, w, h, h, mc, (h + mc - 1) / mc);
+    if (r <= k::kFir2dMaxR && in != out) {  // tiles read halos of their neighbours: not in place
         a.acc_mode = mode;
         a.weight = num::f32(weight);
-        k::iir_col(tmp, out, acc, a, s);
-    } else {
-        k::transpose(tmp, tmp2, nullptr, w, h, 0, 1.0f, s);  // tmp2: h wide, w tall
-        const uint32_t m = (h + 127) / 128;
-        const uint32_t nch = (h + m - 1) / m;
-        k::IirArgs a = iir_args(sigma, h, w, h, m, nch);
-        k::iir_row(tmp2, tmp2, a, s);
-        k::transpose(tmp2, out, acc, h, w, mode, num::f32(weight), s);
+        k::fir2d(in, out, acc, a, np, s);
+        return;
+    }
+    a.acc_mode = 0;
+    k::fir_v(in, tmp, a, np, s);
+    a.acc_mode = mode;
+    a.weight = num::f32(weight);
+    k::fir_h(tmp, out, acc, a, np, s);
+}
+
+// Recursive branch: a row pass into tmp, then the column pass (fused accumulation) into out;
+// columns taller than one workgroup's 32 chunks run as transpose, row pass, transpose.
+void iir_planes(const float* in, float* out, double sigma, uint32_t mode, const float* acc, double weight, uint32_t w,
+                uint32_t h, uint32_t np, float* tmp, float* tmp2, hipStream_t s) {
+    const size_t npx = size_t(w) * h;
+    {
+        const int m = k::iir_row_chunk(w);
+        if (!m) fail_internal("recursive blur: rows longer than 16384 samples are not supported");
+        const k::IirArgs a = iir_args(sigma, w, h, w, uint32_t(m), (w + uint32_t(m) - 1) / uint32_t(m));
+        k::iir_row(in, tmp, a, h * np, s);
+    }
+    const int mc = k::iir_col_chunk(h);
+    if (mc) {
+        k::IirArgs a = iir_args(sigma, w, h, h, uint32_t(mc), (h + uint32_t(mc) - 1) / uint32_t(mc));
+        a.acc_mode = mode;
+        a.weight = num::f32(weight);
+        k::iir_col(tmp, out, acc, a, np, s);
+        return;
+    }
+    const int m = k::iir_row_chunk(h);
+    if (!m) fail_internal("recursive blur: columns taller than 16384 samples are not supported");
+    const k::IirArgs a = iir_args(sigma, h, w, h, uint32_t(m), (h + uint32_t(m) - 1) / uint32_t(m));
+    for (uint32_t p = 0; p < np; ++p) {
+        k::transpose(tmp + p * npx, tmp2, nullptr, w, h, 0, 1.0f, s);  // tmp2: h wide, w tall
+        k::iir_row(tmp2, tmp2, a, w, s);
+        k::transpose(tmp2, out + p * npx, acc ? acc + p * npx : nullptr, h, w, mode, num::f32(weight), s);
     }
 }
 
 }  // namespace
 
+// Per-channel branch (reference src/fx/gaussian.cpp:19-22): identity for sigma <= 0 or a zero FIR
+// radius, FIR below the crossover, recursive otherwise.  The three planes run as one launch per
+// pass when they share branch, sigma and weight.
 void blur3(const float* in, float* out, const double sigma[3], uint32_t mode, const float* acc, const double weight[3],
            uint32_t w, uint32_t h, double crossover, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep) {
     const size_t np = size_t(w) * h;
-    gpu::DevBuf tmp = pool.acquire(np * sizeof(float));
-    gpu::DevBuf tmp2;
-    keep.push_back(tmp);
+    enum Branch { kIdentity, kFir, kIir };
+    Branch br[3];
     for (int c = 0; c < 3; ++c) {
+        const double sg = sigma[c];
+        br[c] = (!(sg > 0.0) || (sg < crossover && setup::fir_radius(sg) == 0)) ? kIdentity
+                : sg < crossover                                               ? kFir
+                                                                               : kIir;
+    }
+    const bool same = br[0] == br[1] && br[1] == br[2] && sigma[0] == sigma[1] && sigma[1] == sigma[2] &&
+                      weight[0] == weight[1] && weight[1] == weight[2];
+    const bool need_tmp = br[0] != kIdentity || br[1] != kIdentity || br[2] != kIdentity;
+    gpu::DevBuf tmp, tmp2;
+    if (need_tmp) {
+        tmp = pool.acquire((same ? 3 : 1) * np * sizeof(float));
+        keep.push_back(tmp);
+    }
+    if ((br[0] == kIir || br[1] == kIir || br[2] == kIir) && !k::iir_col_chunk(h)) {
+        tmp2 = pool.acquire(np * sizeof(float));
+        keep.push_back(tmp2);
+    }
+    auto run = [&](int c, uint32_t nplanes) {
         const float* ic = in + size_t(c) * np;
         float* oc = out + size_t(c) * np;
         const float* ac = acc ? acc + size_t(c) * np : nullptr;
-        const double sg = sigma[c];
-        const int r = setup::fir_radius(sg);
-        if (!(sg > 0.0) || (sg < crossover && r == 0)) {  // identity
-            if (mode == 0) k::copy(ic, oc, np, s);
-            else if (mode == 1) k::lincomb(ic, nullptr, oc, uint32_t(np), num::f32(weight[c]), 0.0f, s);
-            else k::lincomb(ac, ic, oc, uint32_t(np), 1.0f, num::f32(weight[c]), s);
-        } else if (sg < crossover) {
-            fir_channel(ic, oc, sg, mode, ac, weight[c], w, h, gpu::as<float>(tmp), s);
+        if (br[c] == kFir) {
+            fir_planes(ic, oc, sigma[c], mode, ac, weight[c], w, h, nplanes, gpu::as<float>(tmp), s);
+        } else if (br[c] == kIir) {
+            iir_planes(ic, oc, sigma[c], mode, ac, weight[c], w, h, nplanes, gpu::as<float>(tmp),
+                       tmp2 ? gpu::as<float>(tmp2) : nullptr, s);
         } else {
-            if (!tmp2 && h > 32 * 128) {
-                tmp2 = pool.acquire(np * sizeof(float));
-                keep.push_back(tmp2);
+            for (uint32_t p = 0; p < nplanes; ++p) {
+                const size_t o = p * np;
+                if (mode == 0) k::copy(ic + o, oc + o, np, s);
+                else if (mode == 1) k::lincomb(ic + o, nullptr, oc + o, uint32_t(np), num::f32(weight[c]), 0.0f, s);
+                else k::lincomb(ac + o, ic + o, oc + o, uint32_t(np), 1.0f, num::f32(weight[c]), s);
             }
-            iir_channel(ic, oc, sg, mode, ac, weight[c], w, h, gpu::as<float>(tmp), gpu::as<float>(tmp2), s);
         }
+    };
+    if (same) {
+        run(0, 3);
+    } else {
+        for (int c = 0; c < 3; ++c) run(c, 1);
     }
 }
 
@@ -412,9 +440,23 @@ void print(const spk_plan& p, const Tables& t, const float* negative, uint16_t* 
         const double sb[3] = {p.scanner_lens_blur_px, p.scanner_lens_blur_px, p.scanner_lens_blur_px};
         blur3(rgb, rgb, sb, 0, nullptr, one, w, h, p.fir_crossover, pool, s, keep);
     }
-    gpu::DevBuf blurb;
     const double su = p.unsharp_sigma;
-    if (su > 0.0 && p.unsharp_amount > 0.0) {
+    const bool unsharp = su > 0.0 && p.unsharp_amount > 0.0;
+    const int ur = unsharp ? setup::fir_radius(su) : 0;
+    if (unsharp && su < p.fir_crossover && ur >= 1 && ur <= k::kFir2dMaxR) {
+        // blur, combine and encode in one pass over the print planes
+        const std::vector<float> taps = setup::fir_taps(su);
+        k::FirArgs fa{};
+        fa.w = w;
+        fa.h = h;
+        fa.r = uint32_t(ur);
+        for (size_t i = 0; i < taps.size(); ++i) fa.taps[i] = taps[i];
+        k::unsharp_encode2d(rgb, out, stride, dq, fa, w, h, s);
+        mark("unsharp_encode");
+        return;
+    }
+    gpu::DevBuf blurb;
+    if (unsharp) {
         blurb = pool.acquire(3 * np * sizeof(float));
         keep.push_back(blurb);
         const double sg[3] = {su, su, su};

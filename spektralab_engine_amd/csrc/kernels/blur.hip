@@ -19,19 +19,20 @@
 // bit-identical.  Compiled with -ffp-contract=off (the accumulation fuse rounds a product and a sum).
 #include <hip/hip_runtime.h>
 
+#include "fir_tile.h"
 #include "kernels.h"
+#include "xcd.h"
 
 #pragma clang fp contract(off)
 
 namespace spk::k {
 namespace {
 
-__device__ inline int reflect_index(int i, int n) {
-    if (n <= 1) return 0;
-    const int period = 2 * n;
-    int m = i % period;
-    if (m < 0) m += period;
-    return m < n ? m : period - 1 - m;
+// mode 0 NONE: v; 1 ZERO: 0 + w v; 2 PLANE: acc + w v (the accumulation value already loaded).
+__device__ inline float fuse_v(uint32_t mode, float acc, float weight, float v) {
+    if (mode == 0u) return v;
+    if (mode == 1u) return 0.0f + weight * v;
+    return acc + weight * v;
 }
 
 __device__ inline float fuse(uint32_t mode, const float* __restrict__ acc, size_t o, float weight, float v) {
@@ -49,11 +50,22 @@ __device__ inline float fuse(uint32_t mode, const float* __restrict__ acc, size_
 constexpr int kT = 16;
 
 // Vertical: a wave is 64 adjacent columns of one run of kT rows; loads are coalesced rows, and
-// rows outside the frame (wave-uniform) are folded by half-sample reflection.
-__global__ __launch_bounds__(256) void k_fir_v(const float* __restrict__ in, float* __restrict__ out, FirArgs q) {
+// rows outside the frame (wave-uniform) are folded by half-sample reflection.  nplanes
+// contiguous planes of w x h run in one launch.
+__global__ __launch_bounds__(256) void k_fir_v(const float* __restrict__ in, float* __restrict__ out, FirArgs q,
+                                               uint32_t nplanes) {
     const int w = int(q.w), h = int(q.h), R = int(q.r);
-    const int x = int(blockIdx.x * 64 + threadIdx.x);
-    const int y0 = int((blockIdx.y * 4 + threadIdx.y) * kT);
+    const uint32_t gx = (q.w + 63) / 64, gy = (q.h + 4 * kT - 1) / (4 * kT);
+    const uint32_t total = gx * gy * nplanes;
+    const uint32_t item = xcd_item(blockIdx.x, total);
+    if (item >= total) return;
+    // vertically adjacent tiles are consecutive items of one XCD: their halo rows stay in its L2
+    const uint32_t by = item % gy, rest = item / gy, bx = rest % gx, plane = rest / gx;
+    const size_t poff = size_t(plane) * q.w * q.h;
+    in += poff;
+    out += poff;
+    const int x = int(bx * 64 + threadIdx.x);
+    const int y0 = int((by * 4 + threadIdx.y) * kT);
     if (y0 >= h) return;
     const int xc = x < w ? x : w - 1;
     const float* col = in + xc;
@@ -97,9 +109,10 @@ constexpr int kHSeg = 64 * kT;
 __host__ __device__ inline int hpad(int t) { return t + (t >> 4); }
 
 __global__ __launch_bounds__(256) void k_fir_h(const float* __restrict__ in, float* __restrict__ out,
-                                               const float* __restrict__ accp, FirArgs q) {
+                                               const float* __restrict__ accp, FirArgs q, uint32_t nrows_planes) {
     extern __shared__ float lds_h[];
-    const int w = int(q.w), h = int(q.h), R = int(q.r);
+    const int w = int(q.w), R = int(q.r);
+    const int h = int(q.h * nrows_planes);  // rows of all planes, contiguous
     const int y = int(blockIdx.y * 4 + threadIdx.y);
     const int x0 = int(blockIdx.x) * kHSeg;
     const int span = kHSeg + 2 * R + kT;
@@ -143,11 +156,57 @@ __global__ __launch_bounds__(256) void k_fir_h(const float* __restrict__ in, flo
     for (int j = 0; j < kT; ++j) tile[hpad(base + j)] = acc[j];
     __syncthreads();
     if (!rowok) return;
-    for (int t = lane; t < kHSeg; t += 64) {
-        const int x = x0 + t;
-        if (x >= w) break;
-        const size_t o = size_t(y) * w + x;
-        out[o] = fuse(q.acc_mode, accp, o, q.weight, tile[hpad(t)]);
+    // 16 coalesced results per lane; the accumulation plane's loads are issued before the stores
+    float r[kT], av[kT];
+#pragma unroll
+    for (int u = 0; u < kT; ++u) {
+        const int t = lane + 64 * u, x = x0 + t;
+        r[u] = tile[hpad(t)];
+        av[u] = (q.acc_mode == 2u && x < w) ? accp[size_t(y) * w + x] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kT; ++u) {
+        const int x = x0 + lane + 64 * u;
+        if (x < w) out[size_t(y) * w + x] = fuse_v(q.acc_mode, av[u], q.weight, r[u]);
+    }
+}
+
+// Small radii: both passes in one kernel on 64 x 32 output tiles (fir_tile.h); nplanes contiguous
+// planes; tiles walked in XCD order so that neighbours share their halo rows in one L2.
+constexpr int kF2TX = 64, kF2TY = 32, kF2NT = 256;
+
+template <int R>
+__global__ __launch_bounds__(kF2NT) void k_fir2d(const float* __restrict__ in, float* __restrict__ out,
+                                                const float* __restrict__ accp, FirArgs q, uint32_t nplanes) {
+    __shared__ FirTile<R, kF2TX, kF2TY> t;
+    const int w = int(q.w), h = int(q.h);
+    const uint32_t gx = (q.w + kF2TX - 1) / kF2TX, gy = (q.h + kF2TY - 1) / kF2TY;
+    const uint32_t total = gx * gy * nplanes;
+    const uint32_t item = xcd_item(blockIdx.x, total);
+    if (item >= total) return;
+    const uint32_t bx = item % gx, rest = item / gx, by = rest % gy, plane = rest / gy;
+    const size_t poff = size_t(plane) * q.w * q.h;
+    const int x0 = int(bx) * kF2TX, y0 = int(by) * kF2TY;
+    fir_tile_load<R, kF2TX, kF2TY, kF2NT>(t, in + poff, w, h, x0, y0);
+    __syncthreads();
+    fir_tile_v<R, kF2TX, kF2TY, kF2NT>(t, q.taps);
+    __syncthreads();
+    constexpr int PER = kF2TX * kF2TY / kF2NT;
+    float r[PER], av[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = int(threadIdx.x) + u * kF2NT;
+        const int row = e / kF2TX, col = e - row * kF2TX;
+        const int x = x0 + col, y = y0 + row;
+        r[u] = fir_tile_h<R, kF2TX, kF2TY>(t, q.taps, row, col);
+        av[u] = (q.acc_mode == 2u && x < w && y < h) ? accp[poff + size_t(y) * q.w + x] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = int(threadIdx.x) + u * kF2NT;
+        const int row = e / kF2TX, col = e - row * kF2TX;
+        const int x = x0 + col, y = y0 + row;
+        if (x < w && y < h) out[poff + size_t(y) * q.w + x] = fuse_v(q.acc_mode, av[u], q.weight, r[u]);
     }
 }
 
@@ -168,103 +227,6 @@ __device__ inline St mat_apply(const double* M, const St& v) {
 
 __device__ inline St st_add(const St& x, const St& y) { return {x.a + y.a, x.b + y.b, x.c + y.c}; }
 
-constexpr uint32_t kRowThreads = 128;
-
-// One (row, channel) per block; the row's n samples in LDS as nch chunks of m (stride m | 1).
-__global__ __launch_bounds__(128) void k_iir_row(const float* __restrict__ in, float* __restrict__ out, IirArgs q) {
-    extern __shared__ double smem_d[];
-    const uint32_t n = q.n, m = q.m, nch = q.nch;
-    const uint32_t s = m | 1u;
-    double* V = smem_d;                                   // [kRowThreads] x 3
-    float* data = reinterpret_cast<float*>(smem_d + 3 * kRowThreads);
-    const float* src = in + size_t(blockIdx.x) * n;
-    float* dst = out + size_t(blockIdx.x) * n;
-    const uint32_t j = threadIdx.x;
-    for (uint32_t i = j; i < n; i += kRowThreads) data[(i / m) * s + i % m] = src[i];
-    __syncthreads();
-    const bool has = j < nch;
-    const uint32_t len = has ? min(m, n - j * m) : 0u;
-    float* chunk = data + j * s;
-
-    // ---- forward ----
-    St z{0.0, 0.0, 0.0};
-    for (uint32_t k = 0; k < len; ++k) {
-        const double t = yvv_step(q, chunk[k], z);
-        z = {t, z.a, z.b};
-    }
-    const double x0 = data[0];
-    const St e0{x0, x0, x0};
-    if (j == 0) z = st_add(mat_apply(q.pw[0], e0), z);
-    V[3 * j] = z.a;
-    V[3 * j + 1] = z.b;
-    V[3 * j + 2] = z.c;
-    __syncthreads();
-    for (uint32_t d = 1, di = 0; d < nch; d <<= 1, ++di) {
-        St cur{V[3 * j], V[3 * j + 1], V[3 * j + 2]};
-        St prev{0.0, 0.0, 0.0};
-        const bool act = has && j >= d;
-        if (act) prev = {V[3 * (j - d)], V[3 * (j - d) + 1], V[3 * (j - d) + 2]};
-        __syncthreads();
-        if (act) {
-            cur = st_add(cur, mat_apply(q.pw[di], prev));
-            V[3 * j] = cur.a;
-            V[3 * j + 1] = cur.b;
-            V[3 * j + 2] = cur.c;
-        }
-        __syncthreads();
-    }
-    St e = e0;
-    if (j > 0 && has) e = {V[3 * (j - 1)], V[3 * (j - 1) + 1], V[3 * (j - 1) + 2]};
-    __syncthreads();
-    for (uint32_t k = 0; k < len; ++k) {
-        const double t = yvv_step(q, chunk[k], e);
-        e = {t, e.a, e.b};
-        chunk[k] = float(t);
-    }
-
-    // ---- backward (same chunks, processed from the end) ----
-    St zb{0.0, 0.0, 0.0};
-    for (uint32_t k = len; k-- > 0;) {
-        const double t = yvv_step(q, chunk[k], zb);
-        zb = {t, zb.a, zb.b};
-    }
-    const uint32_t last = nch - 1;
-    St eb{0.0, 0.0, 0.0};
-    if (j == last) {
-        const double un = chunk[len - 1];
-        eb = {un, un, un};
-        zb = st_add(mat_apply(q.last, eb), zb);
-    }
-    V[3 * j] = zb.a;
-    V[3 * j + 1] = zb.b;
-    V[3 * j + 2] = zb.c;
-    __syncthreads();
-    for (uint32_t d = 1, di = 0; d < nch; d <<= 1, ++di) {
-        St cur{V[3 * j], V[3 * j + 1], V[3 * j + 2]};
-        St nxt{0.0, 0.0, 0.0};
-        const bool act = has && j + d <= last;
-        if (act) nxt = {V[3 * (j + d)], V[3 * (j + d) + 1], V[3 * (j + d) + 2]};
-        __syncthreads();
-        if (act) {
-            cur = st_add(cur, mat_apply(q.pw[di], nxt));
-            V[3 * j] = cur.a;
-            V[3 * j + 1] = cur.b;
-            V[3 * j + 2] = cur.c;
-        }
-        __syncthreads();
-    }
-    if (has && j < last) eb = {V[3 * (j + 1)], V[3 * (j + 1) + 1], V[3 * (j + 1) + 2]};
-    for (uint32_t k = len; k-- > 0;) {
-        const double t = yvv_step(q, chunk[k], eb);
-        eb = {t, eb.a, eb.b};
-        chunk[k] = float(t);
-    }
-    __syncthreads();
-    for (uint32_t i = j; i < n; i += kRowThreads) dst[i] = data[(i / m) * s + i % m];
-}
-
-constexpr uint32_t kColW = 16, kColChunks = 32;
-
 // Raw buffer access moves 32-bit patterns: reinterpret them, never convert.
 __device__ inline float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
@@ -273,39 +235,31 @@ __device__ inline void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, 
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
 }
 
-// A strip of 16 columns of one channel per block; thread (tx, j) holds rows [j MC, j MC + MC) of
-// column x in registers.  Every chunk has the compile-time length MC: the last chunk's rows past
-// the frame are padded with the replicated edge value u[n-1], which the backward recursion (unit
-// DC gain) carries unchanged to row n-1, where the reference starts from y[n..n+2] = u[n-1].
+// The forward-backward recursion over one line split into nch chunks of MC samples, thread j
+// holding chunk j in v[] (the last chunk's samples past the line end padded).  Each direction:
+// every chunk runs from a zero state; the chunk-end states are combined by a Hillis-Steele scan
+// with the companion powers pw[d] = A^(MC 2^d) through the LDS slots V + jj vs (3 doubles);
+// every chunk re-runs from its true entry state.  The forward start is w[-1..-3] = x0; the
+// backward start is y[n..n+2] = u[n-1]: the last chunk's padding is set to u[n-1] before the
+// backward pass, which the unit-DC-gain recursion carries unchanged to sample n-1.  Called by
+// every thread of the workgroup (it synchronises).
 template <int MC>
-__global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, float* __restrict__ out,
-                                                 const float* __restrict__ accp, IirArgs q) {
-    __shared__ double V[kColChunks][kColW][3];
-    const uint32_t tx = threadIdx.x, j = threadIdx.y;
-    const uint32_t w = q.w;
-    const uint32_t x = blockIdx.x * kColW + tx;
-    const bool colok = x < w;
-    const uint32_t n = q.n, nch = q.nch;
+__device__ inline void yvv_chunks(float (&v)[MC], uint32_t j, uint32_t nch, uint32_t len, double x0,
+                                  const IirArgs& q, double* V, uint32_t vs) {
     const bool has = j < nch;
-    const uint32_t start = j * uint32_t(MC);
-    const uint32_t len = has ? min(uint32_t(MC), n - start) : 0u;
     const uint32_t last = nch - 1;
-    // Buffer addressing with a per-lane base offset.  Loads clamp the row to the chunk's last
-    // valid row (lanes past the frame edge read column w-1; lanes without a chunk read row 0), so
-    // every load stays inside the plane; stores use a wave-uniform row offset and only rows < n.
-    const uint32_t rowb = w * 4u;
-    const uint32_t plane_bytes = q.w * q.h * 4u;
-    const __amdgpu_buffer_rsrc_t rin =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(in), (short)0, plane_bytes, 0x00020000);
-    const uint32_t xc = colok ? x : w - 1u;
-    const uint32_t kmax = has ? len - 1u : 0u;
-    const uint32_t lbase = ((has ? start : 0u) * w + xc) * 4u;
-    const uint32_t voff = (start * w + x) * 4u;
-    float v[MC];
-#pragma unroll
-    for (int k = 0; k < MC; ++k) v[k] = buf_ld(rin, lbase + min(uint32_t(k), kmax) * rowb, 0u);
+    auto put = [&](uint32_t jj, const St& t) {
+        double* p = V + jj * vs;
+        p[0] = t.a;
+        p[1] = t.b;
+        p[2] = t.c;
+    };
+    auto get = [&](uint32_t jj) {
+        const double* p = V + jj * vs;
+        return St{p[0], p[1], p[2]};
+    };
 
-    // ---- forward: zero-state chunk pass, carry scan, true-state pass ----
+    // ---- forward ----
     St z{0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < MC; ++k) {
@@ -313,32 +267,26 @@ __global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, f
         z = {t, z.a, z.b};
     }
     // Opaque register copies: keep the compiler from holding float64 conversions of the whole
-    // chunk live across the two passes.
+    // chunk live across the passes.
 #pragma unroll
     for (int k = 0; k < MC; ++k) asm volatile("" : "+v"(v[k]));
-    const double x0 = double(buf_ld(rin, xc * 4u, 0u));
     const St e0{x0, x0, x0};
     if (j == 0) z = st_add(mat_apply(q.pw[0], e0), z);
-    V[j][tx][0] = z.a;
-    V[j][tx][1] = z.b;
-    V[j][tx][2] = z.c;
+    if (has) put(j, z);
     __syncthreads();
     for (uint32_t d = 1, di = 0; d < nch; d <<= 1, ++di) {
-        St cur{V[j][tx][0], V[j][tx][1], V[j][tx][2]};
-        St prev{0.0, 0.0, 0.0};
         const bool act = has && j >= d;
-        if (act) prev = {V[j - d][tx][0], V[j - d][tx][1], V[j - d][tx][2]};
-        __syncthreads();
+        St cur{0.0, 0.0, 0.0}, prev{0.0, 0.0, 0.0};
         if (act) {
-            cur = st_add(cur, mat_apply(q.pw[di], prev));
-            V[j][tx][0] = cur.a;
-            V[j][tx][1] = cur.b;
-            V[j][tx][2] = cur.c;
+            cur = get(j);
+            prev = get(j - d);
         }
+        __syncthreads();
+        if (act) put(j, st_add(cur, mat_apply(q.pw[di], prev)));
         __syncthreads();
     }
     St e = e0;
-    if (j > 0 && has) e = {V[j - 1][tx][0], V[j - 1][tx][1], V[j - 1][tx][2]};
+    if (j > 0 && has) e = get(j - 1);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < MC; ++k) {
@@ -366,25 +314,20 @@ __global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, f
         zb = {t, zb.a, zb.b};
     }
     if (j == last) zb = st_add(mat_apply(q.pw[0], eb), zb);
-    V[j][tx][0] = zb.a;
-    V[j][tx][1] = zb.b;
-    V[j][tx][2] = zb.c;
+    if (has) put(j, zb);
     __syncthreads();
     for (uint32_t d = 1, di = 0; d < nch; d <<= 1, ++di) {
-        St cur{V[j][tx][0], V[j][tx][1], V[j][tx][2]};
-        St nxt{0.0, 0.0, 0.0};
         const bool act = has && j + d <= last;
-        if (act) nxt = {V[j + d][tx][0], V[j + d][tx][1], V[j + d][tx][2]};
-        __syncthreads();
+        St cur{0.0, 0.0, 0.0}, nxt{0.0, 0.0, 0.0};
         if (act) {
-            cur = st_add(cur, mat_apply(q.pw[di], nxt));
-            V[j][tx][0] = cur.a;
-            V[j][tx][1] = cur.b;
-            V[j][tx][2] = cur.c;
+            cur = get(j);
+            nxt = get(j + d);
         }
         __syncthreads();
+        if (act) put(j, st_add(cur, mat_apply(q.pw[di], nxt)));
+        __syncthreads();
     }
-    if (has && j < last) eb = {V[j + 1][tx][0], V[j + 1][tx][1], V[j + 1][tx][2]};
+    if (has && j < last) eb = get(j + 1);
 #pragma unroll
     for (int k = 0; k < MC; ++k) asm volatile("" : "+v"(v[k]));
 #pragma unroll
@@ -393,17 +336,125 @@ __global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, f
         eb = {t, eb.a, eb.b};
         v[k] = float(t);
     }
+}
+
+constexpr uint32_t kRowThreads = 128;
+constexpr uint32_t kStage = 8;
+
+__device__ inline uint32_t rpad(uint32_t i) { return i + (i >> 5); }
+__host__ inline size_t row_lds_floats(uint32_t n) { return size_t(n) + n / 32 + 1; }
+
+// One line of n = q.n samples per workgroup (lines of all planes are contiguous: line b starts at
+// b n); 128 threads, chunks of MC (nch <= 128).  The line is staged through LDS (one pad word per
+// 32), each thread's chunk held in registers.
+template <int MC>
+__global__ __launch_bounds__(128) void k_iir_row(const float* __restrict__ in, float* __restrict__ out, IirArgs q) {
+    extern __shared__ float srow[];
+    __shared__ double V[kRowThreads * 3];
+    const uint32_t n = q.n, nch = q.nch;
+    const float* src = in + size_t(blockIdx.x) * n;
+    float* dst = out + size_t(blockIdx.x) * n;
+    const uint32_t j = threadIdx.x;
+    // kStage coalesced loads in flight per thread before their LDS stores
+    for (uint32_t i0 = 0; i0 < n; i0 += kRowThreads * kStage) {
+        float t[kStage];
+#pragma unroll
+        for (uint32_t u = 0; u < kStage; ++u) {
+            const uint32_t i = i0 + u * kRowThreads + j;
+            t[u] = i < n ? src[i] : 0.0f;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kStage; ++u) {
+            const uint32_t i = i0 + u * kRowThreads + j;
+            if (i < n) srow[rpad(i)] = t[u];
+        }
+    }
+    __syncthreads();
+    const bool has = j < nch;
+    const uint32_t start = has ? j * uint32_t(MC) : 0u;
+    const uint32_t len = has ? min(uint32_t(MC), n - start) : 0u;
+    const uint32_t kmax = has ? len - 1u : 0u;
+    float v[MC];
+#pragma unroll
+    for (int k = 0; k < MC; ++k) v[k] = srow[rpad(start + min(uint32_t(k), kmax))];
+    const double x0 = double(srow[0]);
+    yvv_chunks<MC>(v, j, nch, len, x0, q, V, 3u);
+#pragma unroll
+    for (int k = 0; k < MC; ++k)
+        if (uint32_t(k) < len) srow[rpad(start + uint32_t(k))] = v[k];
+    __syncthreads();
+    for (uint32_t i0 = 0; i0 < n; i0 += kRowThreads * kStage) {
+        float t[kStage];
+#pragma unroll
+        for (uint32_t u = 0; u < kStage; ++u) {
+            const uint32_t i = i0 + u * kRowThreads + j;
+            t[u] = i < n ? srow[rpad(i)] : 0.0f;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kStage; ++u) {
+            const uint32_t i = i0 + u * kRowThreads + j;
+            if (i < n) dst[i] = t[u];
+        }
+    }
+}
+
+constexpr uint32_t kColW = 16, kColChunks = 32;
+
+// A strip of 16 columns of one plane per workgroup (nplanes contiguous planes, strips walked in
+// XCD order); thread (tx, j) holds rows [j MC, j MC + MC) of column x in registers.  Loads clamp
+// the row to the chunk's last valid row (lanes past the frame edge read column w-1), so every load
+// stays inside the plane; stores use a wave-uniform row offset and only rows < n.  The mixture
+// accumulation (out = acc + w y, or 0 + w y) is fused into the store.
+template <int MC>
+__global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, float* __restrict__ out,
+                                                 const float* __restrict__ accp, IirArgs q, uint32_t nplanes) {
+    __shared__ double V[kColChunks * kColW * 3];
+    const uint32_t w = q.w, n = q.n, nch = q.nch;
+    const uint32_t nstrips = (w + kColW - 1) / kColW;
+    const uint32_t total = nstrips * nplanes;
+    const uint32_t item = xcd_item(blockIdx.x, total);
+    if (item >= total) return;
+    const uint32_t strip = item % nstrips, plane = item / nstrips;
+    const size_t poff = size_t(plane) * q.w * q.h;
+    in += poff;
+    out += poff;
+    if (accp) accp += poff;
+    const uint32_t tx = threadIdx.x, j = threadIdx.y;
+    const uint32_t x = strip * kColW + tx;
+    const bool colok = x < w;
+    const bool has = j < nch;
+    const uint32_t start = j * uint32_t(MC);
+    const uint32_t len = has ? min(uint32_t(MC), n - start) : 0u;
+    const uint32_t rowb = w * 4u;
+    const uint32_t plane_bytes = q.w * q.h * 4u;
+    const __amdgpu_buffer_rsrc_t rin =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(in), (short)0, plane_bytes, 0x00020000);
+    const uint32_t xc = colok ? x : w - 1u;
+    const uint32_t kmax = has ? len - 1u : 0u;
+    const uint32_t lbase = ((has ? start : 0u) * w + xc) * 4u;
+    const uint32_t voff = (start * w + x) * 4u;
+    float v[MC];
+#pragma unroll
+    for (int k = 0; k < MC; ++k) v[k] = buf_ld(rin, lbase + min(uint32_t(k), kmax) * rowb, 0u);
+    const double x0 = double(buf_ld(rin, xc * 4u, 0u));
+    yvv_chunks<MC>(v, j, nch, len, x0, q, V + tx * 3, kColW * 3);
     if (!colok || !has) return;
     const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc(out, (short)0, plane_bytes, 0x00020000);
     if (q.acc_mode == 2u) {
+        // accumulation plane loads batched ahead of the stores (out may be the accumulation plane)
         const __amdgpu_buffer_rsrc_t racc =
             __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(accp), (short)0, plane_bytes, 0x00020000);
+        constexpr int G = MC < 16 ? MC : 16;
 #pragma unroll
-        for (int k = 0; k < MC; ++k)
-            if (uint32_t(k) < len) {
-                const float a = buf_ld(racc, voff, uint32_t(k) * rowb);
-                buf_st(a + q.weight * v[k], rout, voff, uint32_t(k) * rowb);
-            }
+        for (int k0 = 0; k0 < MC; k0 += G) {
+            float av[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                av[u] = uint32_t(k0 + u) < len ? buf_ld(racc, voff, uint32_t(k0 + u) * rowb) : 0.0f;
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (uint32_t(k0 + u) < len) buf_st(av[u] + q.weight * v[k0 + u], rout, voff, uint32_t(k0 + u) * rowb);
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < MC; ++k)
@@ -444,30 +495,65 @@ __global__ __launch_bounds__(256) void k_lincomb(const float* __restrict__ x, co
 
 }  // namespace
 
-void fir_v(const float* in, float* out, const FirArgs& a, hipStream_t s) {
-    const uint32_t rows_per_block = 4 * kT;
-    hipLaunchKernelGGL(k_fir_v, dim3((a.w + 63) / 64, (a.h + rows_per_block - 1) / rows_per_block), dim3(64, 4), 0, s, in,
-                       out, a);
+void fir_v(const float* in, float* out, const FirArgs& a, uint32_t nplanes, hipStream_t s) {
+    const uint32_t gx = (a.w + 63) / 64, gy = (a.h + 4 * kT - 1) / (4 * kT);
+    hipLaunchKernelGGL(k_fir_v, dim3(xcd_grid(gx * gy * nplanes)), dim3(64, 4), 0, s, in, out, a, nplanes);
 }
 
-void fir_h(const float* in, float* out, const float* acc, const FirArgs& a, hipStream_t s) {
+void fir_h(const float* in, float* out, const float* acc, const FirArgs& a, uint32_t nplanes, hipStream_t s) {
     const int span = kHSeg + 2 * int(a.r) + kT;
     const size_t lds = 4 * size_t(hpad(span) + 1) * sizeof(float);
-    hipLaunchKernelGGL(k_fir_h, dim3((a.w + kHSeg - 1) / kHSeg, (a.h + 3) / 4), dim3(64, 4), lds, s, in, out, acc, a);
+    hipLaunchKernelGGL(k_fir_h, dim3((a.w + kHSeg - 1) / kHSeg, (a.h * nplanes + 3) / 4), dim3(64, 4), lds, s, in, out,
+                       acc, a, nplanes);
 }
 
-void iir_row(const float* in, float* out, const IirArgs& a, hipStream_t s) {
-    const size_t lds = 3 * kRowThreads * sizeof(double) + size_t(a.nch) * (a.m | 1u) * sizeof(float);
-    hipLaunchKernelGGL(k_iir_row, dim3(a.h), dim3(kRowThreads), lds, s, in, out, a);
+void fir2d(const float* in, float* out, const float* acc, const FirArgs& a, uint32_t nplanes, hipStream_t s) {
+    const uint32_t gx = (a.w + kF2TX - 1) / kF2TX, gy = (a.h + kF2TY - 1) / kF2TY;
+    const dim3 grid(xcd_grid(gx * gy * nplanes)), block(kF2NT);
+    switch (a.r) {
+        case 1: hipLaunchKernelGGL(k_fir2d<1>, grid, block, 0, s, in, out, acc, a, nplanes); break;
+        case 2: hipLaunchKernelGGL(k_fir2d<2>, grid, block, 0, s, in, out, acc, a, nplanes); break;
+        case 3: hipLaunchKernelGGL(k_fir2d<3>, grid, block, 0, s, in, out, acc, a, nplanes); break;
+        case 4: hipLaunchKernelGGL(k_fir2d<4>, grid, block, 0, s, in, out, acc, a, nplanes); break;
+        case 5: hipLaunchKernelGGL(k_fir2d<5>, grid, block, 0, s, in, out, acc, a, nplanes); break;
+        case 6: hipLaunchKernelGGL(k_fir2d<6>, grid, block, 0, s, in, out, acc, a, nplanes); break;
+        case 7: hipLaunchKernelGGL(k_fir2d<7>, grid, block, 0, s, in, out, acc, a, nplanes); break;
+        default: hipLaunchKernelGGL(k_fir2d<8>, grid, block, 0, s, in, out, acc, a, nplanes); break;
+    }
 }
 
-void iir_col(const float* in, float* out, const float* acc, const IirArgs& a, hipStream_t s) {
-    const dim3 grid((a.w + kColW - 1) / kColW), block(kColW, kColChunks);
+int iir_row_chunk(uint32_t n) {
+    for (int mc : {16, 32, 48, 64, 96, 128})
+        if ((n + uint32_t(mc) - 1) / uint32_t(mc) <= kRowThreads) return mc;
+    return 0;
+}
+
+void iir_row(const float* in, float* out, const IirArgs& a, uint32_t lines, hipStream_t s) {
+    const size_t lds = row_lds_floats(a.n) * sizeof(float);
+    const dim3 grid(lines), block(kRowThreads);
     switch (a.m) {
-        case 16: hipLaunchKernelGGL(k_iir_col<16>, grid, block, 0, s, in, out, acc, a); break;
-        case 32: hipLaunchKernelGGL(k_iir_col<32>, grid, block, 0, s, in, out, acc, a); break;
-        case 64: hipLaunchKernelGGL(k_iir_col<64>, grid, block, 0, s, in, out, acc, a); break;
-        default: hipLaunchKernelGGL(k_iir_col<128>, grid, block, 0, s, in, out, acc, a); break;
+        case 16: hipLaunchKernelGGL(k_iir_row<16>, grid, block, lds, s, in, out, a); break;
+        case 32: hipLaunchKernelGGL(k_iir_row<32>, grid, block, lds, s, in, out, a); break;
+        case 48: hipLaunchKernelGGL(k_iir_row<48>, grid, block, lds, s, in, out, a); break;
+        case 64: hipLaunchKernelGGL(k_iir_row<64>, grid, block, lds, s, in, out, a); break;
+        case 96: hipLaunchKernelGGL(k_iir_row<96>, grid, block, lds, s, in, out, a); break;
+        default: hipLaunchKernelGGL(k_iir_row<128>, grid, block, lds, s, in, out, a); break;
+    }
+}
+
+int iir_col_chunk(uint32_t h) {
+    for (int mc : {16, 32, 64, 128})
+        if ((h + uint32_t(mc) - 1) / uint32_t(mc) <= kColChunks) return mc;
+    return 0;
+}
+
+void iir_col(const float* in, float* out, const float* acc, const IirArgs& a, uint32_t nplanes, hipStream_t s) {
+    const dim3 grid(xcd_grid(((a.w + kColW - 1) / kColW) * nplanes)), block(kColW, kColChunks);
+    switch (a.m) {
+        case 16: hipLaunchKernelGGL(k_iir_col<16>, grid, block, 0, s, in, out, acc, a, nplanes); break;
+        case 32: hipLaunchKernelGGL(k_iir_col<32>, grid, block, 0, s, in, out, acc, a, nplanes); break;
+        case 64: hipLaunchKernelGGL(k_iir_col<64>, grid, block, 0, s, in, out, acc, a, nplanes); break;
+        default: hipLaunchKernelGGL(k_iir_col<128>, grid, block, 0, s, in, out, acc, a, nplanes); break;
     }
 }
 

@@ -84,10 +84,19 @@ struct PrintConsts {
 
 // Launchers (stream-ordered, no synchronisation). Planes as documented above.
 void upsample(const float* rgb, float* H, const float* table, const UpsampleArgs& a, hipStream_t s);
-void fir_v(const float* in, float* out, const FirArgs& a, hipStream_t s);
-void fir_h(const float* in, float* out, const float* acc, const FirArgs& a, hipStream_t s);
-void iir_row(const float* in, float* out, const IirArgs& a, hipStream_t s);
-void iir_col(const float* in, float* out, const float* acc, const IirArgs& a, hipStream_t s);
+// The passes run on nplanes contiguous planes of a.w x a.h with the same arguments.
+void fir_v(const float* in, float* out, const FirArgs& a, uint32_t nplanes, hipStream_t s);
+void fir_h(const float* in, float* out, const float* acc, const FirArgs& a, uint32_t nplanes, hipStream_t s);
+// Recursive passes: a.m is the compile-time chunk length iir_row_chunk(line length) /
+// iir_col_chunk(column height) returns (0: too long for one workgroup).  iir_row filters `lines`
+// contiguous lines of a.n samples.
+// Both passes in one kernel for 1 <= a.r <= kFir2dMaxR (a.acc_mode applies to the result).
+inline constexpr int kFir2dMaxR = 8;
+void fir2d(const float* in, float* out, const float* acc, const FirArgs& a, uint32_t nplanes, hipStream_t s);
+int iir_row_chunk(uint32_t n);
+int iir_col_chunk(uint32_t h);
+void iir_row(const float* in, float* out, const IirArgs& a, uint32_t lines, hipStream_t s);
+void iir_col(const float* in, float* out, const float* acc, const IirArgs& a, uint32_t nplanes, hipStream_t s);
 // Transposes a w x h single-channel plane into h x w; with acc_mode, the transposed value v is
 // combined as out = (mode ZERO ? 0 : acc) + weight v.
 void transpose(const float* in, float* out, const float* acc, uint32_t w, uint32_t h, uint32_t acc_mode, float weight,
@@ -102,6 +111,9 @@ void print_pointwise(const float* neg, float* rgb, const PrintConsts* pc, const 
 void philox_test(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n, hipStream_t s);
 void poisson_test(const float* lam, float* out, uint32_t n, uint32_t stream, uint32_t seed, const float* lgam,
                   hipStream_t s);
+// The unsharp blur (FIR radius 1..kFir2dMaxR, taps in f) fused with the combine and encode.
+void unsharp_encode2d(const float* rgb, uint16_t* out, uint32_t stride, const PrintConsts* pc, const FirArgs& f,
+                      uint32_t w, uint32_t h, hipStream_t s);
 void unsharp_encode(const float* rgb, const float* blurred, uint16_t* out, uint32_t stride, const PrintConsts* pc,
                     uint32_t w, uint32_t h, hipStream_t s);
 

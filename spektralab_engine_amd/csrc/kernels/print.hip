@@ -12,7 +12,9 @@
 // Float contraction is allowed here (the output is compared to the CPU oracle within 1 LSB).
 #include <hip/hip_runtime.h>
 
+#include "fir_tile.h"
 #include "kernels.h"
+#include "xcd.h"
 
 namespace spk::k {
 namespace {
@@ -142,10 +144,10 @@ __device__ inline float paper_lookup(const float* __restrict__ tab, uint32_t K, 
 
 __device__ inline float cctf_encode(float v, uint32_t mode) {
     switch (mode) {
-        case 0: return v <= 0.0031308f ? 12.92f * v : 1.055f * (sgn(v) * powf(fabsf(v), 0.41666666666666669f)) - 0.055f;
-        case 1: return v < 0.001953125f ? 16.0f * v : powf(v, 0.55555555555555558f);
-        case 2: return powf(v, 0.45470692717584371f);
-        case 3: return v < 0.018f ? 4.5f * v : 1.099f * (sgn(v) * powf(fabsf(v), 0.45f)) - 0.099f;
+        case 0: return v <= 0.0031308f ? 12.92f * v : 1.055f * (sgn(v) * fpow(fabsf(v), 0.41666666666666669f)) - 0.055f;
+        case 1: return v < 0.001953125f ? 16.0f * v : fpow(v, 0.55555555555555558f);
+        case 2: return v >= 0.0f ? fpow(v, 0.45470692717584371f) : __builtin_nanf("");
+        case 3: return v < 0.018f ? 4.5f * v : 1.099f * (sgn(v) * fpow(fabsf(v), 0.45f)) - 0.099f;
         default: return v;
     }
 }
@@ -266,12 +268,79 @@ __global__ __launch_bounds__(256) void k_unsharp_encode(const float* __restrict_
     reinterpret_cast<ushort4*>(out)[size_t(y) * stride + x] = px;
 }
 
+// The scanner unsharp mask with its blur (radius R <= kFir2dMaxR) computed in LDS per 64 x 32
+// tile (fir_tile.h), fused with the combine, matrix, transfer function and encode: the print
+// planes are read once, RGBA16 written once.
+constexpr int kUTX = 64, kUTY = 32, kUNT = 256, kUPer = kUTX * kUTY / kUNT;
+
+template <int R>
+__global__ __launch_bounds__(kUNT) void k_unsharp_encode2d(const float* __restrict__ rgb, uint16_t* __restrict__ out,
+                                                          uint32_t stride, const PrintConsts* __restrict__ pcp,
+                                                          FirArgs f, uint32_t w, uint32_t h) {
+    __shared__ FirTile<R, kUTX, kUTY> t;
+    const PrintConsts& q = *pcp;
+    const uint32_t gx = (w + kUTX - 1) / kUTX, gy = (h + kUTY - 1) / kUTY;
+    const uint32_t item = xcd_item(blockIdx.x, gx * gy);
+    if (item >= gx * gy) return;
+    const int x0 = int(item % gx) * kUTX, y0 = int(item / gx) * kUTY;
+    const size_t np = size_t(w) * h;
+    float res[3][kUPer];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        fir_tile_load<R, kUTX, kUTY, kUNT>(t, rgb + c * np, int(w), int(h), x0, y0);
+        __syncthreads();
+        fir_tile_v<R, kUTX, kUTY, kUNT>(t, f.taps);
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kUPer; ++u) {
+            const int e = int(threadIdx.x) + u * kUNT;
+            const int row = e / kUTX, col = e - row * kUTX;
+            const float g = fir_tile_h<R, kUTX, kUTY>(t, f.taps, row, col);
+            res[c][u] = q.unsharp_a * fir_tile_centre<R, kUTX, kUTY>(t, row, col) + q.unsharp_b * g;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < kUPer; ++u) {
+        const int e = int(threadIdx.x) + u * kUNT;
+        const int row = e / kUTX, col = e - row * kUTX;
+        const uint32_t x = uint32_t(x0 + col), y = uint32_t(y0 + row);
+        if (x >= w || y >= h) continue;
+        F3 v{res[0][u], res[1][u], res[2][u]};
+        if (q.cctf_active) {
+            const F3 m = mat_apply(q.m_nid, v);
+            v = {cctf_encode(m.x, q.cctf_mode), cctf_encode(m.y, q.cctf_mode), cctf_encode(m.z, q.cctf_mode)};
+        }
+        ushort4 px;
+        px.x = encode16(v.x);
+        px.y = encode16(v.y);
+        px.z = encode16(v.z);
+        px.w = 65535;
+        reinterpret_cast<ushort4*>(out)[size_t(y) * stride + x] = px;
+    }
+}
+
 }  // namespace
 
 void print_pointwise(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea, const float* paper,
                      const float* cmax, uint32_t w, uint32_t h, hipStream_t s) {
     const uint32_t n = w * h;
     hipLaunchKernelGGL(k_print, dim3((n + 255) / 256), dim3(256), 0, s, neg, rgb, pc, ea, paper, cmax, w, h);
+}
+
+void unsharp_encode2d(const float* rgb, uint16_t* out, uint32_t stride, const PrintConsts* pc, const FirArgs& f,
+                      uint32_t w, uint32_t h, hipStream_t s) {
+    const dim3 grid(xcd_grid(((w + kUTX - 1) / kUTX) * ((h + kUTY - 1) / kUTY))), block(kUNT);
+    switch (f.r) {
+        case 1: hipLaunchKernelGGL(k_unsharp_encode2d<1>, grid, block, 0, s, rgb, out, stride, pc, f, w, h); break;
+        case 2: hipLaunchKernelGGL(k_unsharp_encode2d<2>, grid, block, 0, s, rgb, out, stride, pc, f, w, h); break;
+        case 3: hipLaunchKernelGGL(k_unsharp_encode2d<3>, grid, block, 0, s, rgb, out, stride, pc, f, w, h); break;
+        case 4: hipLaunchKernelGGL(k_unsharp_encode2d<4>, grid, block, 0, s, rgb, out, stride, pc, f, w, h); break;
+        case 5: hipLaunchKernelGGL(k_unsharp_encode2d<5>, grid, block, 0, s, rgb, out, stride, pc, f, w, h); break;
+        case 6: hipLaunchKernelGGL(k_unsharp_encode2d<6>, grid, block, 0, s, rgb, out, stride, pc, f, w, h); break;
+        case 7: hipLaunchKernelGGL(k_unsharp_encode2d<7>, grid, block, 0, s, rgb, out, stride, pc, f, w, h); break;
+        default: hipLaunchKernelGGL(k_unsharp_encode2d<8>, grid, block, 0, s, rgb, out, stride, pc, f, w, h); break;
+    }
 }
 
 void unsharp_encode(const float* rgb, const float* blurred, uint16_t* out, uint32_t stride, const PrintConsts* pc,

@@ -68,7 +68,8 @@ def test_gaussian_matches_oracle(engine, oracle, sigma, crossover):
             assert d.max() <= 2, (sigma, c, d.max())
 
 
-@pytest.mark.parametrize("shape", [(40, 1), (1, 40), (1, 1), (3, 2), (4500, 37), (37, 4500), (700, 1300)])
+@pytest.mark.parametrize("shape", [(40, 1), (1, 40), (1, 1), (3, 2), (4500, 37), (37, 4500), (700, 1300),
+                                   (2, 6001), (3, 9000), (5000, 3)])
 @pytest.mark.parametrize("sigma", [(1.5, 1.5, 1.5), (6.0, 6.0, 6.0), (24.0, 3.0, 100.0)])
 def test_gaussian_shapes(engine, oracle, shape, sigma):
     rng = np.random.default_rng(11)
