@@ -61,6 +61,10 @@ SPK_API spk_status spkx_poisson(spk_engine* engine, const float* lam, uint32_t n
                                 float* out);
 SPK_API spk_status spkx_gaussian(spk_engine* engine, const float* in, float* out, uint32_t width, uint32_t height,
                                  const double* sigma3, double fir_crossover);
+/* The tier downscale (3-channel interleaved host image -> long edge `long_edge`, 0 = none): writes
+ * *out_w, *out_h always and the image when out is not NULL and cap_floats allows. */
+SPK_API spk_status spkx_downscale(spk_engine* engine, const float* in, uint32_t width, uint32_t height,
+                                  uint32_t long_edge, float* out, size_t cap_floats, uint32_t* out_w, uint32_t* out_h);
 
 #ifdef __cplusplus
 }

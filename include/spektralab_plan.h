@@ -24,7 +24,7 @@ extern "C" {
 #endif
 
 #define SPK_PLAN_MAGIC 0x504B5053u /* "SPKP" */
-#define SPK_PLAN_VERSION 1u
+#define SPK_PLAN_VERSION 2u
 
 #define SPK_NL 81        /* wavelength samples 380..780 nm, 5 nm */
 #define SPK_NL3 243      /* [lambda][channel] */
@@ -59,7 +59,10 @@ typedef struct spk_plan {
     float a[9];            /* input matrix A (RGB -> XYZ adapted), row-major */
     double a64[9];
     float ec_gain;         /* f32(2^exposure_compensation_ev); multiply skipped when 1 */
+    uint32_t ae_active;    /* auto exposure: the tier image is multiplied by ae_gain before the upsample */
+    float ae_gain;         /* f32(2^ae_ev) */
     uint32_t pad0;
+    double ae_ev;          /* the metered EV of the session's auto_exposure_method */
 
     uint32_t scatter_active, n_scatter;
     spk_mix_component scatter[4];
@@ -67,6 +70,14 @@ typedef struct spk_plan {
     uint32_t bounce_active, n_bounce;
     spk_mix_component bounce[SPK_PLAN_MAX_MIX];
     double mix_a[3], mix_b[3];             /* raw <- a raw + b H(raw) */
+
+    /* DIR couplers: chi := M^T s(D), s = D (or dmax - D on a positive); chi := the DIR diffusion
+     * mixture of chi; D := F(L - chi; y0) */
+    uint32_t couplers_active, coupler_positive;
+    float coupler_m[9];                    /* M[c][m], row-major, donor c, receiver m */
+    float coupler_dmax[3];
+    uint32_t n_dir;
+    spk_mix_component dir[4];              /* core + exponential-surrogate tail */
 
     uint32_t grain_active;  /* draws happen */
     uint32_t grain_mode;    /* SPK_GRAIN_* */
@@ -91,6 +102,14 @@ typedef struct spk_plan {
     float to_rgb[9];
     double to_rgb64[9];
 
+    /* glare (after the scan, before to_rgb): F := lognormal(mean, std) per pixel on stream 200 with
+     * glare_seed; F := G_sigma(F); XYZ_c += (F / 100) tint_c */
+    uint32_t glare_active, glare_seed;
+    float glare_mean, glare_std;
+    double glare_sigma;
+    float glare_tint[3];
+    uint32_t pad1;
+
     /* CAM16-UCS gamut compression (kernels/print_shared.h:69-95) */
     uint32_t gamut_active;
     uint32_t lc_active;
@@ -111,7 +130,7 @@ typedef struct spk_plan {
     float table[SPK_PLAN_MAX_L * SPK_PLAN_MAX_L * 3];      /* L x L x 3 [i][j][c] */
     double table64[SPK_PLAN_MAX_L * SPK_PLAN_MAX_L * 3];
     float curves[8 * SPK_PLAN_MAX_K];                      /* [x K][inv K][y 3K][y0 3K], y channel-major */
-    double curves_x64[SPK_PLAN_MAX_K], curves_y64[3 * SPK_PLAN_MAX_K];
+    double curves_x64[SPK_PLAN_MAX_K], curves_y64[3 * SPK_PLAN_MAX_K], curves_y064[3 * SPK_PLAN_MAX_K];
     float grain_axis[3 * SPK_PLAN_MAX_K], grain_inv[3 * SPK_PLAN_MAX_K], grain_layers[9 * SPK_PLAN_MAX_K];
     float paper[5 * SPK_PLAN_MAX_K];                       /* x[K], r[K], y[K][3] */
     double paper_x64[SPK_PLAN_MAX_K], paper_y64[3 * SPK_PLAN_MAX_K];

@@ -21,12 +21,17 @@
 //   print                kernels/print.metal:165-198 (spectral, gain), :201-232 (paper),
 //                        :249-258 (matrix), :31-127 + :267-317 (CAM16-UCS gamut), :136-156
 //                        (transfer, 16-bit), :350-366 (encode); src/fx/scanner.cpp:16-33 (unsharp)
+//   auto exposure gain   src/film/develop.cpp:125-131 (the tier image scaled before the upsample)
+//   DIR couplers         src/film/develop.cpp:182-222, kernels/film.metal:264-292, src/fx/dir_blur.cpp
+//   glare                src/fx/glare.cpp:22-51, kernels/fx_random.h:100-115, kernels/fx_glare.metal
+//   tier downscale       src/session/tiers.cpp:48-132, :185-227, kernels/session.metal
 // Arithmetic conventions (binary32 mode): every float operation rounds as written (compiled with
 // -ffp-contract=off); FIR taps accumulate with fused multiply-add in ascending order (the HIP
 // kernels do the same); film-side log10 and the sampler's log / exp use the shared reproducible
 // routines of spk_math.h; print-side transcendentals are the C library's.
 #include <omp.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -189,6 +194,47 @@ void blur3(const T* in, T* out, const double sigma[3], int mode, const T* acc, c
         } else {
             iir_plane(ic, oc, ac, mode, wt, s, w, h);
         }
+    }
+}
+
+// ---- tier downscale (src/session/tiers.cpp:48-55, :63-132, :185-227) ---------------------------
+int64_t mirror_fold(int64_t i, int64_t n) {  // whole-sample "c b | a b c d | c b", period 2(n-1)
+    if (n <= 1) return 0;
+    const int64_t period = 2 * (n - 1);
+    int64_t m = i % period;
+    if (m < 0) m += period;
+    return m < n ? m : period - m;
+}
+
+std::vector<float> ds_taps(uint32_t n, uint32_t m) {
+    const double sigma = std::max(0.0, (double(n) / double(m) - 1.0) / 2.0);
+    const int r = sigma > 0.0 ? int(std::floor(4.0 * sigma + 0.5)) : 0;
+    if (r == 0) return {1.0f};
+    std::vector<double> t(size_t(2 * r + 1));
+    double sum = 0.0;
+    for (int k = -r; k <= r; ++k) {
+        const double u = double(k) / sigma;
+        t[size_t(k + r)] = std::exp(-0.5 * u * u);
+        sum += t[size_t(k + r)];
+    }
+    std::vector<float> f(t.size());
+    for (size_t i = 0; i < t.size(); ++i) f[i] = float(t[i] / sum);
+    return f;
+}
+
+void ds_samples(uint32_t n, uint32_t m, std::vector<uint32_t>& idx, std::vector<float>& tt) {
+    idx.resize(2 * size_t(m));
+    tt.resize(m);
+    const int64_t N = n, M = m, den = 2 * M;
+    for (int64_t o = 0; o < M; ++o) {
+        int64_t num = (2 * o + 1) * N - M;
+        if (num > den * (N - 1)) num = 2 * den * (N - 1) - num;
+        if (num < 0) num = -num;
+        int64_t i0 = num / den;
+        if (i0 > N - 1) i0 = N - 1;
+        idx[size_t(2 * o)] = uint32_t(i0);
+        idx[size_t(2 * o + 1)] = uint32_t(std::min<int64_t>(i0 + 1, N - 1));
+        tt[size_t(o)] = float(num - i0 * den) / float(den);
     }
 }
 
@@ -396,10 +442,17 @@ void develop(const spk_plan& p, const float* rgb, T* neg) {
     for (int i = 0; i < 9; ++i) a[i] = sizeof(T) == 4 ? T(p.a[i]) : T(p.a64[i]);
     const T ec = sizeof(T) == 4 ? T(p.ec_gain) : T(double(p.ec_gain));
     const bool ec_active = p.ec_gain != 1.0f;
+    // auto exposure: the tier image scaled by f32(2^ev) before the upsample (develop.cpp:125-131)
+    const T ae = sizeof(T) == 4 ? T(p.ae_gain) : T(std::exp2(p.ae_ev));
     const uint32_t L = p.L;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < int64_t(np); ++i) {
-        const T r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+        T r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+        if (p.ae_active) {
+            r = r * ae;
+            g = g * ae;
+            b = b * ae;
+        }
         const T X = (a[0] * r + a[1] * g) + a[2] * b;
         const T Y = (a[3] * r + a[4] * g) + a[5] * b;
         const T Z = (a[6] * r + a[7] * g) + a[8] * b;
@@ -460,16 +513,52 @@ void develop(const spk_plan& p, const float* rgb, T* neg) {
                                  : (k + 1 < K && p.curves_x64[k + 1] != p.curves_x64[k] ? T(1.0 / (p.curves_x64[k + 1] - p.curves_x64[k])) : T(0));
     }
     for (uint32_t i = 0; i < 3 * K; ++i) cy[i] = sizeof(T) == 4 ? T(p.curves[2 * K + i]) : T(p.curves_y64[i]);
+    // DIR couplers (develop.cpp:182-222, film.metal:264-292): L and the first development D1;
+    // chi_m := sum_c s_c M[c][m], s = D1 (dmax - D1 on a positive); chi diffused by the DIR
+    // mixture; D := F(L - chi; y0)
+    std::vector<T> Lc, chib;
+    std::vector<T> cy0;
+    if (p.couplers_active) {
+        cy0.resize(3 * K);
+        for (uint32_t i = 0; i < 3 * K; ++i) cy0[i] = sizeof(T) == 4 ? T(p.curves[5 * K + i]) : T(p.curves_y064[i]);
+        Lc.resize(3 * np);
+        std::vector<T> chi(3 * np);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < int64_t(np); ++i) {
+            T s[3];
+            for (int c = 0; c < 3; ++c) {
+                const size_t o = size_t(c) * np + size_t(i);
+                T v = H[o];
+                if (p.bounce_active) v = T(float(p.mix_a[c])) * v + T(float(p.mix_b[c])) * acc[o];
+                const T Lg = o_log10(std::fmax(v, T(0)) + T(1e-10));
+                Lc[o] = Lg;
+                const T D1 = curve_lookup(cx.data(), cinv.data(), cy.data() + c * K, K, Lg);
+                s[c] = p.coupler_positive ? T(p.coupler_dmax[c]) - D1 : D1;
+            }
+            for (int m = 0; m < 3; ++m)
+                chi[size_t(m) * np + size_t(i)] =
+                    (s[0] * T(p.coupler_m[m]) + s[1] * T(p.coupler_m[3 + m])) + s[2] * T(p.coupler_m[6 + m]);
+        }
+        chib.assign(3 * np, T(0));
+        for (uint32_t k = 0; k < p.n_dir; ++k)
+            blur3<T>(chi.data(), chib.data(), p.dir[k].sigma, k == 0 ? 1 : 2, chib.data(), p.dir[k].weight, w, h,
+                     p.fir_crossover);
+    }
     std::vector<T> G(p.grain_active ? 3 * np : 0), D(p.grain_active ? 3 * np : 0);
     for (int c = 0; c < 3; ++c) {
         const T ma = T(float(p.mix_a[c])), mb = T(float(p.mix_b[c]));
 #pragma omp parallel for schedule(dynamic, 4096)
         for (int64_t i = 0; i < int64_t(np); ++i) {
             const size_t o = size_t(c) * np + size_t(i);
-            T v = H[o];
-            if (p.bounce_active) v = ma * v + mb * acc[o];
-            const T Lg = o_log10(std::fmax(v, T(0)) + T(1e-10));
-            const T Dv = curve_lookup(cx.data(), cinv.data(), cy.data() + c * K, K, Lg);
+            T Dv;
+            if (p.couplers_active) {
+                Dv = curve_lookup(cx.data(), cinv.data(), cy0.data() + c * K, K, Lc[o] - chib[o]);
+            } else {
+                T v = H[o];
+                if (p.bounce_active) v = ma * v + mb * acc[o];
+                const T Lg = o_log10(std::fmax(v, T(0)) + T(1e-10));
+                Dv = curve_lookup(cx.data(), cinv.data(), cy.data() + c * K, K, Lg);
+            }
             if (!p.grain_active) {
                 neg[o] = Dv;
             } else {
@@ -700,6 +789,28 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
     q.hundred_over_f_l = T(100) / q.f_l;
     const int rows = SPK_CMAX_ROWS, cols = SPK_CMAX_COLS;
     const T L2 = K<T>::log2_10, tiny = K<T>::tiny;
+    // glare (fx/glare.cpp:22-51, fx_random.h:100-115, fx_glare.metal:13-22): one lognormal value per
+    // pixel from words x, y of block 0 on stream 200, blurred, added as (F / 100) tint to XYZ
+    std::vector<T> glare;
+    if (p.glare_active) {
+        std::vector<T> f3(3 * np), b3(3 * np);
+        const T m = T(p.glare_mean), sd = T(p.glare_std);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < int64_t(np); ++i) {
+            Stream gs{uint32_t(i), 200u, p.glare_seed};
+            const float u1 = gs.uniform(), u2 = gs.uniform();
+            const T z = std::sqrt(T(-2) * std::log(T(u1))) * std::cos(T(6.283185307179586f) * T(u2));
+            const T sig = std::sqrt(std::log(T(1) + (sd * sd) / (m * m)));
+            const T mu = std::log(m) - (T(0.5) * sig) * sig;
+            const T v = std::exp(mu + sig * z);
+            f3[i] = v;
+            f3[np + i] = v;
+            f3[2 * np + i] = v;
+        }
+        const double sg[3] = {p.glare_sigma, p.glare_sigma, p.glare_sigma}, one[3] = {1, 1, 1};
+        blur3<T>(f3.data(), b3.data(), sg, 0, nullptr, one, w, h, p.fir_crossover);
+        glare.assign(b3.begin(), b3.begin() + np);
+    }
     std::vector<T> rgb(3 * np);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < int64_t(np); ++i) {
@@ -737,6 +848,8 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
             xyz[1] = std::fmax(r1, T(0)) + tiny;
             xyz[2] = std::fmax(r2, T(0)) + tiny;
         }
+        if (p.glare_active)
+            for (int c = 0; c < 3; ++c) xyz[c] = xyz[c] + (glare[i] / T(100)) * T(p.glare_tint[c]);
         T c3[3];
         mat_apply(to_rgb, xyz, c3);
         if (p.gamut_active) {
@@ -811,6 +924,72 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
 
 extern "C" {
 
+// Tier downscale of an interleaved w x h x 3 image to the long edge L; returns the output size.
+void oracle_downscale(const float* src, uint32_t w, uint32_t h, uint32_t L, float* dst, uint32_t* ow, uint32_t* oh) {
+    const uint32_t M = std::max(w, h);
+    uint32_t W = w, Hh = h;
+    if (L != 0 && M > L) {
+        const double s = double(L) / double(M);
+        W = std::max<uint32_t>(1, uint32_t(std::floor(double(w) * s)));
+        Hh = std::max<uint32_t>(1, uint32_t(std::floor(double(h) * s)));
+    }
+    *ow = W;
+    *oh = Hh;
+    if (!dst) return;
+    if (W == w && Hh == h) {
+        std::memcpy(dst, src, size_t(3) * w * h * sizeof(float));
+        return;
+    }
+    const std::vector<float> ty = ds_taps(h, Hh), tx = ds_taps(w, W);
+    std::vector<float> a(src, src + size_t(3) * w * h), b(a.size());
+    if (ty.size() > 1) {
+        const int r = int(ty.size() / 2);
+#pragma omp parallel for schedule(static)
+        for (int64_t y = 0; y < int64_t(h); ++y)
+            for (uint32_t x = 0; x < w; ++x) {
+                float acc[3] = {0, 0, 0};
+                for (int k = -r; k <= r; ++k) {
+                    const int64_t yy = mirror_fold(y + k, h);
+                    const float* q = &a[3 * (size_t(yy) * w + x)];
+                    for (int c = 0; c < 3; ++c) acc[c] = std::fma(ty[size_t(k + r)], q[c], acc[c]);
+                }
+                for (int c = 0; c < 3; ++c) b[3 * (size_t(y) * w + x) + c] = acc[c];
+            }
+        a.swap(b);
+    }
+    if (tx.size() > 1) {
+        const int r = int(tx.size() / 2);
+#pragma omp parallel for schedule(static)
+        for (int64_t y = 0; y < int64_t(h); ++y)
+            for (uint32_t x = 0; x < w; ++x) {
+                float acc[3] = {0, 0, 0};
+                for (int k = -r; k <= r; ++k) {
+                    const int64_t xx = mirror_fold(int64_t(x) + k, w);
+                    const float* q = &a[3 * (size_t(y) * w + size_t(xx))];
+                    for (int c = 0; c < 3; ++c) acc[c] = std::fma(tx[size_t(k + r)], q[c], acc[c]);
+                }
+                for (int c = 0; c < 3; ++c) b[3 * (size_t(y) * w + x) + c] = acc[c];
+            }
+        a.swap(b);
+    }
+    std::vector<uint32_t> xi, yi;
+    std::vector<float> xt, yt;
+    ds_samples(w, W, xi, xt);
+    ds_samples(h, Hh, yi, yt);
+    for (uint32_t y = 0; y < Hh; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            const float* pa = &a[3 * (size_t(yi[2 * y]) * w + xi[2 * x])];
+            const float* pb = &a[3 * (size_t(yi[2 * y]) * w + xi[2 * x + 1])];
+            const float* pd = &a[3 * (size_t(yi[2 * y + 1]) * w + xi[2 * x])];
+            const float* pe = &a[3 * (size_t(yi[2 * y + 1]) * w + xi[2 * x + 1])];
+            for (int c = 0; c < 3; ++c) {
+                const float top = std::fma(xt[x], pb[c] - pa[c], pa[c]);
+                const float bot = std::fma(xt[x], pe[c] - pd[c], pd[c]);
+                dst[3 * (size_t(y) * W + x) + c] = std::fma(yt[y], bot - top, top);
+            }
+        }
+}
+
 // negative: 3 planar planes (w*h*3).
 void oracle_develop_f32(const spk_plan* p, const float* rgb, float* negative) { develop<float>(*p, rgb, negative); }
 void oracle_develop_f64(const spk_plan* p, const float* rgb, double* negative) { develop<double>(*p, rgb, negative); }
@@ -869,6 +1048,7 @@ long oracle_plan_offset(const char* name) {
     F(width) F(height) F(pitch_um) F(rho_tier) F(fir_crossover) F(L) F(K) F(a) F(a64) F(ec_gain)
     F(scatter_active) F(n_scatter) F(scatter) F(bounce_active) F(n_bounce) F(bounce) F(mix_a) F(mix_b)
     F(grain_active) F(grain_mode) F(grain_seed) F(grain_k) F(dmin_l) F(sat_d) F(count) F(uniformity) F(dmin_c)
+    F(ae_active) F(ae_gain) F(ae_ev) F(couplers_active) F(n_dir) F(dir) F(glare_active) F(glare_seed) F(glare_sigma)
     F(grain_final_blur) F(grain_amount) F(grain_axis) F(grain_inv) F(grain_layers) F(printed) F(paper_k) F(enl_eps64) F(enl_beta64) F(enl_wt64) F(enl_g64)
     F(enl_o64) F(k_gain) F(k_gain64) F(scan_eps64) F(scan_beta64) F(scan_wt64) F(to_rgb64) F(cam64) F(m16_inv64)
     F(m_to_xyz64) F(m_to_rgb64) F(unsharp_sigma) F(unsharp_amount) F(cctf_active) F(cctf_mode) F(m_nid64)

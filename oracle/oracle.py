@@ -44,6 +44,8 @@ def lib():
         l.oracle_fir_radius.argtypes = [C.c_double]
         l.oracle_yvv.argtypes = [C.c_double, D]
         l.oracle_reflect.argtypes = [C.c_int, C.c_int]
+        l.oracle_downscale.argtypes = [F, C.c_uint32, C.c_uint32, C.c_uint32, F, U, U]
+        l.oracle_downscale.restype = None
         l.oracle_plan_offset.argtypes = [C.c_char_p]
         l.oracle_plan_offset.restype = C.c_long
         for n in ("oracle_log10f", "oracle_logf", "oracle_expf"):
@@ -65,7 +67,8 @@ _DT = {"width": "u4", "height": "u4", "L": "u4", "K": "u4", "scatter_active": "u
        "grain_k": "u4", "printed": "u4", "paper_k": "u4", "cctf_active": "u4", "cctf_mode": "u4",
        "a": "f4", "ec_gain": "f4", "dmin_l": "f4", "sat_d": "f4", "count": "f4", "uniformity": "f4",
        "dmin_c": "f4", "k_gain": "f4", "table": "f4", "curves": "f4", "paper": "f4", "cmax": "f4",
-       "grain_axis": "f4", "grain_inv": "f4", "grain_layers": "f4"}
+       "grain_axis": "f4", "grain_inv": "f4", "grain_layers": "f4", "ae_active": "u4", "ae_gain": "f4",
+       "couplers_active": "u4", "glare_active": "u4", "glare_seed": "u4", "n_dir": "u4"}
 
 
 def field(plan, name, count=1):
@@ -125,6 +128,17 @@ def gaussian(image, sigma3, crossover=3.0):
     out = np.zeros_like(img)
     s = (C.c_double * 3)(*[float(v) for v in sigma3])
     lib().oracle_gaussian_f32(_p(img, C.c_float), _p(out, C.c_float), w, h, s, crossover)
+    return out
+
+
+def downscale(image, long_edge):
+    """Tier downscale of an H x W x 3 float32 image to the long edge (the oracle's restatement)."""
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    h, w, _ = img.shape
+    ow, oh = C.c_uint32(), C.c_uint32()
+    lib().oracle_downscale(_p(img, C.c_float), w, h, long_edge, None, C.byref(ow), C.byref(oh))
+    out = np.zeros((oh.value, ow.value, 3), np.float32)
+    lib().oracle_downscale(_p(img, C.c_float), w, h, long_edge, _p(out, C.c_float), C.byref(ow), C.byref(oh))
     return out
 
 

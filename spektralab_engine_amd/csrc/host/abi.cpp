@@ -11,6 +11,7 @@
 
 #include "../kernels/kernels.h"
 #include "engine.h"
+#include "tiers.h"
 #include "spektralab.h"
 #include "spektralab_ext.h"
 #include "util.h"
@@ -525,6 +526,32 @@ SPK_API spk_status spkx_gaussian(spk_engine* engine, const float* in, float* out
         spk::gpu::check(hipMemcpy(planar.data(), b->ptr, 3 * np * 4, hipMemcpyDeviceToHost), "hipMemcpy");
         for (size_t i = 0; i < np; ++i)
             for (int c = 0; c < 3; ++c) out[3 * i + size_t(c)] = planar[size_t(c) * np + i];
+        return SPK_OK;
+    });
+}
+
+SPK_API spk_status spkx_downscale(spk_engine* engine, const float* in, uint32_t width, uint32_t height,
+                                  uint32_t long_edge, float* out, size_t cap_floats, uint32_t* out_w, uint32_t* out_h) {
+    le::clear();
+    if (!engine || !in || !out_w || !out_h) return invalid("a required argument is NULL");
+    if (width == 0 || height == 0) return invalid("empty image");
+    return guarded([&] {
+        Engine& e = *E(engine);
+        spk::gpu::DeviceScope ds(e.device());
+        const spk::tiers::Size o = spk::tiers::downscale_size(width, height, long_edge);
+        *out_w = o.w;
+        *out_h = o.h;
+        if (!out) return SPK_OK;
+        const size_t n_out = size_t(o.w) * o.h * 3;
+        if (cap_floats < n_out) return invalid("output too small");
+        const size_t n_in = size_t(width) * height * 3;
+        spk::gpu::DevBuf src = e.pool().acquire(n_in * 4);
+        spk::gpu::check(hipMemcpy(src->ptr, in, n_in * 4, hipMemcpyHostToDevice), "hipMemcpy");
+        spk::tiers::Size got;
+        spk::gpu::DevBuf dst = spk::tiers::downscale(static_cast<const float*>(src->ptr), width, height, long_edge,
+                                                     e.pool(), nullptr, &got);
+        spk::gpu::check(hipDeviceSynchronize(), "downscale");
+        spk::gpu::check(hipMemcpy(out, dst ? dst->ptr : src->ptr, n_out * 4, hipMemcpyDeviceToHost), "hipMemcpy");
         return SPK_OK;
     });
 }

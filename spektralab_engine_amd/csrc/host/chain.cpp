@@ -156,6 +156,28 @@ void iir_planes(const float* in, float* out, double sigma, uint32_t mode, const 
     }
 }
 
+// One plane with the per-channel branch rule of blur3.
+void blur1(const float* in, float* out, double sigma, double crossover, uint32_t w, uint32_t h, gpu::Pool& pool,
+           hipStream_t s, std::vector<gpu::DevBuf>& keep) {
+    const size_t np = size_t(w) * h;
+    if (!(sigma > 0.0) || (sigma < crossover && setup::fir_radius(sigma) == 0)) {
+        k::copy(in, out, np, s);
+        return;
+    }
+    gpu::DevBuf tmp = pool.acquire(np * sizeof(float));
+    keep.push_back(tmp);
+    if (sigma < crossover) {
+        fir_planes(in, out, sigma, 0, nullptr, 1.0, w, h, 1, gpu::as<float>(tmp), s);
+        return;
+    }
+    gpu::DevBuf tmp2;
+    if (!k::iir_col_chunk(h)) {
+        tmp2 = pool.acquire(np * sizeof(float));
+        keep.push_back(tmp2);
+    }
+    iir_planes(in, out, sigma, 0, nullptr, 1.0, w, h, 1, gpu::as<float>(tmp), tmp2 ? gpu::as<float>(tmp2) : nullptr, s);
+}
+
 }  // namespace
 
 // Per-channel branch (reference src/fx/gaussian.cpp:19-22): identity for sigma <= 0 or a zero FIR
@@ -342,6 +364,8 @@ void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negati
     std::memcpy(ua.a, p.a, sizeof ua.a);
     ua.ec_gain = p.ec_gain;
     ua.ec_active = p.ec_gain != 1.0f ? 1u : 0u;
+    ua.pre_gain = p.ae_gain;
+    ua.pre_active = p.ae_active;
     k::upsample(rgb, H, gpu::as<float>(t.table), ua, s);
     mark("upsample");
 
@@ -381,6 +405,30 @@ void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negati
     std::memcpy(da.uniformity, p.uniformity, sizeof da.uniformity);
     std::memcpy(da.dmin_c, p.dmin_c, sizeof da.dmin_c);
 
+    if (p.couplers_active) {
+        // DIR couplers (develop.cpp:182-222): stage 1 writes log exposure L and the coupler
+        // source chi; chi is diffused by the DIR mixture; stage 2 develops F(L - chi; y0).
+        gpu::DevBuf Lb = pool.acquire(3 * np * sizeof(float)), chib = pool.acquire(3 * np * sizeof(float));
+        gpu::DevBuf chibl = pool.acquire(3 * np * sizeof(float));
+        keep.push_back(Lb);
+        keep.push_back(chib);
+        keep.push_back(chibl);
+        k::DevelopArgs d1 = da;
+        d1.coupler_stage = 1;
+        d1.grain_active = 0;
+        d1.write_density = 0;
+        d1.coupler_positive = p.coupler_positive;
+        std::memcpy(d1.coupler_m, p.coupler_m, sizeof d1.coupler_m);
+        std::memcpy(d1.coupler_dmax, p.coupler_dmax, sizeof d1.coupler_dmax);
+        k::develop(H, acc, gpu::as<float>(Lb), gpu::as<float>(chib), gpu::as<float>(t.curves), nullptr, nullptr, nullptr,
+                   nullptr, d1, s);
+        mixture(gpu::as<float>(chib), gpu::as<float>(chibl), p.dir, p.n_dir, w, h, p.fir_crossover, pool, s, keep);
+        mark("couplers");
+        H = gpu::as<float>(Lb);
+        acc = gpu::as<float>(chibl);
+        da.coupler_stage = 2;
+        da.mix_active = 0;
+    }
     if (!p.grain_active) {
         k::develop(H, acc, negative, nullptr, gpu::as<float>(t.curves), nullptr, nullptr, nullptr, nullptr, da, s);
         mark("develop");
@@ -430,10 +478,24 @@ void print(const spk_plan& p, const Tables& t, const float* negative, uint16_t* 
     ea.k_gain = p.k_gain;
     const k::PrintConsts* dq = gpu::as<const k::PrintConsts>(t.print_consts);
 
+    gpu::DevBuf glare;
+    if (p.glare_active) {
+        // The glare field does not depend on the image: drawn and blurred first, then added to the
+        // scanned XYZ inside the print kernel (print/run.cpp:158-161, fx/glare.cpp:22-51).
+        gpu::DevBuf raw = pool.acquire(np * sizeof(float));
+        glare = pool.acquire(np * sizeof(float));
+        keep.push_back(raw);
+        keep.push_back(glare);
+        k::glare_field(gpu::as<float>(raw), uint32_t(np), p.glare_mean, p.glare_std, p.glare_seed, s);
+        blur1(gpu::as<float>(raw), gpu::as<float>(glare), p.glare_sigma, p.fir_crossover, w, h, pool, s, keep);
+        for (int c = 0; c < 3; ++c) ea.glare_tint[c] = p.glare_tint[c];
+        mark("glare");
+    }
     gpu::DevBuf rgbb = pool.acquire(3 * np * sizeof(float));
     keep.push_back(rgbb);
     float* rgb = gpu::as<float>(rgbb);
-    k::print_pointwise(negative, rgb, dq, ea, gpu::as<float>(t.paper), gpu::as<float>(t.cmax), w, h, s);
+    k::print_pointwise(negative, rgb, dq, ea, gpu::as<float>(t.paper), gpu::as<float>(t.cmax),
+                       glare ? gpu::as<const float>(glare) : nullptr, w, h, s);
     mark("print");
     const double one[3] = {1.0, 1.0, 1.0};
     if (p.scanner_lens_blur_px > 0.0) {

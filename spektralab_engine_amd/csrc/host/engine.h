@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "chain.h"
+#include "tiers.h"
 #include "gpu.h"
 #include "params.h"
 #include "resources.h"
@@ -106,9 +107,19 @@ struct Pipeline {
 };
 
 struct TierState {
-    gpu::DevBuf negative;  // 3 planar planes
+    gpu::DevBuf negative;  // 3 planar planes of `size`
+    tiers::Size size;
     double pitch_um = 0.0;
     uint32_t film_seed = 0;
+    std::optional<double> ae_ev;  // the EV the negative was exposed with
+};
+
+// The tier's source image: the frame downscaled to the tier's long edge (src empty: the frame).
+struct TierImage {
+    gpu::DevBuf src;
+    tiers::Size size;
+    uint32_t long_edge = 0;
+    bool built = false;
 };
 
 class Session {
@@ -142,6 +153,10 @@ private:
                             std::shared_ptr<const res::Profile> print) const;
     void digest(params::ParamSet& p, const res::Profile& film) const;
     int tier_index(const char* tier) const;
+    uint32_t tier_long_edge(int t) const;
+    const TierImage& tier_image(int t, hipStream_t s);
+    // auto_exposure: the EV of the configured method (meters once, session.cpp:296-319)
+    std::optional<double> session_ev(hipStream_t s);
     // Develops (unless reusing) and prints into `out`; returns the plan's reprint flag.
     bool run(int tier, bool reprint, uint16_t* out, uint32_t stride, hipStream_t s, std::vector<gpu::DevBuf>& keep);
     void node(const char* label);
@@ -156,6 +171,8 @@ private:
     gpu::DevBuf source_;  // interleaved w x h x 3
     uint32_t w_ = 0, h_ = 0;
     TierState tiers_[3];
+    TierImage images_[3];
+    std::optional<tiers::MeterValues> meter_;
     std::unique_ptr<gpu::Stream> stream_;
     std::unique_ptr<spk_plan> plan_;
     bool have_plan_ = false;

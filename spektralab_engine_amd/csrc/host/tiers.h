@@ -1,0 +1,51 @@
+// Tiers, the tier downscale and the auto-exposure meter.
+//
+// Tier long edges: live = preview_long_edge, preview = 3400, full = the frame (reference
+// src/session/tiers.cpp:39-55).  The downscale (tiers.cpp:136-183) runs on the GPU from host-built
+// taps and exact-integer sampling tables; the meter (src/film/meter.cpp) is float64 on the host
+// over the 1600 px meter image.
+#pragma once
+
+#include <cstdint>
+#include <string_view>
+#include <vector>
+
+#include "gpu.h"
+
+namespace spk::tiers {
+
+inline constexpr uint32_t kPreviewLongEdge = 3400;
+inline constexpr uint32_t kMeterLongEdge = 1600;
+
+struct Size {
+    uint32_t w = 0, h = 0;
+};
+
+// floor of w, h scaled to the long edge (at least 1); unchanged when long_edge is 0 or not smaller.
+Size downscale_size(uint32_t w, uint32_t h, uint32_t long_edge);
+// rho_tier: the tier image's long edge over the frame's (never L / M: the floor can give L - 1).
+double tier_ratio(Size tier, Size full);
+
+// Per axis of n input and m output samples: Gaussian prefilter taps, sigma = max(0, (n/m - 1)/2),
+// truncate 4 (one tap {1} = no pass), and the bilinear sample tables.
+std::vector<float> axis_taps(uint32_t n, uint32_t m);
+struct AxisSamples {
+    std::vector<uint32_t> idx;  // 2 per output: i0, i1
+    std::vector<float> t;
+};
+AxisSamples axis_samples(uint32_t n, uint32_t m);
+
+// Downscales an interleaved w x h x 3 device image to downscale_size(w, h, long_edge) on stream s
+// (returns once enqueued; the result is valid after the stream reaches it).
+gpu::DevBuf downscale(const float* src, uint32_t w, uint32_t h, uint32_t long_edge, gpu::Pool& pool, hipStream_t s,
+                      Size* out);
+
+// The meter values of the auto-exposure methods (reference src/film/meter.cpp:17-122).
+struct MeterValues {
+    double center_weighted = 0, average = 0, median = 0, balanced = 0, center = 0, protect_highlights = 0,
+           protect_shadows = 0;
+    double ev(std::string_view method) const;
+};
+MeterValues meter_values(const float* rgb, uint32_t sw, uint32_t sh, const double rgb_to_xyz[9]);
+
+}  // namespace spk::tiers

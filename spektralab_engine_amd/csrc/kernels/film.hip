@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "philox.h"
 #include "spk_math.h"
 
 #pragma clang fp contract(off)
@@ -64,34 +65,6 @@ __device__ inline float curve_lookup(const float* __restrict__ x, const float* _
     const float t = (v - x[l]) * inv[l];
     return y[l] + t * (y[l + 1] - y[l]);
 }
-
-// ---- Philox4x32-10 (Salmon et al., SC 2011) ------------------------------------------------------
-// Ten rounds on counter (c0..c3) with key (k0, k1) (reference spk_common.h:23-34).
-__device__ inline uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const uint64_t p0 = uint64_t(0xD2511F53u) * c0;
-        const uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
-        const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
-        c1 = uint32_t(p1);
-        c3 = uint32_t(p0);
-        c0 = n0;
-        c2 = n2;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    return make_uint4(c0, c1, c2, c3);
-}
-
-// Block j of the (pixel, stream) generator: counter (pixel, stream, j, 0), key (seed, 0); its words
-// are consumed x, y, z, w before block j + 1 (reference spk_common.h:41-67).
-__device__ inline uint4 philox_block(uint32_t pixel, uint32_t stream, uint32_t j, uint32_t seed) {
-    return philox4x32_10(pixel, stream, j, 0u, seed, 0u);
-}
-
-// u = ((w >> 8) + 1/2) 2^-24 (reference spk_common.h:37); 1.0f is reachable.
-__device__ inline float u01(uint32_t w) { return (float(w >> 8) + 0.5f) * 0x1p-24f; }
 
 // Hormann's PTRS constants for one lam >= 10 (reference kernels/fx_random.h:52-58).
 struct Ptrs {
@@ -177,7 +150,12 @@ __global__ __launch_bounds__(256) void k_upsample(const float* __restrict__ rgb,
                                                   const float* __restrict__ table, UpsampleArgs q) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= q.n) return;
-    const float r = rgb[3 * size_t(i)], g = rgb[3 * size_t(i) + 1], b = rgb[3 * size_t(i) + 2];
+    float r = rgb[3 * size_t(i)], g = rgb[3 * size_t(i) + 1], b = rgb[3 * size_t(i) + 2];
+    if (q.pre_active) {  // the auto-exposure scale of the tier image (develop.cpp:125-131)
+        r = r * q.pre_gain;
+        g = g * q.pre_gain;
+        b = b * q.pre_gain;
+    }
     const float X = (q.a[0] * r + q.a[1] * g) + q.a[2] * b;
     const float Y = (q.a[3] * r + q.a[4] * g) + q.a[5] * b;
     const float Z = (q.a[6] * r + q.a[7] * g) + q.a[8] * b;
@@ -241,13 +219,25 @@ __global__ __launch_bounds__(256) void k_develop(const float* __restrict__ H, co
     const float* inv = curves + K;
     const float* y = curves + 2 * K;
     const float hl2p = spkm::half_log_2pi();
+    float s1[3];  // coupler stage 1: the coupler source of the first development
 #pragma unroll 1
     for (uint32_t c = 0; c < 3; ++c) {
         const size_t o = size_t(c) * n + i;
-        float v = H[o];
-        if (q.mix_active) v = q.mix_a[c] * v + q.mix_b[c] * acc[o];
-        const float L = spkm::log10f_(fmaxf(v, 0.0f) + 1e-10f);
-        const float D = curve_lookup(x, inv, y + c * K, K, L);
+        float D;
+        if (q.coupler_stage == 2u) {  // film.metal:282-292: F(L - chi; y0)
+            const float l0 = H[o] - acc[o];
+            D = curve_lookup(x, inv, curves + 5 * K + c * K, K, l0);
+        } else {
+            float v = H[o];
+            if (q.mix_active) v = q.mix_a[c] * v + q.mix_b[c] * acc[o];
+            const float L = spkm::log10f_(fmaxf(v, 0.0f) + 1e-10f);
+            D = curve_lookup(x, inv, y + c * K, K, L);
+            if (q.coupler_stage == 1u) {  // film.metal:264-278
+                out[o] = L;
+                s1[c] = q.coupler_positive ? q.coupler_dmax[c] - D : D;
+                continue;
+            }
+        }
         if (q.write_density) density[o] = D;
         if (!q.grain_active) {
             out[o] = D;
@@ -285,6 +275,11 @@ __global__ __launch_bounds__(256) void k_develop(const float* __restrict__ H, co
         float G = ((0.0f + val[0]) + val[1]) + val[2];
         G = G * 1.0f + (-q.dmin_c[c]);
         out[o] = G;
+    }
+    if (q.coupler_stage == 1u) {
+#pragma unroll
+        for (uint32_t m = 0; m < 3; ++m)
+            density[size_t(m) * n + i] = (s1[0] * q.coupler_m[m] + s1[1] * q.coupler_m[3 + m]) + s1[2] * q.coupler_m[6 + m];
     }
 }
 

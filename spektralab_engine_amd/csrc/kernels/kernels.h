@@ -46,6 +46,8 @@ struct UpsampleArgs {
     float a[9];
     float ec_gain;
     uint32_t ec_active;
+    float pre_gain;           // auto exposure: the input is multiplied by pre_gain first
+    uint32_t pre_active;
 };
 
 // The fused develop kernel: halation mix, log10, density curves, grain draw.
@@ -57,6 +59,11 @@ struct DevelopArgs {
     uint32_t write_density;   // also write the pre-grain density (grain_amount != 1)
     uint32_t seed, positive, gk;
     float dmin_l[9], sat_d[9], count[9], uniformity[3], dmin_c[3];
+    // DIR couplers: stage 1 writes L to out and chi := M^T s(D) to density; stage 2 reads L from H
+    // and the blurred chi from acc and develops D := F(L - chi; y0) (then grain as usual).
+    uint32_t coupler_stage;   // 0: no couplers
+    uint32_t coupler_positive;
+    float coupler_m[9], coupler_dmax[3];
 };
 
 // Per-run constants of the enlarger (they follow the live print fields), passed by value.
@@ -64,6 +71,7 @@ struct DevelopArgs {
 struct EnlargerArgs {
     float eps[SPK_NL3], beta[SPK_NL], wt[SPK_NL3], g[3], o[3];
     float k_gain;
+    float glare_tint[3];      // XYZ += (F / 100) tint with the glare field F when it is given
 };
 
 // Pipeline constants of the print side, device-resident (uploaded with the pipeline's tables).
@@ -106,8 +114,11 @@ void lincomb(const float* x, const float* y, float* out, uint32_t n, float a, fl
 void copy(const float* x, float* out, size_t n, hipStream_t s);
 void develop(const float* H, const float* acc, float* out, float* density, const float* curves, const float* gaxis,
              const float* ginv, const float* glayers, const float* lgam, const DevelopArgs& a, hipStream_t s);
+// glare: the blurred glare field (one plane) or nullptr.
 void print_pointwise(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea, const float* paper,
-                     const float* cmax, uint32_t w, uint32_t h, hipStream_t s);
+                     const float* cmax, const float* glare, uint32_t w, uint32_t h, hipStream_t s);
+// One lognormal glare value per pixel (stream 200).
+void glare_field(float* out, uint32_t n, float mean, float sd, uint32_t seed, hipStream_t s);
 void philox_test(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n, hipStream_t s);
 void poisson_test(const float* lam, float* out, uint32_t n, uint32_t stream, uint32_t seed, const float* lgam,
                   hipStream_t s);
@@ -116,5 +127,11 @@ void unsharp_encode2d(const float* rgb, uint16_t* out, uint32_t stride, const Pr
                       uint32_t w, uint32_t h, hipStream_t s);
 void unsharp_encode(const float* rgb, const float* blurred, uint16_t* out, uint32_t stride, const PrintConsts* pc,
                     uint32_t w, uint32_t h, hipStream_t s);
+
+// Tier downscale passes on interleaved 3-channel images (session.hip).
+void ds_blur_v(const float* src, float* dst, const float* taps, uint32_t w, uint32_t h, int r, hipStream_t s);
+void ds_blur_h(const float* src, float* dst, const float* taps, uint32_t w, uint32_t h, int r, hipStream_t s);
+void ds_resample(const float* src, float* dst, const uint32_t* xi, const float* xt, const uint32_t* yi, const float* yt,
+                 uint32_t in_w, uint32_t out_w, uint32_t out_h, hipStream_t s);
 
 }  // namespace spk::k

@@ -14,6 +14,7 @@
 
 #include "fir_tile.h"
 #include "kernels.h"
+#include "philox.h"
 #include "xcd.h"
 
 namespace spk::k {
@@ -158,10 +159,25 @@ __device__ inline uint16_t encode16(float v) {
     return uint16_t(floorf(c * 65535.0f + 0.5f));
 }
 
+// Print glare field (reference kernels/fx_random.h:100-115, fx_grain.metal:102-116 with same_all):
+// one lognormal value per pixel from words x, y of block 0 of (pixel, stream 200, seed),
+// z = sqrt(-2 ln u1) cos(2 pi u2), sig = sqrt(ln(1 + s^2 / m^2)), mu = ln m - sig^2 / 2, F = exp(mu + sig z).
+__global__ __launch_bounds__(256) void k_glare_field(float* __restrict__ out, uint32_t n, float mean, float sd,
+                                                     uint32_t seed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4 b = philox_block(i, 200u, 0u, seed);
+    const float u1 = u01(b.x), u2 = u01(b.y);
+    const float z = sqrtf(-2.0f * fln(u1)) * __builtin_amdgcn_cosf(u2);  // cos(2 pi u2): argument in revolutions
+    const float sig = sqrtf(fln(1.0f + (sd * sd) / (mean * mean)));
+    const float mu = fln(mean) - (0.5f * sig) * sig;
+    out[i] = fexp(mu + sig * z);
+}
+
 __global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, float* __restrict__ rgb,
                                                const PrintConsts* __restrict__ pcp, EnlargerArgs e,
                                                const float* __restrict__ paper, const float* __restrict__ cmax,
-                                               uint32_t w, uint32_t h) {
+                                               const float* __restrict__ glare, uint32_t w, uint32_t h) {
     const PrintConsts& q = *pcp;
     const uint32_t n = w * h;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -201,6 +217,12 @@ __global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, fl
             r2 += T * q.scan_wt[3 * l + 2];
         }
         xyz = {fmaxf(r0, 0.0f) + kTiny, fmaxf(r1, 0.0f) + kTiny, fmaxf(r2, 0.0f) + kTiny};
+    }
+    if (glare) {  // fx_glare.metal:13-22
+        const float F = glare[i];
+        xyz.x = xyz.x + (F / 100.0f) * e.glare_tint[0];
+        xyz.y = xyz.y + (F / 100.0f) * e.glare_tint[1];
+        xyz.z = xyz.z + (F / 100.0f) * e.glare_tint[2];
     }
     F3 o = mat_apply(q.to_rgb, xyz);
 
@@ -323,9 +345,13 @@ __global__ __launch_bounds__(kUNT) void k_unsharp_encode2d(const float* __restri
 }  // namespace
 
 void print_pointwise(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea, const float* paper,
-                     const float* cmax, uint32_t w, uint32_t h, hipStream_t s) {
+                     const float* cmax, const float* glare, uint32_t w, uint32_t h, hipStream_t s) {
     const uint32_t n = w * h;
-    hipLaunchKernelGGL(k_print, dim3((n + 255) / 256), dim3(256), 0, s, neg, rgb, pc, ea, paper, cmax, w, h);
+    hipLaunchKernelGGL(k_print, dim3((n + 255) / 256), dim3(256), 0, s, neg, rgb, pc, ea, paper, cmax, glare, w, h);
+}
+
+void glare_field(float* out, uint32_t n, float mean, float sd, uint32_t seed, hipStream_t s) {
+    hipLaunchKernelGGL(k_glare_field, dim3((n + 255) / 256), dim3(256), 0, s, out, n, mean, sd, seed);
 }
 
 void unsharp_encode2d(const float* rgb, uint16_t* out, uint32_t stride, const PrintConsts* pc, const FirArgs& f,

@@ -180,7 +180,7 @@ def test_errors_map_to_statuses(engine):
         with pytest.raises(spk.SpkError) as e:
             s.set_params({"print_exposure": 100.0})
         assert e.value.status == -2
-        s.set_params({"glare_active": True})
+        s.set_params({"extended_dynamic_range": True})
         with pytest.raises(spk.SpkError) as e:
             s.render("full")
-        assert e.value.status == -5 and "glare" in e.value.message
+        assert e.value.status == -5 and "extended_dynamic_range" in e.value.message

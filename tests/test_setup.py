@@ -140,7 +140,26 @@ def test_halation_mixture_override(resources, oracle):
 
 
 def test_refuses_stages_it_does_not_build(resources):
-    with pytest.raises(spk.SpkError, match="auto_exposure"):
-        spk.host_plan(resources, delta(auto_exposure=True), 64, 64)
-    with pytest.raises(spk.SpkError, match="glare"):
-        spk.host_plan(resources, delta(glare_active=True), 64, 64)
+    with pytest.raises(spk.SpkError, match="geometry"):
+        spk.host_plan(resources, delta(geometry_quarter_turns=1), 64, 64)
+    with pytest.raises(spk.SpkError, match="input_cctf_decoding"):
+        spk.host_plan(resources, delta(input_cctf_decoding=True), 64, 64)
+    with pytest.raises(spk.SpkError, match="extended_dynamic_range"):
+        spk.host_plan(resources, delta(extended_dynamic_range=True), 64, 64)
+
+
+def test_coupler_and_glare_plan_fields(oracle, resources):
+    """DIR diffusion = (1 - 0.06) G(20 um / p) + exponential surrogate(0.06, 200 um / p)
+    (dir_blur.hpp:15-19); glare mean 0.03 amount, std 0.7 mean, sigma 0.5 rho sqrt(amount)."""
+    w, h = 640, 480
+    plan = spk.host_plan(resources, delta(dir_couplers_active=True, glare_active=True, glare_amount=2.0), w, h)
+    assert oracle.field(plan, "couplers_active")[0] == 1 and oracle.field(plan, "n_dir")[0] == 4
+    p = 1000.0 * 35.0 / w
+    comps = oracle.field(plan, "dir", 4 * 6).reshape(4, 2, 3)
+    assert np.allclose(comps[0, 1], 20.0 / p, rtol=1e-15) and np.allclose(comps[0, 0], 0.94, rtol=1e-15)
+    lam = 200.0 / p
+    np.testing.assert_allclose(comps[1:, 1, 0], np.array([0.5360, 1.5236, 2.7684]) * lam, rtol=1e-15)
+    np.testing.assert_allclose(comps[1:, 0, 0], 0.06 * np.array([0.1633, 0.6496, 0.1870]), rtol=1e-15)
+    assert abs(comps[:, 0, 0].sum() - (0.94 + 0.06 * 0.9999)) < 1e-15
+    assert oracle.field(plan, "glare_active")[0] == 1
+    assert np.isclose(oracle.field(plan, "glare_sigma")[0], 0.5 * np.sqrt(2.0), rtol=1e-15)
