@@ -58,14 +58,23 @@ def main():
                     help="cfg: FIR for sigma <= 16; ref: the reference's FIR for sigma < 3")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-sample", type=str, default="1536x1024")
+    ap.add_argument("--spectral", choices=["fp32", "fp16"], default="fp32",
+                    help="precision of the print's 81-bin spectral integrals (fp16: matrix cores, FP32 accumulation)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 1 GiB device-to-device copy (a reference dispatch of known "
                          "traffic for hardware byte counters)")
+    ap.add_argument("--study", choices=["precision"], default=None,
+                    help="precision: one developed negative, a reprint sweep of print_exposure 0.5..2.0 in FP32 and "
+                         "FP16, each against the float64 oracle (16-bit LSB and CAM16-UCS dE)")
+    ap.add_argument("--sweep", type=int, default=32, help="reprints in the precision study")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the render chain runs on a single "
               "MI355X and multi-GPU scaling is not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.study == "precision":
+        precision_study(args)
+        return
 
     import torch
     import spektralab_engine_amd as spk
@@ -76,7 +85,7 @@ def main():
                  dir_couplers_active=False, glare_active=False, grain_active=True, halation_active=True,
                  film_format_mm=35.0)
     overrides = {"halation_mixture": [[2.0, 0.6], [8.0, 0.3], [32.0, 0.1]], "halation_scatter": False,
-                 "fir_crossover": crossover}
+                 "fir_crossover": crossover, "spectral_precision": args.spectral}
     res = synth.default_resources_dir(192)
     W, H = args.width, args.height
     log(f"[bench] generating the {W}x{H} frame")
@@ -175,6 +184,101 @@ def main():
     print(json.dumps(line), flush=True)
     sess.close()
     eng.close()
+
+
+def precision_study(args):
+    """The benchmark's configuration 4: one developed W x H negative of the chain workload, then
+    `sweep` spk_reprint calls at print_exposure 0.5..2.0, each in FP32 and in FP16 spectral
+    precision.  Every reprint is compared with the float64 oracle's print of the same negative on a
+    central crop (the print is pointwise; the unsharp blur's 2-pixel reach is excluded at the crop
+    border).  Prints one JSON line with the errors in 16-bit LSB and CAM16-UCS dE and the reprint
+    rates."""
+    import torch
+    import spektralab_engine_amd as spk
+    from spektralab_engine_amd import colour, synth
+    from oracle import oracle as orc
+
+    delta = dict(film_stock=synth.FILM_STOCK, print_stock=synth.PAPER_STOCK, auto_exposure=False,
+                 dir_couplers_active=False, glare_active=False, grain_active=True, halation_active=True,
+                 film_format_mm=35.0)
+    overrides = {"halation_mixture": [[2.0, 0.6], [8.0, 0.3], [32.0, 0.1]], "halation_scatter": False,
+                 "fir_crossover": 16.5}
+    W, H = args.width, args.height
+    cw, ch = min(2048, W), min(1536, H)
+    x0, y0 = (W - cw) // 2, (H - ch) // 2
+    m = 4  # interior margin of the crop
+    res = synth.default_resources_dir(192)
+    frame = synth.make_frame(W, H, 0)
+    eng = spk.Engine(res)
+    sess = eng.open(frame, delta)
+    sess.set_overrides(overrides)
+    out = torch.empty((H, W, 4), dtype=torch.int16, device="cuda")
+    sess.render_device(out.data_ptr(), W)
+    torch.cuda.synchronize()
+    neg = sess.negative("full")[:, y0:y0 + ch, x0:x0 + cw].copy()
+    modes = ("fp32", "fp16")
+    hist = {k: np.zeros(65536, np.int64) for k in modes}
+    de_edges = np.linspace(0.0, 2.0, 20001)
+    de_hist = {k: np.zeros(len(de_edges) - 1, np.int64) for k in modes}
+    de_max = {k: 0.0 for k in modes}
+    times = {k: [] for k in modes}
+    exposures = np.linspace(0.5, 2.0, args.sweep)
+    for n_e, ev in enumerate(exposures):
+        sess.set_params({"print_exposure": float(ev)})
+        codes = {}
+        for mode in modes:
+            sess.set_overrides({"spectral_precision": mode})
+            sess.render_device(out.data_ptr(), W, reprint=True)  # first reprint after a change uploads tables
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                sess.render_device(out.data_ptr(), W, reprint=True)
+            torch.cuda.synchronize()
+            times[mode].append((time.perf_counter() - t0) * 1000.0 / 3)
+            codes[mode] = out[y0:y0 + ch, x0:x0 + cw].cpu().numpy().view(np.uint16)
+        plan = bytearray(bytes(sess.last_plan()))
+        plan[8:16] = np.array([cw, ch], np.uint32).tobytes()  # the print constants do not depend on the size
+        ref = orc.print_((ctypes_buffer(plan)), neg, f64=True)[m:-m, m:-m]
+        for mode in modes:
+            got = codes[mode][m:-m, m:-m]
+            d = np.abs(got[..., :3].astype(np.int64) - ref[..., :3].astype(np.int64)).ravel()
+            hist[mode] += np.bincount(d, minlength=65536)[:65536]
+            de = colour.delta_e_ucs(got, ref).ravel()
+            de_max[mode] = max(de_max[mode], float(de.max()))
+            de_hist[mode] += np.histogram(np.minimum(de, de_edges[-1] - 1e-12), bins=de_edges)[0]
+        log(f"[precision] print_exposure {ev:.3f}: fp32 {times['fp32'][-1]:.3f} ms, fp16 {times['fp16'][-1]:.3f} ms "
+            f"({n_e + 1}/{len(exposures)})")
+
+    def pct(h, q, edges=None):
+        c = np.cumsum(h)
+        i = int(np.searchsorted(c, q * c[-1]))
+        return float(edges[i + 1]) if edges is not None else float(i)
+
+    summary = {}
+    for mode in modes:
+        h = hist[mode]
+        n = h.sum()
+        summary[mode] = {
+            "lsb_max": int(np.nonzero(h)[0].max()), "lsb_p99": pct(h, 0.99), "lsb_p999": pct(h, 0.999),
+            "lsb_mean": float((h * np.arange(65536)).sum() / n),
+            "de_max": round(de_max[mode], 5), "de_p99": pct(de_hist[mode], 0.99, de_edges),
+            "de_p999": pct(de_hist[mode], 0.999, de_edges),
+            "reprint_ms": round(float(np.median(times[mode])), 4),
+            "reprint_mpx_s": round(W * H / (np.median(times[mode]) / 1000.0) / 1e6, 1)}
+    line = {"study": "precision", "config": {
+        "workload": f"config 4: one developed {W}x{H} negative of the chain workload, {len(exposures)} reprints "
+                    f"at print_exposure 0.5..2.0, FP32 vs FP16 (matrix cores, FP32 accumulation) spectral integrals",
+        "reference": f"float64 oracle print of the same negative on the central {cw}x{ch} crop, {m}-px border "
+                     "excluded", "pixels_compared": int(hist["fp32"].sum() // 3)},
+        "fp32": summary["fp32"], "fp16": summary["fp16"]}
+    print(json.dumps(line), flush=True)
+    sess.close()
+    eng.close()
+
+
+def ctypes_buffer(b):
+    import ctypes
+    return ctypes.create_string_buffer(bytes(b), len(b))
 
 
 if __name__ == "__main__":

@@ -1049,7 +1049,8 @@ long oracle_plan_offset(const char* name) {
     F(scatter_active) F(n_scatter) F(scatter) F(bounce_active) F(n_bounce) F(bounce) F(mix_a) F(mix_b)
     F(grain_active) F(grain_mode) F(grain_seed) F(grain_k) F(dmin_l) F(sat_d) F(count) F(uniformity) F(dmin_c)
     F(ae_active) F(ae_gain) F(ae_ev) F(couplers_active) F(n_dir) F(dir) F(glare_active) F(glare_seed) F(glare_sigma)
-    F(grain_final_blur) F(grain_amount) F(grain_axis) F(grain_inv) F(grain_layers) F(printed) F(paper_k) F(enl_eps64) F(enl_beta64) F(enl_wt64) F(enl_g64)
+    F(grain_final_blur) F(grain_amount) F(grain_axis) F(grain_inv) F(grain_layers) F(printed) F(paper_k) F(spectral_fp16)
+    F(enl_eps64) F(enl_beta64) F(enl_wt64) F(enl_g64)
     F(enl_o64) F(k_gain) F(k_gain64) F(scan_eps64) F(scan_beta64) F(scan_wt64) F(to_rgb64) F(cam64) F(m16_inv64)
     F(m_to_xyz64) F(m_to_rgb64) F(unsharp_sigma) F(unsharp_amount) F(cctf_active) F(cctf_mode) F(m_nid64)
     F(table) F(table64) F(curves) F(curves_x64) F(curves_y64) F(paper) F(paper_x64) F(paper_y64) F(cmax) F(cmax64)

@@ -68,7 +68,7 @@ _DT = {"width": "u4", "height": "u4", "L": "u4", "K": "u4", "scatter_active": "u
        "a": "f4", "ec_gain": "f4", "dmin_l": "f4", "sat_d": "f4", "count": "f4", "uniformity": "f4",
        "dmin_c": "f4", "k_gain": "f4", "table": "f4", "curves": "f4", "paper": "f4", "cmax": "f4",
        "grain_axis": "f4", "grain_inv": "f4", "grain_layers": "f4", "ae_active": "u4", "ae_gain": "f4",
-       "couplers_active": "u4", "glare_active": "u4", "glare_seed": "u4", "n_dir": "u4"}
+       "couplers_active": "u4", "glare_active": "u4", "glare_seed": "u4", "n_dir": "u4", "spectral_fp16": "u4"}
 
 
 def field(plan, name, count=1):

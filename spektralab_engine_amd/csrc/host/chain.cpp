@@ -26,6 +26,60 @@ gpu::DevBuf upload(const void* src, size_t bytes, gpu::Pool& pool) {
     return b;
 }
 
+// IEEE binary16 bits of v, rounded to nearest even from float64 (|v| >= 65520 -> infinity).
+uint16_t f16_bits(double v) {
+    const uint16_t sign = std::signbit(v) ? 0x8000u : 0u;
+    const double a = std::fabs(v);
+    if (a == 0.0) return sign;
+    if (!(a < 65520.0)) return uint16_t(sign | 0x7C00u);
+    int e = 0;
+    const double m = std::frexp(a, &e);  // a = m 2^e, m in [0.5, 1)
+    int E = e + 14;                       // biased exponent of a = (2m) 2^(e-1)
+    if (E < 1) return uint16_t(sign | uint16_t(std::nearbyint(std::ldexp(a, 24))));  // subnormal (1024 -> 2^-14)
+    long q = std::lrint((2.0 * m - 1.0) * 1024.0);
+    if (q == 1024) {
+        q = 0;
+        ++E;
+    }
+    return uint16_t(sign | uint16_t(E << 10) | uint16_t(q));
+}
+
+// The matrix-core form of one 81-bin integral (layout stated with k::SpecMfma): beta as the FP32
+// path narrows it (-log2(10) scaled) plus `shift`, wt from float64 to fp16 after a per-channel
+// power-of-two scale.  Densities are clamped to [lo, hi] = [-0.5, 16]; shift is the largest
+// whole number <= 12 that keeps every exp2 result below 2^15.5 over that range.
+k::SpecMfma spec_mfma(const double* eps64, const double* beta64, const double* wt64) {
+    constexpr double lo = -0.5, hi = 16.0;
+    float e[SPK_NL][3], b[SPK_NL];
+    double bound = -1e300;
+    for (int l = 0; l < SPK_NL; ++l) {
+        b[l] = num::f32(-beta64[l] * kLog2Of10);
+        double s = b[l];
+        for (int k = 0; k < 3; ++k) {
+            e[l][k] = num::f32(-eps64[3 * l + k] * kLog2Of10);
+            s += std::max(hi * double(e[l][k]), lo * double(e[l][k]));
+        }
+        bound = std::max(bound, s);
+    }
+    const int shift = int(std::min(12.0, std::floor(15.5 - bound)));
+    k::SpecMfma t{};
+    int ex[3];
+    for (int c = 0; c < 3; ++c) {
+        double mx = 0.0;
+        for (int l = 0; l < SPK_NL; ++l) mx = std::max(mx, std::fabs(wt64[3 * l + c]));
+        ex[c] = 0;
+        if (mx > 0.0) std::frexp(mx, &ex[c]);
+        t.unscale[c] = num::f32(std::ldexp(1.0, ex[c] - shift));
+    }
+    for (int l = 0; l < SPK_NL; ++l) {
+        t.beta_sh[l] = b[l] + float(shift);
+        for (int c = 0; c < 3; ++c) t.wt[c][l] = f16_bits(std::ldexp(wt64[3 * l + c], -ex[c]));
+    }
+    t.lo = float(lo);
+    t.hi = float(hi);
+    return t;
+}
+
 // Powers A^k of the recursion's companion matrix A = [[B1 B2 B3] [1 0 0] [0 1 0]] for the chunk
 // carries: pw[d] = A^(m 2^d) (d < 7) and last = A^len_last.  For wide Gaussians the three poles
 // nearly coincide near 1 and A is far from normal, so repeated squaring loses most digits (at
@@ -343,6 +397,8 @@ Tables upload_tables(const spk_plan& p, gpu::Pool& pool) {
     q.unsharp_a = num::f32(1.0 + p.unsharp_amount);
     q.unsharp_b = num::f32(-p.unsharp_amount);
     t.print_consts = upload(&q, sizeof q, pool);
+    const k::SpecMfma sm = spec_mfma(p.scan_eps64, p.scan_beta64, p.scan_wt64);
+    t.scan_mfma = upload(&sm, sizeof sm, pool);
     return t;
 }
 
@@ -494,8 +550,17 @@ void print(const spk_plan& p, const Tables& t, const float* negative, uint16_t* 
     gpu::DevBuf rgbb = pool.acquire(3 * np * sizeof(float));
     keep.push_back(rgbb);
     float* rgb = gpu::as<float>(rgbb);
-    k::print_pointwise(negative, rgb, dq, ea, gpu::as<float>(t.paper), gpu::as<float>(t.cmax),
-                       glare ? gpu::as<const float>(glare) : nullptr, w, h, s);
+    if (p.spectral_fp16) {
+        const k::SpecMfma em = spec_mfma(p.enl_eps64, p.enl_beta64, p.enl_wt64);
+        gpu::DevBuf emb = upload(&em, sizeof em, pool);
+        keep.push_back(emb);
+        k::print_pointwise_fp16(negative, rgb, dq, ea, gpu::as<const k::SpecMfma>(emb),
+                                gpu::as<const k::SpecMfma>(t.scan_mfma), gpu::as<float>(t.paper),
+                                gpu::as<float>(t.cmax), glare ? gpu::as<const float>(glare) : nullptr, w, h, s);
+    } else {
+        k::print_pointwise(negative, rgb, dq, ea, gpu::as<float>(t.paper), gpu::as<float>(t.cmax),
+                           glare ? gpu::as<const float>(glare) : nullptr, w, h, s);
+    }
     mark("print");
     const double one[3] = {1.0, 1.0, 1.0};
     if (p.scanner_lens_blur_px > 0.0) {

@@ -39,7 +39,7 @@ struct Profiler {
 std::vector<float> lgamma_table();
 
 struct Tables {
-    gpu::DevBuf table, curves, gaxis, ginv, glayers, lgam, paper, cmax, print_consts;
+    gpu::DevBuf table, curves, gaxis, ginv, glayers, lgam, paper, cmax, print_consts, scan_mfma;
     uint32_t L = 0, K = 0;
 };
 Tables upload_tables(const spk_plan& p, gpu::Pool& pool);

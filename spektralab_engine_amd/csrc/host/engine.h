@@ -54,8 +54,9 @@ struct OutputImage {
 
 class Session;
 
-// Overrides of spkx_set_overrides parsed onto `base`.
-setup::Overrides parse_overrides(const char* json, setup::Overrides base);
+// Overrides of spkx_set_overrides parsed onto `base`; *film_side (when given) is set when a key
+// other than spectral_precision (a print-side key) was present.
+setup::Overrides parse_overrides(const char* json, setup::Overrides base, bool* film_side = nullptr);
 // spkx_plan: the float64 set-up and the plan of a full-tier render, without any device.
 size_t host_plan(const char* dir, const char* delta, const char* overrides, uint32_t w, uint32_t h, uint32_t film_seed,
                  spk_plan* out);

@@ -90,6 +90,27 @@ struct PrintConsts {
     uint32_t unsharp_active;
 };
 
+// One 81-bin spectral integral R_c = sum_l exp2(e(l) . d + beta(l)) wt_c(l) with the products and
+// the sum on the matrix cores: per wavefront of 64 pixels, 11 steps of v_mfma_f32_16x16x32_f16
+// over 8 bins each (bins 81..87 are zero).  Lane l = 16g + j owns pixel l and supplies its own
+// transmittances as the B operand (B[k = 8g + s][col j] = T(pixel l, bin 8m + s)); the A operand
+// is wt block-diagonal in g (A[row 4g + c][k in group g] = wt_c, zero elsewhere), so the result
+// C[4g + c][j], held by lane 16g + j, is R_c of that lane's own pixel.  Lane l holds
+// A[row l%16][k = 8(l/16) + s]: channel row (l%16)%4 of wt when (l%16)/4 == l/16, else the zero
+// row 3.
+// wt is scaled per channel by 2^-e_c (max |wt_c| in [0.5, 1)) and stored as IEEE binary16 bits;
+// the exp2 arguments carry +shift (beta_sh = f32(beta) + shift) so that fp16 keeps relative
+// precision down to T = 2^-(24 + shift); unscale_c = 2^(e_c - shift).  Densities are clamped to
+// [lo, hi], which keeps every exp2 result below 2^15.5 (finite in fp16: a zero of A times an
+// infinite B would otherwise spread NaN to other pixels of the wavefront).
+struct SpecMfma {
+    float beta_sh[88];
+    uint16_t wt[4][88];
+    float unscale[3];
+    float lo, hi;
+    uint32_t pad[3];
+};
+
 // Launchers (stream-ordered, no synchronisation). Planes as documented above.
 void upsample(const float* rgb, float* H, const float* table, const UpsampleArgs& a, hipStream_t s);
 // The passes run on nplanes contiguous planes of a.w x a.h with the same arguments.
@@ -117,6 +138,10 @@ void develop(const float* H, const float* acc, float* out, float* density, const
 // glare: the blurred glare field (one plane) or nullptr.
 void print_pointwise(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea, const float* paper,
                      const float* cmax, const float* glare, uint32_t w, uint32_t h, hipStream_t s);
+// The same with both spectral integrals on the matrix cores (enl: enlarger, scan: scan operands).
+void print_pointwise_fp16(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea,
+                          const SpecMfma* enl, const SpecMfma* scan, const float* paper, const float* cmax,
+                          const float* glare, uint32_t w, uint32_t h, hipStream_t s);
 // One lognormal glare value per pixel (stream 200).
 void glare_field(float* out, uint32_t n, float mean, float sd, uint32_t seed, hipStream_t s);
 void philox_test(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n, hipStream_t s);

@@ -12,6 +12,8 @@
 // Float contraction is allowed here (the output is compared to the CPU oracle within 1 LSB).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fir_tile.h"
 #include "kernels.h"
 #include "philox.h"
@@ -174,6 +176,21 @@ __global__ __launch_bounds__(256) void k_glare_field(float* __restrict__ out, ui
     out[i] = fexp(mu + sig * z);
 }
 
+// Enlarger response R -> log10 exposure on the paper -> print gain -> paper densities
+// (print.metal:192-232).
+__device__ inline F3 expose_paper(float r0, float r1, float r2, const EnlargerArgs& e, const float* __restrict__ paper,
+                                  uint32_t K) {
+    float l0 = flog10(fmaxf(r0 * e.g[0] + e.o[0], 0.0f) + kTiny);
+    float l1 = flog10(fmaxf(r1 * e.g[1] + e.o[1], 0.0f) + kTiny);
+    float l2 = flog10(fmaxf(r2 * e.g[2] + e.o[2], 0.0f) + kTiny);
+    l0 = flog10(fmaxf(fexp2(l0 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
+    l1 = flog10(fmaxf(fexp2(l1 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
+    l2 = flog10(fmaxf(fexp2(l2 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
+    return {paper_lookup(paper, K, 0, l0), paper_lookup(paper, K, 1, l1), paper_lookup(paper, K, 2, l2)};
+}
+
+__device__ inline F3 scan_to_rgb(F3 xyz, const PrintConsts& q, const float* __restrict__ cmax);
+
 __global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, float* __restrict__ rgb,
                                                const PrintConsts* __restrict__ pcp, EnlargerArgs e,
                                                const float* __restrict__ paper, const float* __restrict__ cmax,
@@ -194,15 +211,10 @@ __global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, fl
             r1 += T * e.wt[3 * l + 1];
             r2 += T * e.wt[3 * l + 2];
         }
-        float l0 = flog10(fmaxf(r0 * e.g[0] + e.o[0], 0.0f) + kTiny);
-        float l1 = flog10(fmaxf(r1 * e.g[1] + e.o[1], 0.0f) + kTiny);
-        float l2 = flog10(fmaxf(r2 * e.g[2] + e.o[2], 0.0f) + kTiny);
-        l0 = flog10(fmaxf(fexp2(l0 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
-        l1 = flog10(fmaxf(fexp2(l1 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
-        l2 = flog10(fmaxf(fexp2(l2 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
-        d0 = paper_lookup(paper, q.paper_k, 0, l0);
-        d1 = paper_lookup(paper, q.paper_k, 1, l1);
-        d2 = paper_lookup(paper, q.paper_k, 2, l2);
+        const F3 d = expose_paper(r0, r1, r2, e, paper, q.paper_k);
+        d0 = d.x;
+        d1 = d.y;
+        d2 = d.z;
     }
     // scan: XYZ of the print
     F3 xyz;
@@ -224,6 +236,79 @@ __global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, fl
         xyz.y = xyz.y + (F / 100.0f) * e.glare_tint[1];
         xyz.z = xyz.z + (F / 100.0f) * e.glare_tint[2];
     }
+    const F3 o = scan_to_rgb(xyz, q, cmax);
+    rgb[i] = o.x;
+    rgb[size_t(n) + i] = o.y;
+    rgb[2 * size_t(n) + i] = o.z;
+}
+
+// ---- the spectral integrals on the matrix cores ---------------------------------------------
+// Operand layout in kernels.h (SpecMfma).  Each lane owns one pixel: it evaluates the 81
+// transmittances of its pixel in FP32 (the same dot products and exp2 as k_print, with the
+// densities clamped and the argument shifted), packs them to fp16 eight bins at a time, and the
+// matrix core forms the three weighted sums with FP32 accumulation.  The matrix steps need all
+// 64 lanes: lanes past the frame's end work on a clamped pixel and store nothing.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+
+// eps: [bin][3] density-to-exp2 coefficients (uniform across the wavefront); d: clamped densities.
+__device__ inline F3 spectral_mfma(const SpecMfma* __restrict__ t, const float* __restrict__ eps, float d0, float d1,
+                                   float d2, uint32_t lane) {
+    const uint32_t row = lane & 15u;
+    const uint32_t arow = ((row >> 2) == (lane >> 4) && (row & 3u) < 3u) ? (row & 3u) : 3u;
+    const h8v* __restrict__ a = reinterpret_cast<const h8v*>(t->wt[arow]);
+    f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (uint32_t m = 0; m < 11; ++m) {
+        h8v b;
+#pragma unroll
+        for (uint32_t s = 0; s < 8; ++s) {  // round to nearest even (v_cvt_pk_f16_f32 pairs)
+            const uint32_t l = 8 * m + s;
+            b[s] = l < SPK_NL
+                       ? _Float16(fexp2(fmaf(d2, eps[3 * l + 2], fmaf(d1, eps[3 * l + 1], fmaf(d0, eps[3 * l], t->beta_sh[l])))))
+                       : _Float16(0.0f);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[m], b, acc, 0, 0, 0);
+    }
+    return {acc[0] * t->unscale[0], acc[1] * t->unscale[1], acc[2] * t->unscale[2]};
+}
+
+// k_print with both 81-bin integrals on the matrix cores (FP16 products, FP32 accumulation).
+__global__ __launch_bounds__(256) void k_print16(const float* __restrict__ neg, float* __restrict__ rgb,
+                                                 const PrintConsts* __restrict__ pcp, EnlargerArgs e,
+                                                 const SpecMfma* __restrict__ enl, const SpecMfma* __restrict__ scn,
+                                                 const float* __restrict__ paper, const float* __restrict__ cmax,
+                                                 const float* __restrict__ glare, uint32_t w, uint32_t h) {
+    const PrintConsts& q = *pcp;
+    const uint32_t n = w * h;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    if (blockIdx.x * blockDim.x + (threadIdx.x & ~63u) >= n) return;  // whole wavefront past the end
+    const uint32_t p = min(i, n - 1);
+    const float lo = enl->lo, hi = enl->hi;
+    const F3 r = spectral_mfma(enl, e.eps, fmaxf(fminf(neg[p], hi), lo), fmaxf(fminf(neg[size_t(n) + p], hi), lo),
+                               fmaxf(fminf(neg[2 * size_t(n) + p], hi), lo), lane);
+    const F3 d = expose_paper(r.x, r.y, r.z, e, paper, q.paper_k);
+    const float sl = scn->lo, sh = scn->hi;
+    const F3 x = spectral_mfma(scn, q.scan_eps, fmaxf(fminf(d.x, sh), sl), fmaxf(fminf(d.y, sh), sl),
+                               fmaxf(fminf(d.z, sh), sl), lane);
+    if (i >= n) return;
+    F3 xyz{fmaxf(x.x, 0.0f) + kTiny, fmaxf(x.y, 0.0f) + kTiny, fmaxf(x.z, 0.0f) + kTiny};
+    if (glare) {
+        const float F = glare[i];
+        xyz.x = xyz.x + (F / 100.0f) * e.glare_tint[0];
+        xyz.y = xyz.y + (F / 100.0f) * e.glare_tint[1];
+        xyz.z = xyz.z + (F / 100.0f) * e.glare_tint[2];
+    }
+    const F3 o = scan_to_rgb(xyz, q, cmax);
+    rgb[i] = o.x;
+    rgb[size_t(n) + i] = o.y;
+    rgb[2 * size_t(n) + i] = o.z;
+}
+
+// Scanned XYZ -> output RGB (print.metal:249-258), then the CAM16-UCS gamut compression
+// (:267-317) when it is on.
+__device__ inline F3 scan_to_rgb(F3 xyz, const PrintConsts& q, const float* __restrict__ cmax) {
     F3 o = mat_apply(q.to_rgb, xyz);
 
     if (q.gamut_active) {
@@ -259,9 +344,7 @@ __global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, fl
         const F3 back = cam16_inverse(Jp, dd * cs, hh, q, k);
         o = mat_apply(q.m_to_rgb, back);
     }
-    rgb[i] = o.x;
-    rgb[size_t(n) + i] = o.y;
-    rgb[2 * size_t(n) + i] = o.z;
+    return o;
 }
 
 __global__ __launch_bounds__(256) void k_unsharp_encode(const float* __restrict__ rgb, const float* __restrict__ blurred,
@@ -348,6 +431,14 @@ void print_pointwise(const float* neg, float* rgb, const PrintConsts* pc, const 
                      const float* cmax, const float* glare, uint32_t w, uint32_t h, hipStream_t s) {
     const uint32_t n = w * h;
     hipLaunchKernelGGL(k_print, dim3((n + 255) / 256), dim3(256), 0, s, neg, rgb, pc, ea, paper, cmax, glare, w, h);
+}
+
+void print_pointwise_fp16(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea,
+                          const SpecMfma* enl, const SpecMfma* scan, const float* paper, const float* cmax,
+                          const float* glare, uint32_t w, uint32_t h, hipStream_t s) {
+    const uint32_t n = w * h;
+    hipLaunchKernelGGL(k_print16, dim3((n + 255) / 256), dim3(256), 0, s, neg, rgb, pc, ea, enl, scan, paper, cmax,
+                       glare, w, h);
 }
 
 void glare_field(float* out, uint32_t n, float mean, float sd, uint32_t seed, hipStream_t s) {

@@ -180,7 +180,9 @@ def test_errors_map_to_statuses(engine):
         with pytest.raises(spk.SpkError) as e:
             s.set_params({"print_exposure": 100.0})
         assert e.value.status == -2
-        s.set_params({"extended_dynamic_range": True})
+        # the synthetic paper carries no EDR tone map: a user error when the delta is applied
         with pytest.raises(spk.SpkError) as e:
-            s.render("full")
-        assert e.value.status == -5 and "extended_dynamic_range" in e.value.message
+            s.set_params({"extended_dynamic_range": True})
+        assert e.value.status == -2 and "extended_dynamic_range" in e.value.message
+        out, _ = s.render("full")  # the session still renders with its previous parameters
+        assert out.shape == (48, 64, 4)
