@@ -11,7 +11,11 @@ synchronisations.
 
 The JSON line reports Mpx/s, the live HIP-event duration and algorithmic-byte roofline of the
 dominant kernel, and a CPU baseline: the oracle (oracle/chain.cpp, binary32) timed on a bounded
-sample (a 1536 x 1024 crop of the same workload) on this host's cores.
+sample (a 1536 x 1024 crop of the same workload) on this host's cores.  It also carries the
+batched workload (configuration 3, "batched": 64 frames of mixed sizes and per-frame parameters,
+one hipGraph launch, against 64 sequential spk_render and spkx_render_device calls).
+`--study precision` runs configuration 4 instead (a reprint sweep, FP16 vs FP32 spectral
+integrals against the float64 oracle).
 """
 from __future__ import annotations
 
@@ -67,6 +71,8 @@ def main():
                     help="precision: one developed negative, a reprint sweep of print_exposure 0.5..2.0 in FP32 and "
                          "FP16, each against the float64 oracle (16-bit LSB and CAM16-UCS dE)")
     ap.add_argument("--sweep", type=int, default=32, help="reprints in the precision study")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-rendering measurement")
+    ap.add_argument("--batch-frames", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the render chain runs on a single "
@@ -136,6 +142,11 @@ def main():
     log(f"[bench] {ms:.3f} ms/frame, {mpxs:.1f} Mpx/s; stages (ms): " +
         ", ".join(f"{k} {v:.3f}" for k, v in per_stage.items()))
 
+    batched = None
+    if not args.no_batch:
+        sess.close()
+        batched = batch_measure(eng, res, args)
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
@@ -180,10 +191,94 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "launch_ms": round(dom_ms, 4), "bytes_per_px": STAGE_BYTES[dom]},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
     sess.close()
     eng.close()
+
+
+def batch_measure(eng, res, args):
+    """The benchmark's configuration 3: `batch_frames` frames of log-uniform area 256x256 ..
+    4096x2730 with mixed aspect ratios and per-frame print exposure, filter shifts and halation
+    sigmas (synth.batch_workload), each in its own open session.  Times three ways of rendering
+    every frame once, inputs resident on the device:
+      - sequential spk_render (the reference ABI: host-readable RGBA16 per frame, one call each);
+      - sequential spkx_render_device (device outputs, one call each);
+      - one batch run (spkx_batch_run: one hipGraph launch; autotuned at creation, untimed).
+    Returns frames/s and Mpx/s of each and the batch's speedups."""
+    import ctypes
+    import torch
+    import spektralab_engine_amd as spk
+    from spektralab_engine_amd import synth
+
+    wl = synth.batch_workload(args.batch_frames, seed=0)
+    base = dict(film_stock=synth.FILM_STOCK, print_stock=synth.PAPER_STOCK, auto_exposure=False,
+                dir_couplers_active=False, glare_active=False, grain_active=True, halation_active=True,
+                film_format_mm=35.0)
+    log(f"[bench] batch workload: {len(wl)} frames, {sum(f['width'] * f['height'] for f in wl) / 1e6:.1f} MP")
+    sessions = []
+    for f in wl:
+        d = dict(base)
+        d.update(f["params"])
+        s = eng.open(synth.make_frame(f["width"], f["height"], f["frame_seed"]), d)
+        s.set_overrides({"halation_mixture": f["mixture"], "halation_scatter": False, "fir_crossover": 16.5})
+        sessions.append(s)
+    px = sum(f["width"] * f["height"] for f in wl)
+    outs = [torch.empty((f["height"], f["width"], 4), dtype=torch.int16, device="cuda") for f in wl]
+    lib = eng.lib
+
+    def seq_render():
+        for s in sessions:
+            r = spk.Result()
+            st = lib.spk_render(s.h, b"full", ctypes.byref(r))
+            if st != 0:
+                raise spk.SpkError(st, eng.error())
+            lib.spk_result_free(ctypes.byref(r))
+
+    def seq_device():
+        for s, o, f in zip(sessions, outs, wl):
+            s.render_device(o.data_ptr(), f["width"])
+        torch.cuda.synchronize()
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1000.0 / reps
+
+    t_render = timed(seq_render, 2)
+    t_device = timed(seq_device, 3)
+    t0 = time.perf_counter()
+    b = spk.Batch(eng, sessions, [o.data_ptr() for o in outs], [f["width"] for f in wl])
+    t_create = time.perf_counter() - t0
+
+    def batch_run():
+        b.run()
+        b.sync()
+
+    t_batch = timed(batch_run, 5)
+    info = b.info()
+    b.close()
+    for s in sessions:
+        s.close()
+    log(f"[bench] batch: spk_render x{len(wl)} {t_render:.1f} ms, render_device x{len(wl)} {t_device:.1f} ms, "
+        f"batch {t_batch:.1f} ms (create + tune {t_create:.2f} s)")
+    n = len(wl)
+    return {"workload": f"config 3: {n} frames, {px / 1e6:.1f} MP total, log-uniform area 256x256..4096x2730, "
+                        "per-frame print_exposure/filters/halation sigmas",
+            "frames": n, "pixels": px,
+            "frames_per_s": round(n / (t_batch / 1000.0), 1), "mpx_s": round(px / (t_batch / 1000.0) / 1e6, 1),
+            "ms_per_batch": round(t_batch, 3),
+            "sequential_spk_render": {"ms": round(t_render, 3), "frames_per_s": round(n / (t_render / 1000.0), 1)},
+            "sequential_render_device": {"ms": round(t_device, 3),
+                                         "frames_per_s": round(n / (t_device / 1000.0), 1)},
+            "speedup_vs_spk_render": round(t_render / t_batch, 3),
+            "speedup_vs_render_device": round(t_device / t_batch, 3),
+            "tuned_config": info["config"], "create_and_tune_s": round(t_create, 2)}
 
 
 def precision_study(args):

@@ -5,7 +5,8 @@
  *   - device-resident rendering for hosts and benchmarks that keep frames in HBM
  *   - the render plan and cached negative, so a CPU checker can replay a render
  *   - build-defined overrides of the benchmark workloads (explicit halation mixture, FIR/IIR
- *     crossover)
+ *     crossover, spectral precision)
+ *   - batched rendering of many sessions in one hipGraph launch
  *   - self-tests of the Philox generator, the Poisson sampler and the Gaussian machinery
  */
 #ifndef SPEKTRALAB_EXT_H
@@ -23,6 +24,9 @@ extern "C" {
  *                                                    (same sigma and weight on every channel)
  *   "halation_scatter": bool                         false: no scatter mixture
  *   "fir_crossover": number > 0                      FIR for 0 < sigma < crossover (3 = reference)
+ *   "spectral_precision": "fp32" | "fp16"            the print's two 81-bin spectral integrals as
+ *                                                    FP32 sums, or FP16 matrix-core products with
+ *                                                    FP32 accumulation (keeps the negatives)
  * Returns SPK_OK | SPK_ERR_INVALID_ARG | SPK_ERR_USER. */
 SPK_API spk_status spkx_set_overrides(spk_session* session, const char* json);
 
@@ -51,6 +55,30 @@ SPK_API spk_status spkx_negative(spk_session* session, const char* tier, float* 
  * the reason in spk_last_error. */
 SPK_API size_t spkx_plan(const char* resources_dir, const char* params_delta_json, const char* overrides_json,
                          uint32_t width, uint32_t height, uint32_t film_seed, void* out, size_t cap);
+
+/* Batched rendering.  spkx_batch_create snapshots the parameters of n open sessions of one engine
+ * and captures one hipGraph that renders every session's full tier (develop + print, fresh seeds
+ * from each session's own sequence at every run, as spk_render draws them) into dev_out[i]
+ * (RGBA16, row pitch row_stride_px[i] >= width).  The develop and print kernels run as one
+ * persistent launch over all frames; the per-frame blur passes run on forked streams.
+ * options_json (NULL = defaults): {"autotune": bool (default true: the stream count and the
+ * persistent launches' grid and workgroup size of the phased schedule are chosen by timing
+ * candidate graphs), "schedule": "phased" (one develop and one print launch for all frames
+ * between per-frame phases; the default) | "streams" (each frame's whole chain on its stream;
+ * with autotune off, and no session in FP16 spectral precision), "streams": 1..4,
+ * "grid_per_cu": 1..64, "block": 64|128|256}.  Parameter changes made to the
+ * sessions after creation are not seen by the batch.  Returns NULL with the reason in
+ * spk_last_error on failure.
+ * spkx_batch_run enqueues one batch render on `stream` (a hipStream_t; NULL = the batch's own) and
+ * returns; spkx_batch_sync waits for it; spkx_batch_info returns {"frames", "pixels", "config",
+ * "tuning": [{..., "ms"}]}. */
+typedef struct spkx_batch spkx_batch;
+SPK_API spkx_batch* spkx_batch_create(spk_engine* engine, spk_session* const* sessions, int32_t n,
+                                      void* const* dev_out, const uint32_t* row_stride_px, const char* options_json);
+SPK_API spk_status spkx_batch_run(spkx_batch* batch, void* stream);
+SPK_API spk_status spkx_batch_sync(spkx_batch* batch);
+SPK_API spk_status spkx_batch_info(spkx_batch* batch, char** out_json);
+SPK_API void spkx_batch_destroy(spkx_batch* batch);
 
 /* Self-tests on the engine's GPU:
  *   out[4i..4i+3] = Philox4x32-10(ctr[4i..4i+3], key[2i..2i+1])

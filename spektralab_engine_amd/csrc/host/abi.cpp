@@ -10,6 +10,7 @@
 #include <string>
 
 #include "../kernels/kernels.h"
+#include "batch.h"
 #include "engine.h"
 #include "tiers.h"
 #include "spektralab.h"
@@ -552,6 +553,60 @@ SPK_API spk_status spkx_downscale(spk_engine* engine, const float* in, uint32_t 
                                                      e.pool(), nullptr, &got);
         spk::gpu::check(hipDeviceSynchronize(), "downscale");
         spk::gpu::check(hipMemcpy(out, dst ? dst->ptr : src->ptr, n_out * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        return SPK_OK;
+    });
+}
+
+SPK_API spkx_batch* spkx_batch_create(spk_engine* engine, spk_session* const* sessions, int32_t n,
+                                      void* const* dev_out, const uint32_t* row_stride_px, const char* options_json) {
+    le::clear();
+    if (!engine || !sessions || !dev_out || !row_stride_px) {
+        invalid("a required argument is NULL");
+        return nullptr;
+    }
+    spkx_batch* out = nullptr;
+    const int32_t st = guarded([&] {
+        std::vector<Session*> ss(size_t(std::max(n, 0)));
+        for (int32_t i = 0; i < n; ++i) ss[size_t(i)] = S(sessions[i]);
+        std::unique_ptr<spk::Batch> b =
+            spk::Batch::create(*E(engine), ss.data(), n, const_cast<void**>(dev_out), row_stride_px, options_json);
+        out = reinterpret_cast<spkx_batch*>(b.release());
+        return SPK_OK;
+    });
+    return st == SPK_OK ? out : nullptr;
+}
+
+SPK_API spk_status spkx_batch_run(spkx_batch* batch, void* stream) {
+    le::clear();
+    if (!batch) return invalid("batch is NULL");
+    return guarded([&] {
+        reinterpret_cast<spk::Batch*>(batch)->run(static_cast<hipStream_t>(stream));
+        return SPK_OK;
+    });
+}
+
+SPK_API spk_status spkx_batch_sync(spkx_batch* batch) {
+    le::clear();
+    if (!batch) return invalid("batch is NULL");
+    return guarded([&] {
+        reinterpret_cast<spk::Batch*>(batch)->sync();
+        return SPK_OK;
+    });
+}
+
+SPK_API spk_status spkx_batch_info(spkx_batch* batch, char** out_json) {
+    le::clear();
+    if (!batch || !out_json) return invalid("a required argument is NULL");
+    return guarded([&] {
+        put_json(out_json, reinterpret_cast<spk::Batch*>(batch)->info());
+        return SPK_OK;
+    });
+}
+
+SPK_API void spkx_batch_destroy(spkx_batch* batch) {
+    le::clear();
+    (void)guarded([&] {
+        delete reinterpret_cast<spk::Batch*>(batch);
         return SPK_OK;
     });
 }

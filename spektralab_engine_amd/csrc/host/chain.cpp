@@ -402,8 +402,8 @@ Tables upload_tables(const spk_plan& p, gpu::Pool& pool) {
     return t;
 }
 
-void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
-             std::vector<gpu::DevBuf>& keep, Profiler* prof) {
+void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
+                 std::vector<gpu::DevBuf>& keep, DevelopStage* st, Profiler* prof) {
     auto mark = [&](const char* l) {
         if (prof && prof->on) prof->mark(l, s);
     };
@@ -485,22 +485,45 @@ void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negati
         da.coupler_stage = 2;
         da.mix_active = 0;
     }
+    k::DevelopJob& j = st->job;
+    j = k::DevelopJob{};
+    j.H = H;
+    j.acc = acc;
+    j.curves = gpu::as<float>(t.curves);
+    j.lgam = gpu::as<float>(t.lgam);
+    j.a = da;
+    st->G = st->D = nullptr;
+    st->blend = false;
     if (!p.grain_active) {
-        k::develop(H, acc, negative, nullptr, gpu::as<float>(t.curves), nullptr, nullptr, nullptr, nullptr, da, s);
-        mark("develop");
+        j.out = negative;
         return;
     }
     gpu::DevBuf Gb = pool.acquire(3 * np * sizeof(float));
     keep.push_back(Gb);
-    float* G = gpu::as<float>(Gb);
-    gpu::DevBuf Db;
+    st->G = gpu::as<float>(Gb);
     if (blend) {
-        Db = pool.acquire(3 * np * sizeof(float));
+        gpu::DevBuf Db = pool.acquire(3 * np * sizeof(float));
         keep.push_back(Db);
+        st->D = gpu::as<float>(Db);
+        st->blend = true;
     }
-    k::develop(H, acc, G, gpu::as<float>(Db), gpu::as<float>(t.curves), gpu::as<float>(t.gaxis), gpu::as<float>(t.ginv),
-               gpu::as<float>(t.glayers), gpu::as<float>(t.lgam), da, s);
-    mark("develop");
+    j.out = st->G;
+    j.density = st->D;
+    j.gaxis = gpu::as<float>(t.gaxis);
+    j.ginv = gpu::as<float>(t.ginv);
+    j.glayers = gpu::as<float>(t.glayers);
+}
+
+void develop_post(const spk_plan& p, const DevelopStage& st, float* negative, gpu::Pool& pool, hipStream_t s,
+                  std::vector<gpu::DevBuf>& keep, Profiler* prof) {
+    if (!p.grain_active) return;
+    auto mark = [&](const char* l) {
+        if (prof && prof->on) prof->mark(l, s);
+    };
+    const uint32_t w = p.width, h = p.height;
+    const size_t np = size_t(w) * h;
+    float* G = st.G;
+    const bool blend = st.blend;
     float* Gout = blend ? G : negative;
     if (p.grain_final_blur > 0.0) {
         const double sg[3] = {p.grain_final_blur, p.grain_final_blur, p.grain_final_blur};
@@ -511,13 +534,29 @@ void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negati
     }
     if (blend) {
         const float a = num::f32(1.0 - p.grain_amount), b = num::f32(p.grain_amount);
-        k::lincomb(gpu::as<float>(Db), G, negative, uint32_t(3 * np), a, b, s);
+        k::lincomb(st.D, G, negative, uint32_t(3 * np), a, b, s);
     }
     mark("grain_blur");
 }
 
-void print(const spk_plan& p, const Tables& t, const float* negative, uint16_t* out, uint32_t stride, gpu::Pool& pool,
-           hipStream_t s, std::vector<gpu::DevBuf>& keep, Profiler* prof) {
+void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
+             std::vector<gpu::DevBuf>& keep, Profiler* prof) {
+    DevelopStage st;
+    develop_pre(p, t, rgb, negative, pool, s, keep, &st, prof);
+    const k::DevelopJob& j = st.job;
+    k::develop(j.H, j.acc, j.out, j.density, j.curves, j.gaxis, j.ginv, j.glayers, j.lgam, j.a, s);
+    if (prof && prof->on) prof->mark("develop", s);
+    develop_post(p, st, negative, pool, s, keep, prof);
+}
+
+gpu::DevBuf upload_enlarger_mfma(const spk_plan& p, gpu::Pool& pool) {
+    const k::SpecMfma em = spec_mfma(p.enl_eps64, p.enl_beta64, p.enl_wt64);
+    return upload(&em, sizeof em, pool);
+}
+
+void print_pre(const spk_plan& p, const Tables& t, const float* negative, gpu::Pool& pool, hipStream_t s,
+               std::vector<gpu::DevBuf>& keep, PrintStage* st, const uint32_t* glare_seed, Profiler* prof,
+               const k::SpecMfma* enl_mfma) {
     auto mark = [&](const char* l) {
         if (prof && prof->on) prof->mark(l, s);
     };
@@ -542,26 +581,52 @@ void print(const spk_plan& p, const Tables& t, const float* negative, uint16_t* 
         glare = pool.acquire(np * sizeof(float));
         keep.push_back(raw);
         keep.push_back(glare);
-        k::glare_field(gpu::as<float>(raw), uint32_t(np), p.glare_mean, p.glare_std, p.glare_seed, s);
+        k::glare_field(gpu::as<float>(raw), uint32_t(np), p.glare_mean, p.glare_std, p.glare_seed, s, glare_seed);
         blur1(gpu::as<float>(raw), gpu::as<float>(glare), p.glare_sigma, p.fir_crossover, w, h, pool, s, keep);
         for (int c = 0; c < 3; ++c) ea.glare_tint[c] = p.glare_tint[c];
         mark("glare");
     }
     gpu::DevBuf rgbb = pool.acquire(3 * np * sizeof(float));
     keep.push_back(rgbb);
-    float* rgb = gpu::as<float>(rgbb);
+    k::PrintJob& j = st->job;
+    j.neg = negative;
+    j.rgb = gpu::as<float>(rgbb);
+    j.pc = dq;
+    j.paper = gpu::as<float>(t.paper);
+    j.cmax = gpu::as<float>(t.cmax);
+    j.glare = glare ? gpu::as<const float>(glare) : nullptr;
+    j.w = w;
+    j.h = h;
+    j.e = ea;
+    st->enl_mfma = nullptr;
+    st->scan_mfma = gpu::as<const k::SpecMfma>(t.scan_mfma);
     if (p.spectral_fp16) {
-        const k::SpecMfma em = spec_mfma(p.enl_eps64, p.enl_beta64, p.enl_wt64);
-        gpu::DevBuf emb = upload(&em, sizeof em, pool);
-        keep.push_back(emb);
-        k::print_pointwise_fp16(negative, rgb, dq, ea, gpu::as<const k::SpecMfma>(emb),
-                                gpu::as<const k::SpecMfma>(t.scan_mfma), gpu::as<float>(t.paper),
-                                gpu::as<float>(t.cmax), glare ? gpu::as<const float>(glare) : nullptr, w, h, s);
-    } else {
-        k::print_pointwise(negative, rgb, dq, ea, gpu::as<float>(t.paper), gpu::as<float>(t.cmax),
-                           glare ? gpu::as<const float>(glare) : nullptr, w, h, s);
+        if (!enl_mfma) {
+            gpu::DevBuf emb = upload_enlarger_mfma(p, pool);
+            keep.push_back(emb);
+            enl_mfma = gpu::as<const k::SpecMfma>(emb);
+        }
+        st->enl_mfma = enl_mfma;
     }
-    mark("print");
+}
+
+void print_kernel(const PrintStage& st, hipStream_t s) {
+    const k::PrintJob& j = st.job;
+    if (st.enl_mfma)
+        k::print_pointwise_fp16(j.neg, j.rgb, j.pc, j.e, st.enl_mfma, st.scan_mfma, j.paper, j.cmax, j.glare, j.w, j.h, s);
+    else
+        k::print_pointwise(j.neg, j.rgb, j.pc, j.e, j.paper, j.cmax, j.glare, j.w, j.h, s);
+}
+
+void print_post(const spk_plan& p, const Tables& t, const PrintStage& st, uint16_t* out, uint32_t stride,
+                gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep, Profiler* prof) {
+    auto mark = [&](const char* l) {
+        if (prof && prof->on) prof->mark(l, s);
+    };
+    const uint32_t w = p.width, h = p.height;
+    const size_t np = size_t(w) * h;
+    const k::PrintConsts* dq = gpu::as<const k::PrintConsts>(t.print_consts);
+    float* rgb = st.job.rgb;
     const double one[3] = {1.0, 1.0, 1.0};
     if (p.scanner_lens_blur_px > 0.0) {
         const double sb[3] = {p.scanner_lens_blur_px, p.scanner_lens_blur_px, p.scanner_lens_blur_px};
@@ -592,6 +657,15 @@ void print(const spk_plan& p, const Tables& t, const float* negative, uint16_t* 
     mark("unsharp_blur");
     k::unsharp_encode(rgb, gpu::as<float>(blurb), out, stride, dq, w, h, s);
     mark("encode");
+}
+
+void print(const spk_plan& p, const Tables& t, const float* negative, uint16_t* out, uint32_t stride, gpu::Pool& pool,
+           hipStream_t s, std::vector<gpu::DevBuf>& keep, Profiler* prof) {
+    PrintStage st;
+    print_pre(p, t, negative, pool, s, keep, &st, nullptr, prof);
+    print_kernel(st, s);
+    if (prof && prof->on) prof->mark("print", s);
+    print_post(p, t, st, out, stride, pool, s, keep, prof);
 }
 
 }  // namespace spk::chain

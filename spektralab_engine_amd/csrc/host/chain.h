@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "../kernels/kernels.h"
 #include "gpu.h"
 #include "spektralab_plan.h"
 
@@ -50,6 +51,35 @@ void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negati
 // out: RGBA16 with a row pitch of `stride` pixels.
 void print(const spk_plan& p, const Tables& t, const float* negative, uint16_t* out, uint32_t stride, gpu::Pool& pool,
            hipStream_t s, std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr);
+
+// The same two stages in phases around their pointwise kernel, for callers that launch that
+// kernel themselves (many frames in one launch): develop = develop_pre, the DevelopJob's kernel,
+// develop_post; print = print_pre, print_kernel (or the PrintJob in a batch), print_post.
+struct DevelopStage {
+    k::DevelopJob job;
+    float* G = nullptr;  // the grain draw's output (grain on), before the final blur
+    float* D = nullptr;  // the pre-grain density (grain_amount != 1)
+    bool blend = false;
+};
+void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
+                 std::vector<gpu::DevBuf>& keep, DevelopStage* st, Profiler* prof = nullptr);
+void develop_post(const spk_plan& p, const DevelopStage& st, float* negative, gpu::Pool& pool, hipStream_t s,
+                  std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr);
+struct PrintStage {
+    k::PrintJob job;
+    const k::SpecMfma* enl_mfma = nullptr;  // set when the plan asks for FP16 spectral integrals
+    const k::SpecMfma* scan_mfma = nullptr;
+};
+// glare_seed: device pointer overriding the plan's glare seed (may be null).  enl_mfma: the
+// enlarger's FP16 operands already on the device (upload_enlarger_mfma; null: uploaded here with a
+// blocking copy, so callers capturing a graph pass them in).
+void print_pre(const spk_plan& p, const Tables& t, const float* negative, gpu::Pool& pool, hipStream_t s,
+               std::vector<gpu::DevBuf>& keep, PrintStage* st, const uint32_t* glare_seed, Profiler* prof = nullptr,
+               const k::SpecMfma* enl_mfma = nullptr);
+gpu::DevBuf upload_enlarger_mfma(const spk_plan& p, gpu::Pool& pool);
+void print_kernel(const PrintStage& st, hipStream_t s);
+void print_post(const spk_plan& p, const Tables& t, const PrintStage& st, uint16_t* out, uint32_t stride,
+                gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr);
 
 // One per-channel separable Gaussian on 3 planar planes (exposed for the blur tests):
 // out_c = fuse(mode, acc_c, weight_c, G_{sigma_c}(in_c)); mode 0 none, 1 zero, 2 plane.

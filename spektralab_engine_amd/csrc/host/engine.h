@@ -148,6 +148,18 @@ public:
 
     Engine& engine() { return *engine_; }
 
+    // For batches: the full tier's plan under the current parameters (its seeds are supplied at
+    // each run), with shared references to the pipeline tables and the source image.
+    struct BatchFrame {
+        std::unique_ptr<spk_plan> plan;
+        chain::Tables tables;
+        gpu::DevBuf src;
+        uint32_t w = 0, h = 0;
+    };
+    void batch_frame(BatchFrame* f, hipStream_t s);
+    // The seeds of one fresh render from the session's sequence, in the reference's order.
+    void draw_seeds(uint32_t* film, uint32_t* print);
+
 private:
     Session() = default;
     Pipeline build_pipeline(const params::ParamSet& p, std::shared_ptr<const res::Profile> film,

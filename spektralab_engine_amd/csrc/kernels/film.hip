@@ -15,6 +15,16 @@
 namespace spk::k {
 namespace {
 
+// Per-frame constants are read through the constant address space, so that their uniform loads
+// become scalar loads however the constants reached the kernel (kernel argument or a batch's job
+// table in memory).
+#define SPK_CONST __attribute__((address_space(4)))
+using CDA = const DevelopArgs SPK_CONST;
+template <class T>
+__device__ inline const T SPK_CONST& cref(const T& x) {
+    return *(const T SPK_CONST*)(&x);
+}
+
 // ---- Mitchell-Netravali (B = C = 1/3) and the table lookup (kernels/film.metal:58-101) --------
 __device__ inline float mitchell(float u) {
     const float a = fabsf(u);
@@ -201,19 +211,13 @@ __global__ __launch_bounds__(256) void k_upsample(const float* __restrict__ rgb,
 // Per pixel and channel: [raw := mix_a raw + mix_b acc] -> L := log10(max(raw, 0) + 1e-10) ->
 // D := F(L) -> [G := sum of the three sub-layer draws - D_min,c]
 // (halation.cpp:321-324 lincomb, film.metal:242-261, fx_grain.metal:39-97 SUM mode with shift).
-__global__ __launch_bounds__(256) void k_develop(const float* __restrict__ H, const float* __restrict__ acc,
-                                                 float* __restrict__ out, float* __restrict__ density,
-                                                 const float* __restrict__ curves, const float* __restrict__ gaxis,
-                                                 const float* __restrict__ ginv, const float* __restrict__ glayers,
-                                                 const float* __restrict__ lgam, DevelopArgs q) {
-    __shared__ float s_lgam[spkm::kLgammaTable];
-    const uint32_t n = q.w * q.h;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q.grain_active) {
-        for (uint32_t t = threadIdx.x; t < uint32_t(spkm::kLgammaTable); t += blockDim.x) s_lgam[t] = lgam[t];
-        __syncthreads();
-    }
-    if (i >= n) return;
+// Pixel i (< n = w h) of one frame; seed is the frame's film seed; s_lgam the LDS lgamma table
+// (filled when q.grain_active).
+__device__ __forceinline__ void develop_px(uint32_t i, uint32_t n, const float* __restrict__ H, const float* __restrict__ acc,
+                                  float* __restrict__ out, float* __restrict__ density, const float* __restrict__ curves,
+                                  const float* __restrict__ gaxis, const float* __restrict__ ginv,
+                                  const float* __restrict__ glayers, CDA& q, uint32_t seed,
+                                  const float* s_lgam) {
     const uint32_t K = q.K;
     const float* x = curves;
     const float* inv = curves + K;
@@ -269,7 +273,7 @@ __global__ __launch_bounds__(256) void k_develop(const float* __restrict__ H, co
         for (uint32_t l = 0; l < 3; ++l) {
             const uint32_t lc = l * 3 + c;
             const float dl = d[l] + q.dmin_l[lc];
-            val[l] = sublayer_draw(dl, q.sat_d[lc], q.count[lc], q.uniformity[c], i, c + 10u * l, q.seed, hl2p,
+            val[l] = sublayer_draw(dl, q.sat_d[lc], q.count[lc], q.uniformity[c], i, c + 10u * l, seed, hl2p,
                                    s_lgam);
         }
         float G = ((0.0f + val[0]) + val[1]) + val[2];
@@ -280,6 +284,55 @@ __global__ __launch_bounds__(256) void k_develop(const float* __restrict__ H, co
 #pragma unroll
         for (uint32_t m = 0; m < 3; ++m)
             density[size_t(m) * n + i] = (s1[0] * q.coupler_m[m] + s1[1] * q.coupler_m[3 + m]) + s1[2] * q.coupler_m[6 + m];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_develop(const float* __restrict__ H, const float* __restrict__ acc,
+                                                 float* __restrict__ out, float* __restrict__ density,
+                                                 const float* __restrict__ curves, const float* __restrict__ gaxis,
+                                                 const float* __restrict__ ginv, const float* __restrict__ glayers,
+                                                 const float* __restrict__ lgam, DevelopArgs q) {
+    __shared__ float s_lgam[spkm::kLgammaTable];
+    const uint32_t n = q.w * q.h;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q.grain_active) {
+        for (uint32_t t = threadIdx.x; t < uint32_t(spkm::kLgammaTable); t += blockDim.x) s_lgam[t] = lgam[t];
+        __syncthreads();
+    }
+    if (i >= n) return;
+    develop_px(i, n, H, acc, out, density, curves, gaxis, ginv, glayers, cref(q), q.seed, s_lgam);
+}
+
+// The frame of tile t: the last f with prefix[f] <= t (prefix ascending, prefix[0] = 0).
+__device__ inline uint32_t frame_of_tile(const uint32_t* __restrict__ prefix, uint32_t nframes, uint32_t t) {
+    uint32_t lo = 0, hi = nframes;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (prefix[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// k_develop over many frames: persistent workgroups stride over every frame's 256-pixel tiles.
+// The lgamma table is the same for every frame (the sampler's own function).
+__global__ __launch_bounds__(256) void k_develop_batch(const DevelopJob* __restrict__ jobs,
+                                                       const uint32_t* __restrict__ prefix, uint32_t nframes,
+                                                       const uint32_t* __restrict__ seeds) {
+    __shared__ float s_lgam[spkm::kLgammaTable];
+    for (uint32_t t = threadIdx.x; t < uint32_t(spkm::kLgammaTable); t += blockDim.x) s_lgam[t] = jobs[0].lgam[t];
+    __syncthreads();
+    const uint32_t total = prefix[nframes];
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        const uint32_t f = frame_of_tile(prefix, nframes, t);
+        const DevelopJob SPK_CONST& j = ((const DevelopJob SPK_CONST*)jobs)[f];
+        const uint32_t n = j.a.w * j.a.h;
+        const uint32_t seed = seeds ? seeds[2 * f] : j.a.seed;
+        for (uint32_t u = threadIdx.x; u < kBatchTile; u += blockDim.x) {
+            const uint32_t i = (t - prefix[f]) * kBatchTile + u;
+            if (i < n)
+                develop_px(i, n, j.H, j.acc, j.out, j.density, j.curves, j.gaxis, j.ginv, j.glayers, j.a, seed, s_lgam);
+        }
     }
 }
 
@@ -326,6 +379,11 @@ void develop(const float* H, const float* acc, float* out, float* density, const
     const uint32_t n = a.w * a.h;
     hipLaunchKernelGGL(k_develop, dim3((n + 255) / 256), dim3(256), 0, s, H, acc, out, density, curves, gaxis, ginv,
                        glayers, lgam, a);
+}
+
+void develop_batch(const DevelopJob* jobs, const uint32_t* tile_prefix, uint32_t nframes, const uint32_t* seeds,
+                   uint32_t grid, uint32_t block, hipStream_t s) {
+    hipLaunchKernelGGL(k_develop_batch, dim3(grid), dim3(block), 0, s, jobs, tile_prefix, nframes, seeds);
 }
 
 }  // namespace spk::k

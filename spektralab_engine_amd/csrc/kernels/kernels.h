@@ -111,6 +111,31 @@ struct SpecMfma {
     uint32_t pad[3];
 };
 
+// ---- batched (many frames per launch) forms of the pointwise stages -----------------------------
+// One job per frame; a launch covers every frame's pixels in 256-pixel tiles: tile_prefix[f] is
+// the first tile of frame f (tile_prefix[nframes] = total), and persistent workgroups stride over
+// the tile list.  seeds (device, may be null) overrides each frame's seed: develop reads
+// seeds[2 f] (the film seed), the glare draw seeds[2 f + 1] (the print seed).
+struct DevelopJob {
+    const float* H;
+    const float* acc;
+    float* out;
+    float* density;
+    const float *curves, *gaxis, *ginv, *glayers, *lgam;
+    DevelopArgs a;
+};
+
+struct PrintJob {
+    const float* neg;
+    float* rgb;
+    const PrintConsts* pc;
+    const float *paper, *cmax, *glare;
+    uint32_t w, h;
+    EnlargerArgs e;
+};
+
+inline constexpr uint32_t kBatchTile = 256;
+
 // Launchers (stream-ordered, no synchronisation). Planes as documented above.
 void upsample(const float* rgb, float* H, const float* table, const UpsampleArgs& a, hipStream_t s);
 // The passes run on nplanes contiguous planes of a.w x a.h with the same arguments.
@@ -142,8 +167,15 @@ void print_pointwise(const float* neg, float* rgb, const PrintConsts* pc, const 
 void print_pointwise_fp16(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea,
                           const SpecMfma* enl, const SpecMfma* scan, const float* paper, const float* cmax,
                           const float* glare, uint32_t w, uint32_t h, hipStream_t s);
-// One lognormal glare value per pixel (stream 200).
-void glare_field(float* out, uint32_t n, float mean, float sd, uint32_t seed, hipStream_t s);
+// One lognormal glare value per pixel (stream 200); seed_ptr (device, may be null) overrides seed.
+void glare_field(float* out, uint32_t n, float mean, float sd, uint32_t seed, hipStream_t s,
+                 const uint32_t* seed_ptr = nullptr);
+// Batched develop / print over nframes jobs (device arrays), `grid` persistent workgroups of
+// `block` threads (block divides kBatchTile).
+void develop_batch(const DevelopJob* jobs, const uint32_t* tile_prefix, uint32_t nframes, const uint32_t* seeds,
+                   uint32_t grid, uint32_t block, hipStream_t s);
+void print_batch(const PrintJob* jobs, const uint32_t* tile_prefix, uint32_t nframes, uint32_t grid, uint32_t block,
+                 hipStream_t s);
 void philox_test(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n, hipStream_t s);
 void poisson_test(const float* lam, float* out, uint32_t n, uint32_t stream, uint32_t seed, const float* lgam,
                   hipStream_t s);

@@ -43,7 +43,19 @@ struct F3 {
     float x, y, z;
 };
 
-__device__ inline F3 mat_apply(const float* m, F3 v) {
+// Per-frame constants are read through the constant address space, so that their uniform loads
+// become scalar loads however the constants reached the kernel (kernel argument or a batch's job
+// table in memory).
+#define SPK_CONST __attribute__((address_space(4)))
+using CPC = const PrintConsts SPK_CONST;
+using CEA = const EnlargerArgs SPK_CONST;
+template <class T>
+__device__ inline const T SPK_CONST& cref(const T& x) {
+    return *(const T SPK_CONST*)(&x);
+}
+
+template <class P>
+__device__ inline F3 mat_apply(P m, F3 v) {
     return {(m[0] * v.x + m[1] * v.y) + m[2] * v.z, (m[3] * v.x + m[4] * v.y) + m[5] * v.z,
             (m[6] * v.x + m[7] * v.y) + m[8] * v.z};
 }
@@ -63,7 +75,7 @@ struct Cam16Derived {
     float cz, inv_cz, f_l_qrt, chroma_k, hundred_over_f_l;
 };
 
-__device__ inline F3 cam16_forward(F3 xyz, const PrintConsts& q, const Cam16Derived& k) {
+__device__ inline F3 cam16_forward(F3 xyz, CPC& q, const Cam16Derived& k) {
     const F3 x100{100.0f * xyz.x, 100.0f * xyz.y, 100.0f * xyz.z};
     const F3 rgb = mat_apply(q.m16, x100);
     const float ra = post_adapt(q.d_rgb[0] * rgb.x, q.f_l);
@@ -85,7 +97,7 @@ __device__ inline F3 cam16_forward(F3 xyz, const PrintConsts& q, const Cam16Deri
     return {Jp, Mp, h};
 }
 
-__device__ inline F3 cam16_inverse(float Jp, float Mp, float h, const PrintConsts& q, const Cam16Derived& k) {
+__device__ inline F3 cam16_inverse(float Jp, float Mp, float h, CPC& q, const Cam16Derived& k) {
     const float M = (fexp(0.0228f * Mp) - 1.0f) * (1.0f / 0.0228f);
     const float J = Jp / (1.7f - 0.007f * Jp);
     const float C = M / k.f_l_qrt;
@@ -165,9 +177,10 @@ __device__ inline uint16_t encode16(float v) {
 // one lognormal value per pixel from words x, y of block 0 of (pixel, stream 200, seed),
 // z = sqrt(-2 ln u1) cos(2 pi u2), sig = sqrt(ln(1 + s^2 / m^2)), mu = ln m - sig^2 / 2, F = exp(mu + sig z).
 __global__ __launch_bounds__(256) void k_glare_field(float* __restrict__ out, uint32_t n, float mean, float sd,
-                                                     uint32_t seed) {
+                                                     uint32_t seed, const uint32_t* __restrict__ seed_ptr) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (seed_ptr) seed = *seed_ptr;
     const uint4 b = philox_block(i, 200u, 0u, seed);
     const float u1 = u01(b.x), u2 = u01(b.y);
     const float z = sqrtf(-2.0f * fln(u1)) * __builtin_amdgcn_cosf(u2);  // cos(2 pi u2): argument in revolutions
@@ -178,7 +191,7 @@ __global__ __launch_bounds__(256) void k_glare_field(float* __restrict__ out, ui
 
 // Enlarger response R -> log10 exposure on the paper -> print gain -> paper densities
 // (print.metal:192-232).
-__device__ inline F3 expose_paper(float r0, float r1, float r2, const EnlargerArgs& e, const float* __restrict__ paper,
+__device__ inline F3 expose_paper(float r0, float r1, float r2, CEA& e, const float* __restrict__ paper,
                                   uint32_t K) {
     float l0 = flog10(fmaxf(r0 * e.g[0] + e.o[0], 0.0f) + kTiny);
     float l1 = flog10(fmaxf(r1 * e.g[1] + e.o[1], 0.0f) + kTiny);
@@ -189,16 +202,12 @@ __device__ inline F3 expose_paper(float r0, float r1, float r2, const EnlargerAr
     return {paper_lookup(paper, K, 0, l0), paper_lookup(paper, K, 1, l1), paper_lookup(paper, K, 2, l2)};
 }
 
-__device__ inline F3 scan_to_rgb(F3 xyz, const PrintConsts& q, const float* __restrict__ cmax);
+__device__ inline F3 scan_to_rgb(F3 xyz, CPC& q, const float* __restrict__ cmax);
 
-__global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, float* __restrict__ rgb,
-                                               const PrintConsts* __restrict__ pcp, EnlargerArgs e,
-                                               const float* __restrict__ paper, const float* __restrict__ cmax,
-                                               const float* __restrict__ glare, uint32_t w, uint32_t h) {
-    const PrintConsts& q = *pcp;
-    const uint32_t n = w * h;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// Pixel i (< n) of one frame: negative densities -> enlarger -> paper -> scan -> output RGB.
+__device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __restrict__ neg, float* __restrict__ rgb,
+                                CPC& q, CEA& e, const float* __restrict__ paper,
+                                const float* __restrict__ cmax, const float* __restrict__ glare) {
     float d0 = neg[i], d1 = neg[size_t(n) + i], d2 = neg[2 * size_t(n) + i];
 
     // enlarger: log10 exposure on the paper, then the print gain
@@ -242,6 +251,42 @@ __global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, fl
     rgb[2 * size_t(n) + i] = o.z;
 }
 
+__global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, float* __restrict__ rgb,
+                                               const PrintConsts* __restrict__ pcp, EnlargerArgs e,
+                                               const float* __restrict__ paper, const float* __restrict__ cmax,
+                                               const float* __restrict__ glare, uint32_t w, uint32_t h) {
+    const uint32_t n = w * h;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    print_px(i, n, neg, rgb, cref(*pcp), cref(e), paper, cmax, glare);
+}
+
+// The frame of tile t: the last f with prefix[f] <= t (prefix ascending, prefix[0] = 0).
+__device__ inline uint32_t frame_of_tile(const uint32_t* __restrict__ prefix, uint32_t nframes, uint32_t t) {
+    uint32_t lo = 0, hi = nframes;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (prefix[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// k_print over many frames: persistent workgroups stride over every frame's 256-pixel tiles.
+__global__ __launch_bounds__(256) void k_print_batch(const PrintJob* __restrict__ jobs,
+                                                     const uint32_t* __restrict__ prefix, uint32_t nframes) {
+    const uint32_t total = prefix[nframes];
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        const uint32_t f = frame_of_tile(prefix, nframes, t);
+        const PrintJob SPK_CONST& j = ((const PrintJob SPK_CONST*)jobs)[f];
+        const uint32_t n = j.w * j.h;
+        for (uint32_t u = threadIdx.x; u < kBatchTile; u += blockDim.x) {
+            const uint32_t i = (t - prefix[f]) * kBatchTile + u;
+            if (i < n) print_px(i, n, j.neg, j.rgb, *(CPC*)j.pc, j.e, j.paper, j.cmax, j.glare);
+        }
+    }
+}
+
 // ---- the spectral integrals on the matrix cores ---------------------------------------------
 // Operand layout in kernels.h (SpecMfma).  Each lane owns one pixel: it evaluates the 81
 // transmittances of its pixel in FP32 (the same dot products and exp2 as k_print, with the
@@ -252,8 +297,8 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 
 // eps: [bin][3] density-to-exp2 coefficients (uniform across the wavefront); d: clamped densities.
-__device__ inline F3 spectral_mfma(const SpecMfma* __restrict__ t, const float* __restrict__ eps, float d0, float d1,
-                                   float d2, uint32_t lane) {
+template <class EP>
+__device__ inline F3 spectral_mfma(const SpecMfma* __restrict__ t, EP eps, float d0, float d1, float d2, uint32_t lane) {
     const uint32_t row = lane & 15u;
     const uint32_t arow = ((row >> 2) == (lane >> 4) && (row & 3u) < 3u) ? (row & 3u) : 3u;
     const h8v* __restrict__ a = reinterpret_cast<const h8v*>(t->wt[arow]);
@@ -279,16 +324,16 @@ __global__ __launch_bounds__(256) void k_print16(const float* __restrict__ neg, 
                                                  const SpecMfma* __restrict__ enl, const SpecMfma* __restrict__ scn,
                                                  const float* __restrict__ paper, const float* __restrict__ cmax,
                                                  const float* __restrict__ glare, uint32_t w, uint32_t h) {
-    const PrintConsts& q = *pcp;
+    CPC& q = cref(*pcp);
     const uint32_t n = w * h;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     if (blockIdx.x * blockDim.x + (threadIdx.x & ~63u) >= n) return;  // whole wavefront past the end
     const uint32_t p = min(i, n - 1);
     const float lo = enl->lo, hi = enl->hi;
-    const F3 r = spectral_mfma(enl, e.eps, fmaxf(fminf(neg[p], hi), lo), fmaxf(fminf(neg[size_t(n) + p], hi), lo),
-                               fmaxf(fminf(neg[2 * size_t(n) + p], hi), lo), lane);
-    const F3 d = expose_paper(r.x, r.y, r.z, e, paper, q.paper_k);
+    const F3 r = spectral_mfma(enl, cref(e).eps, fmaxf(fminf(neg[p], hi), lo),
+                               fmaxf(fminf(neg[size_t(n) + p], hi), lo), fmaxf(fminf(neg[2 * size_t(n) + p], hi), lo), lane);
+    const F3 d = expose_paper(r.x, r.y, r.z, cref(e), paper, q.paper_k);
     const float sl = scn->lo, sh = scn->hi;
     const F3 x = spectral_mfma(scn, q.scan_eps, fmaxf(fminf(d.x, sh), sl), fmaxf(fminf(d.y, sh), sl),
                                fmaxf(fminf(d.z, sh), sl), lane);
@@ -308,7 +353,7 @@ __global__ __launch_bounds__(256) void k_print16(const float* __restrict__ neg, 
 
 // Scanned XYZ -> output RGB (print.metal:249-258), then the CAM16-UCS gamut compression
 // (:267-317) when it is on.
-__device__ inline F3 scan_to_rgb(F3 xyz, const PrintConsts& q, const float* __restrict__ cmax) {
+__device__ inline F3 scan_to_rgb(F3 xyz, CPC& q, const float* __restrict__ cmax) {
     F3 o = mat_apply(q.to_rgb, xyz);
 
     if (q.gamut_active) {
@@ -441,8 +486,13 @@ void print_pointwise_fp16(const float* neg, float* rgb, const PrintConsts* pc, c
                        glare, w, h);
 }
 
-void glare_field(float* out, uint32_t n, float mean, float sd, uint32_t seed, hipStream_t s) {
-    hipLaunchKernelGGL(k_glare_field, dim3((n + 255) / 256), dim3(256), 0, s, out, n, mean, sd, seed);
+void glare_field(float* out, uint32_t n, float mean, float sd, uint32_t seed, hipStream_t s, const uint32_t* seed_ptr) {
+    hipLaunchKernelGGL(k_glare_field, dim3((n + 255) / 256), dim3(256), 0, s, out, n, mean, sd, seed, seed_ptr);
+}
+
+void print_batch(const PrintJob* jobs, const uint32_t* tile_prefix, uint32_t nframes, uint32_t grid, uint32_t block,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_print_batch, dim3(grid), dim3(block), 0, s, jobs, tile_prefix, nframes);
 }
 
 void unsharp_encode2d(const float* rgb, uint16_t* out, uint32_t stride, const PrintConsts* pc, const FirArgs& f,

@@ -273,6 +273,30 @@ def default_resources_dir(L: int = 192) -> Path:
 
 
 # ---- synthetic input frames ------------------------------------------------------------------
+ASPECTS = (1.0, 4.0 / 3.0, 3.0 / 2.0, 16.0 / 9.0)
+
+
+def batch_workload(n: int = 64, seed: int = 0, min_area: int = 256 * 256, max_area: int = 4096 * 2730):
+    """The batched workload: n frames with log-uniform areas in [min_area, max_area] (mean about
+    2.2 MP for the defaults), aspect ratios 1, 4:3, 3:2 and 16:9 in either orientation, and
+    per-frame parameters: print_exposure U(0.5, 2), magenta and yellow filter shifts U(-0.3, 0.3)
+    (the reference's live filter controls; it has no cyan shift), and the halation mixture
+    sigma 2/8/32 px each scaled by U(0.5, 2), weights 0.6/0.3/0.1.  Returns a list of dicts
+    {width, height, frame_seed, params, mixture}."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        area = float(np.exp(rng.uniform(np.log(min_area), np.log(max_area))))
+        r = float(ASPECTS[rng.integers(0, len(ASPECTS))])
+        if rng.integers(0, 2):
+            r = 1.0 / r
+        w = max(16, int(round(np.sqrt(area * r))))
+        h = max(16, int(round(area / w)))
+        params = {"print_exposure": float(rng.uniform(0.5, 2.0)), "m_filter_shift": float(rng.uniform(-0.3, 0.3)),
+                  "y_filter_shift": float(rng.uniform(-0.3, 0.3))}
+        mixture = [[s * float(rng.uniform(0.5, 2.0)), wt] for s, wt in ((2.0, 0.6), (8.0, 0.3), (32.0, 0.1))]
+        out.append({"width": w, "height": h, "frame_seed": 1000 + i, "params": params, "mixture": mixture})
+    return out
 def make_frame(width: int, height: int, seed: int = 0, rects: int = 64) -> np.ndarray:
     """Linear ProPhoto float32 H x W x 3: per-channel smooth gradients in log2 space spanning
     -8..+4 (values 2^-8..2^4), then `rects` axis-aligned rectangles with log-uniform colours."""
