@@ -102,18 +102,22 @@ __device__ inline bool ptrs_try(const Ptrs& P, float u, float v, const float* __
 // lam >= 10: Hormann's transformed rejection (PTRS, 1993), at most 64 tries, two per Philox
 // block, then floor(lam); lam < 10: multiply uniforms until the product falls to exp(-lam), at
 // most 512 steps, four per block.  NaN lam takes the second branch and returns 0.
+__device__ inline void ptrs_setup(Ptrs& P, float lam) {
+    P.lam = lam;
+    const float slam = sqrtf(lam);
+    P.loglam = spkm::logf_(lam);
+    P.b = 0.931f + (2.53f * slam);
+    P.a = -0.059f + (0.02483f * P.b);
+    const float invalpha = 1.1239f + (1.1328f / (P.b - 3.4f));
+    P.vr = 0.9277f - (3.6224f / (P.b - 2.0f));
+    P.loginv = spkm::logf_(invalpha);
+}
+
 __device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t seed, float hl2p,
                          const float* __restrict__ lgam) {
     if (lam >= 10.0f) {
         Ptrs P;
-        P.lam = lam;
-        const float slam = sqrtf(lam);
-        P.loglam = spkm::logf_(lam);
-        P.b = 0.931f + (2.53f * slam);
-        P.a = -0.059f + (0.02483f * P.b);
-        const float invalpha = 1.1239f + (1.1328f / (P.b - 3.4f));
-        P.vr = 0.9277f - (3.6224f / (P.b - 2.0f));
-        P.loginv = spkm::logf_(invalpha);
+        ptrs_setup(P, lam);
         for (uint32_t j = 0; j < 32; ++j) {
             const uint4 w = philox_block(pixel, stream, j, seed);
             float k;
@@ -151,6 +155,98 @@ __device__ inline float sublayer_draw(float d, float D, float n, float u, uint32
     const float sat = 1.0f - ((q * u) * 0.999999f);
     const float lam = (n / sat) * q;
     return poisson(lam, pixel, stream, seed, hl2p, lgam) * ((D / n) * sat);
+}
+
+// ---- the grain draws of one wavefront, compacted ---------------------------------------------
+// A wavefront's 64 pixels make 576 draws (3 channels x 3 sub-layers).  Drawn where they arise,
+// lanes whose lam lies on the other side of 10 (the two methods) or that need more uniform blocks
+// than their neighbours sit idle; here the draws are first listed by method in the wavefront's
+// LDS slice, then each list is drained by all 64 lanes, a lane taking the next draw as soon as
+// its current one is done.  Every draw consumes the same Philox blocks in the same order as
+// poisson() above, so the counts are identical; only the lane doing the work changes.
+struct GrainScratch {
+    float v[9][64];     // lam of draw (j = 3 c + l, pixel lane), then its Poisson count
+    uint16_t q[576];    // draw ids (j << 6 | lane): lam >= 10 from the front, the others from the back
+};
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+// Drains `count` listed draws of one method; base: the wavefront's first pixel.
+template <bool kLarge>
+__device__ __forceinline__ void drain(GrainScratch& gs, uint32_t count, uint32_t lane, uint32_t base, uint32_t seed,
+                                      float hl2p, const float* __restrict__ lgam) {
+    uint32_t head = 0;
+    bool busy = false;
+    uint32_t j = 0, src = 0, blk = 0, stream = 0;
+    float lam = 0.0f, E = 0.0f, X = 0.0f, prod = 1.0f;
+    Ptrs P{};
+    for (;;) {
+        const bool want = !busy;
+        const uint64_t wm = __ballot(want);
+        if (want) {
+            const uint32_t idx = head + lanes_below(wm);
+            if (idx < count) {
+                const uint32_t id = gs.q[kLarge ? idx : 575u - idx];
+                j = id >> 6;
+                src = id & 63u;
+                lam = gs.v[j][src];
+                stream = j / 3u + 10u * (j % 3u);
+                blk = 0;
+                busy = true;
+                if (kLarge) {
+                    ptrs_setup(P, lam);
+                } else {
+                    E = spkm::expf_(-lam);
+                    X = 0.0f;
+                    prod = 1.0f;
+                }
+            }
+        }
+        head = min(head + uint32_t(__popcll(wm)), count);
+        if (__ballot(busy) == 0) break;
+        if (busy) {
+            const uint4 w = philox_block(base + src, stream, blk, seed);
+            bool done = false;
+            float r = 0.0f;
+            if (kLarge) {
+                if (ptrs_try(P, u01(w.x), u01(w.y), lgam, hl2p, &r)) done = true;
+                else if (ptrs_try(P, u01(w.z), u01(w.w), lgam, hl2p, &r)) done = true;
+                else if (++blk == 32u) {
+                    r = floorf(lam);
+                    done = true;
+                }
+            } else {
+                const uint32_t wv[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    if (done) break;
+                    prod = prod * u01(wv[t]);
+                    if (!(prod > E)) {
+                        r = X;
+                        done = true;
+                    } else {
+                        X = X + 1.0f;
+                    }
+                }
+                if (!done && ++blk == 128u) {
+                    r = X;
+                    done = true;
+                }
+            }
+            if (done) {
+                gs.v[j][src] = r;
+                busy = false;
+            }
+        }
+    }
 }
 
 // ---- kernels ---------------------------------------------------------------------------------
@@ -211,21 +307,26 @@ __global__ __launch_bounds__(256) void k_upsample(const float* __restrict__ rgb,
 // Per pixel and channel: [raw := mix_a raw + mix_b acc] -> L := log10(max(raw, 0) + 1e-10) ->
 // D := F(L) -> [G := sum of the three sub-layer draws - D_min,c]
 // (halation.cpp:321-324 lincomb, film.metal:242-261, fx_grain.metal:39-97 SUM mode with shift).
-// Pixel i (< n = w h) of one frame; seed is the frame's film seed; s_lgam the LDS lgamma table
-// (filled when q.grain_active).
-__device__ __forceinline__ void develop_px(uint32_t i, uint32_t n, const float* __restrict__ H, const float* __restrict__ acc,
-                                  float* __restrict__ out, float* __restrict__ density, const float* __restrict__ curves,
-                                  const float* __restrict__ gaxis, const float* __restrict__ ginv,
-                                  const float* __restrict__ glayers, CDA& q, uint32_t seed,
-                                  const float* s_lgam) {
+// Pixel i of one frame (valid: i < n = w h), called by all 64 lanes of a wavefront together
+// (consecutive pixels; the grain draws are shared out across the wavefront through gs).  seed is
+// the frame's film seed; s_lgam the LDS lgamma table (filled when q.grain_active).
+__device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n, const float* __restrict__ H,
+                                            const float* __restrict__ acc, float* __restrict__ out,
+                                            float* __restrict__ density, const float* __restrict__ curves,
+                                            const float* __restrict__ gaxis, const float* __restrict__ ginv,
+                                            const float* __restrict__ glayers, CDA& q, uint32_t seed,
+                                            const float* s_lgam, GrainScratch& gs) {
     const uint32_t K = q.K;
     const float* x = curves;
     const float* inv = curves + K;
     const float* y = curves + 2 * K;
-    const float hl2p = spkm::half_log_2pi();
-    float s1[3];  // coupler stage 1: the coupler source of the first development
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool grain = q.grain_active != 0u;
+    float s1[3];      // coupler stage 1: the coupler source of the first development
+    float mult[9];    // grain: the scale (D / n) sat of draw 3 c + l
 #pragma unroll 1
     for (uint32_t c = 0; c < 3; ++c) {
+        if (!valid) continue;
         const size_t o = size_t(c) * n + i;
         float D;
         if (q.coupler_stage == 2u) {  // film.metal:282-292: F(L - chi; y0)
@@ -243,7 +344,7 @@ __device__ __forceinline__ void develop_px(uint32_t i, uint32_t n, const float* 
             }
         }
         if (q.write_density) density[o] = D;
-        if (!q.grain_active) {
+        if (!grain) {
             out[o] = D;
             continue;
         }
@@ -269,21 +370,58 @@ __device__ __forceinline__ void develop_px(uint32_t i, uint32_t n, const float* 
                 d[l] = y0 + t * (y1 - y0);
             }
         }
-        float val[3];
+        // sublayer_draw's arithmetic, the Poisson draw itself deferred (fx_random.h:91-99)
         for (uint32_t l = 0; l < 3; ++l) {
             const uint32_t lc = l * 3 + c;
             const float dl = d[l] + q.dmin_l[lc];
-            val[l] = sublayer_draw(dl, q.sat_d[lc], q.count[lc], q.uniformity[c], i, c + 10u * l, seed, hl2p,
-                                   s_lgam);
+            const float Dl = q.sat_d[lc], nl = q.count[lc];
+            float qq = dl / Dl;
+            qq = qq < 1e-6f ? 1e-6f : (qq > 0.999999f ? 0.999999f : qq);
+            const float sat = 1.0f - ((qq * q.uniformity[c]) * 0.999999f);
+            gs.v[3 * c + l][lane] = (nl / sat) * qq;
+            mult[3 * c + l] = (Dl / nl) * sat;
         }
-        float G = ((0.0f + val[0]) + val[1]) + val[2];
-        G = G * 1.0f + (-q.dmin_c[c]);
-        out[o] = G;
     }
     if (q.coupler_stage == 1u) {
+        if (valid) {
 #pragma unroll
-        for (uint32_t m = 0; m < 3; ++m)
-            density[size_t(m) * n + i] = (s1[0] * q.coupler_m[m] + s1[1] * q.coupler_m[3 + m]) + s1[2] * q.coupler_m[6 + m];
+            for (uint32_t m = 0; m < 3; ++m)
+                density[size_t(m) * n + i] =
+                    (s1[0] * q.coupler_m[m] + s1[1] * q.coupler_m[3 + m]) + s1[2] * q.coupler_m[6 + m];
+        }
+        return;
+    }
+    if (!grain) return;
+
+    // list the draws by method, then drain each list with the whole wavefront
+    wave_sync();
+    uint32_t n_large = 0, n_small = 0;
+#pragma unroll
+    for (uint32_t jd = 0; jd < 9; ++jd) {
+        const float lam = valid ? gs.v[jd][lane] : 0.0f;
+        const bool large = valid && lam >= 10.0f;
+        const bool small = valid && !(lam >= 10.0f);
+        const uint64_t bl = __ballot(large), bs = __ballot(small);
+        const uint16_t id = uint16_t((jd << 6) | lane);
+        if (large) gs.q[n_large + lanes_below(bl)] = id;
+        if (small) gs.q[575u - (n_small + lanes_below(bs))] = id;
+        n_large += uint32_t(__popcll(bl));
+        n_small += uint32_t(__popcll(bs));
+    }
+    wave_sync();
+    const float hl2p = spkm::half_log_2pi();
+    const uint32_t base = i - lane;
+    drain<true>(gs, n_large, lane, base, seed, hl2p, s_lgam);
+    drain<false>(gs, n_small, lane, base, seed, hl2p, s_lgam);
+    wave_sync();
+    if (!valid) return;
+#pragma unroll
+    for (uint32_t c = 0; c < 3; ++c) {
+        float val[3];
+        for (uint32_t l = 0; l < 3; ++l) val[l] = gs.v[3 * c + l][lane] * mult[3 * c + l];
+        float G = ((0.0f + val[0]) + val[1]) + val[2];
+        G = G * 1.0f + (-q.dmin_c[c]);
+        out[size_t(c) * n + i] = G;
     }
 }
 
@@ -293,14 +431,16 @@ __global__ __launch_bounds__(256) void k_develop(const float* __restrict__ H, co
                                                  const float* __restrict__ ginv, const float* __restrict__ glayers,
                                                  const float* __restrict__ lgam, DevelopArgs q) {
     __shared__ float s_lgam[spkm::kLgammaTable];
+    __shared__ GrainScratch gs[4];
     const uint32_t n = q.w * q.h;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (q.grain_active) {
         for (uint32_t t = threadIdx.x; t < uint32_t(spkm::kLgammaTable); t += blockDim.x) s_lgam[t] = lgam[t];
         __syncthreads();
     }
-    if (i >= n) return;
-    develop_px(i, n, H, acc, out, density, curves, gaxis, ginv, glayers, cref(q), q.seed, s_lgam);
+    if (blockIdx.x * blockDim.x + (threadIdx.x & ~63u) >= n) return;  // whole wavefront past the end
+    develop_wave(i, i < n, n, H, acc, out, density, curves, gaxis, ginv, glayers, cref(q), q.seed, s_lgam,
+                 gs[threadIdx.x >> 6]);
 }
 
 // The frame of tile t: the last f with prefix[f] <= t (prefix ascending, prefix[0] = 0).
@@ -320,6 +460,7 @@ __global__ __launch_bounds__(256) void k_develop_batch(const DevelopJob* __restr
                                                        const uint32_t* __restrict__ prefix, uint32_t nframes,
                                                        const uint32_t* __restrict__ seeds) {
     __shared__ float s_lgam[spkm::kLgammaTable];
+    __shared__ GrainScratch gs[4];
     for (uint32_t t = threadIdx.x; t < uint32_t(spkm::kLgammaTable); t += blockDim.x) s_lgam[t] = jobs[0].lgam[t];
     __syncthreads();
     const uint32_t total = prefix[nframes];
@@ -330,8 +471,9 @@ __global__ __launch_bounds__(256) void k_develop_batch(const DevelopJob* __restr
         const uint32_t seed = seeds ? seeds[2 * f] : j.a.seed;
         for (uint32_t u = threadIdx.x; u < kBatchTile; u += blockDim.x) {
             const uint32_t i = (t - prefix[f]) * kBatchTile + u;
-            if (i < n)
-                develop_px(i, n, j.H, j.acc, j.out, j.density, j.curves, j.gaxis, j.ginv, j.glayers, j.a, seed, s_lgam);
+            if ((t - prefix[f]) * kBatchTile + (u & ~63u) >= n) break;  // whole wavefront past the frame's end
+            develop_wave(i, i < n, n, j.H, j.acc, j.out, j.density, j.curves, j.gaxis, j.ginv, j.glayers, j.a, seed,
+                         s_lgam, gs[threadIdx.x >> 6]);
         }
     }
 }
