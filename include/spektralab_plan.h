@@ -24,7 +24,7 @@ extern "C" {
 #endif
 
 #define SPK_PLAN_MAGIC 0x504B5053u /* "SPKP" */
-#define SPK_PLAN_VERSION 3u
+#define SPK_PLAN_VERSION 4u
 
 #define SPK_NL 81        /* wavelength samples 380..780 nm, 5 nm */
 #define SPK_NL3 243      /* [lambda][channel] */
@@ -93,6 +93,10 @@ typedef struct spk_plan {
     uint32_t paper_k;
     uint32_t spectral_fp16; /* 81-bin print and scan integrals as FP16 MFMA products, FP32 accumulation */
     uint32_t pad2;
+    /* Per-layer maps of the curve arguments, identity by default (a build-defined superset of the
+     * reference's single density_curve_gamma, for the differentiable render):
+     * film D_c = F_c(film_gamma_c L + film_offset_c); paper D_c = P_c(paper_gamma_c l + paper_offset_c). */
+    double film_gamma[3], film_offset[3], paper_gamma[3], paper_offset[3];
     float enl_eps[SPK_NL3], enl_beta[SPK_NL], enl_wt[SPK_NL3], enl_g[3], enl_o[3];
     double enl_eps64[SPK_NL3], enl_beta64[SPK_NL], enl_wt64[SPK_NL3], enl_g64[3], enl_o64[3];
     float k_gain;

@@ -36,20 +36,55 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "../include/spektralab_plan.h"
 #include "../spektralab_engine_amd/csrc/kernels/spk_math.h"
+#include "dual.h"
 
 namespace {
 
+using oad::Dual;
+using oad::is_dual;
+// Unqualified elementary functions: the C library's for float and double, the forward-mode rules
+// of dual.h (found by argument-dependent lookup) for Dual.
+using std::atan2;
+using std::cos;
+using std::exp;
+using std::exp2;
+using std::fabs;
+using std::floor;
+using std::fma;
+using std::fmax;
+using std::fmin;
+using std::isnan;
+using std::log;
+using std::log10;
+using std::pow;
+using std::sin;
+using std::sqrt;
+
 // ---- type-dependent elementary functions -------------------------------------------------------
 inline float o_log10(float x) { return spkm::log10f_(x); }
-inline double o_log10(double x) { return std::log10(x); }
+inline double o_log10(double x) { return log10(x); }
 inline float p_log10(float x) { return log10f(x); }
-inline double p_log10(double x) { return std::log10(x); }
+inline double p_log10(double x) { return log10(x); }
 inline float p_exp2(float x) { return exp2f(x); }
-inline double p_exp2(double x) { return std::exp2(x); }
+inline double p_exp2(double x) { return exp2(x); }
+template <int N>
+Dual<N> o_log10(const Dual<N>& x) {
+    return oad::log10(x);
+}
+template <int N>
+Dual<N> p_log10(const Dual<N>& x) {
+    return oad::log10(x);
+}
+template <int N>
+Dual<N> p_exp2(const Dual<N>& x) {
+    return oad::exp2(x);
+}
 
 template <class T>
 struct K;  // per-type constants
@@ -63,6 +98,11 @@ struct K<double> {
     static constexpr double log2_10 = 3.3219280948873623478703194294894;
     static constexpr double tiny = 1e-10;
 };
+template <int N>
+struct K<Dual<N>> {
+    static constexpr double log2_10 = 3.3219280948873623478703194294894;
+    static constexpr double tiny = 1e-10;
+};
 
 // ---- numeric helpers ----------------------------------------------------------------------------
 int reflect_index(int64_t i, int64_t n) {
@@ -73,7 +113,7 @@ int reflect_index(int64_t i, int64_t n) {
     return int(m < n ? m : period - 1 - m);
 }
 
-int fir_radius(double sigma) { return sigma > 0.0 ? int(std::floor(3.0 * sigma + 0.5)) : 0; }
+int fir_radius(double sigma) { return sigma > 0.0 ? int(floor(3.0 * sigma + 0.5)) : 0; }
 
 std::vector<double> fir_taps64(double sigma) {
     const int r = fir_radius(sigma);
@@ -82,7 +122,7 @@ std::vector<double> fir_taps64(double sigma) {
     double sum = 0.0;
     for (int k = -r; k <= r; ++k) {
         const double t = double(k) / sigma;
-        const double e = std::exp(-0.5 * t * t);
+        const double e = exp(-0.5 * t * t);
         w[size_t(k + r)] = e;
         sum += e;
     }
@@ -94,7 +134,7 @@ struct Yvv {
     double B, B1, B2, B3;
 };
 Yvv yvv(double sigma) {
-    const double q = sigma >= 2.5 ? 0.98711 * sigma - 0.96330 : 3.97156 - 4.14554 * std::sqrt(1.0 - 0.26891 * sigma);
+    const double q = sigma >= 2.5 ? 0.98711 * sigma - 0.96330 : 3.97156 - 4.14554 * sqrt(1.0 - 0.26891 * sigma);
     const double q2 = q * q, q3 = q2 * q;
     const double b0 = ((1.57825 + 2.44413 * q) + 1.4281 * q2) + 0.422205 * q3;
     const double b1 = (2.44413 * q + 2.85619 * q2) + 1.26661 * q3;
@@ -105,7 +145,7 @@ Yvv yvv(double sigma) {
 
 template <class T>
 T fma_t(T a, T b, T c) {
-    return std::fma(a, b, c);
+    return fma(a, b, c);
 }
 
 // ---- the Gaussian machinery on one channel plane (w x h, row-major) ----------------------------
@@ -175,10 +215,106 @@ void iir_plane(const T* in, T* out, const T* acc, int mode, T wt, double sigma, 
     for (size_t o = 0; o < size_t(w) * h; ++o) out[o] = fuse(mode, acc ? acc[o] : T(0), wt, col[o]);
 }
 
+// ---- the same passes on dual numbers (the differentiable reference): taps and recursion
+// coefficients as functions of a Dual sigma, float64 throughout (no binary32 storage between
+// passes); radius and branch follow the value of sigma.
+template <int N>
+void fir_plane_ad(const Dual<N>* in, Dual<N>* out, const Dual<N>* acc, int mode, Dual<N> wt, Dual<N> sigma, int w, int h) {
+    using D = Dual<N>;
+    const int r = fir_radius(sigma.v);
+    std::vector<D> taps(size_t(2 * r + 1));
+    D sum = 0.0;
+    for (int k = -r; k <= r; ++k) {
+        const D t = D(double(k)) / sigma;
+        const D e = oad::exp(D(-0.5) * t * t);
+        taps[size_t(k + r)] = e;
+        sum = sum + e;
+    }
+    for (D& e : taps) e = e / sum;
+    std::vector<D> tmp(size_t(w) * h);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            D a = 0.0;
+            for (int i = -r; i <= r; ++i) a = taps[size_t(i + r)] * in[size_t(reflect_index(y + i, h)) * w + x] + a;
+            tmp[size_t(y) * w + x] = a;
+        }
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            D a = 0.0;
+            for (int i = -r; i <= r; ++i) a = taps[size_t(i + r)] * tmp[size_t(y) * w + reflect_index(x + i, w)] + a;
+            const size_t o = size_t(y) * w + x;
+            out[o] = fuse(mode, acc ? acc[o] : D(0.0), wt, a);
+        }
+}
+
+template <int N>
+void iir_plane_ad(const Dual<N>* in, Dual<N>* out, const Dual<N>* acc, int mode, Dual<N> wt, Dual<N> sigma, int w, int h) {
+    using D = Dual<N>;
+    const D q = sigma.v >= 2.5 ? D(0.98711) * sigma - D(0.96330) : D(3.97156) - D(4.14554) * oad::sqrt(D(1.0) - D(0.26891) * sigma);
+    const D q2 = q * q, q3 = q2 * q;
+    const D b0 = ((D(1.57825) + D(2.44413) * q) + D(1.4281) * q2) + D(0.422205) * q3;
+    const D b1 = (D(2.44413) * q + D(2.85619) * q2) + D(1.26661) * q3;
+    const D b2 = -(D(1.4281) * q2 + D(1.26661) * q3);
+    const D b3 = D(0.422205) * q3;
+    const D B = D(1.0) - ((b1 + b2) + b3) / b0, B1 = b1 / b0, B2 = b2 / b0, B3 = b3 / b0;
+    auto line = [&](const D* src, D* dst, size_t n, size_t stride) {
+        std::vector<D> u(n);
+        D w1 = src[0], w2 = w1, w3 = w1;
+        for (size_t i = 0; i < n; ++i) {
+            const D t = ((B * src[i * stride] + B1 * w1) + B2 * w2) + B3 * w3;
+            w3 = w2;
+            w2 = w1;
+            w1 = t;
+            u[i] = t;
+        }
+        D y1 = u[n - 1], y2 = y1, y3 = y1;
+        for (size_t i = n; i-- > 0;) {
+            const D t = ((B * u[i] + B1 * y1) + B2 * y2) + B3 * y3;
+            y3 = y2;
+            y2 = y1;
+            y1 = t;
+            dst[i * stride] = t;
+        }
+    };
+    std::vector<D> tmp(size_t(w) * h), col(size_t(w) * h);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < h; ++y) line(in + size_t(y) * w, tmp.data() + size_t(y) * w, size_t(w), 1);
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < w; ++x) line(tmp.data() + x, col.data() + x, size_t(h), size_t(w));
+    for (size_t o = 0; o < size_t(w) * h; ++o) out[o] = fuse(mode, acc ? acc[o] : D(0.0), wt, col[o]);
+}
+
+template <int N>
+void blur3_ad(const Dual<N>* in, Dual<N>* out, const Dual<N> sigma[3], int mode, const Dual<N>* acc,
+              const Dual<N> weight[3], int w, int h, double crossover) {
+    const size_t np = size_t(w) * h;
+    for (int c = 0; c < 3; ++c) {
+        const Dual<N>* ic = in + c * np;
+        Dual<N>* oc = out + c * np;
+        const Dual<N>* ac = acc ? acc + c * np : nullptr;
+        const double s = sigma[c].v;
+        if (!(s > 0.0) || (s < crossover && fir_radius(s) == 0)) {
+            std::vector<Dual<N>> cp(ic, ic + np);
+            for (size_t i = 0; i < np; ++i) oc[i] = fuse(mode, ac ? ac[i] : Dual<N>(0.0), weight[c], cp[i]);
+        } else if (s < crossover) {
+            fir_plane_ad(ic, oc, ac, mode, weight[c], sigma[c], w, h);
+        } else {
+            iir_plane_ad(ic, oc, ac, mode, weight[c], sigma[c], w, h);
+        }
+    }
+}
+
 // Per-channel Gaussian on 3 planar planes with the reference's branch rules.
 template <class T>
 void blur3(const T* in, T* out, const double sigma[3], int mode, const T* acc, const double weight[3], int w, int h,
            double crossover) {
+    if constexpr (is_dual<T>::value) {
+        const T s3[3] = {T(sigma[0]), T(sigma[1]), T(sigma[2])}, w3[3] = {T(weight[0]), T(weight[1]), T(weight[2])};
+        blur3_ad(in, out, s3, mode, acc, w3, w, h, crossover);
+        return;
+    }
     const size_t np = size_t(w) * h;
     for (int c = 0; c < 3; ++c) {
         const T* ic = in + c * np;
@@ -208,13 +344,13 @@ int64_t mirror_fold(int64_t i, int64_t n) {  // whole-sample "c b | a b c d | c 
 
 std::vector<float> ds_taps(uint32_t n, uint32_t m) {
     const double sigma = std::max(0.0, (double(n) / double(m) - 1.0) / 2.0);
-    const int r = sigma > 0.0 ? int(std::floor(4.0 * sigma + 0.5)) : 0;
+    const int r = sigma > 0.0 ? int(floor(4.0 * sigma + 0.5)) : 0;
     if (r == 0) return {1.0f};
     std::vector<double> t(size_t(2 * r + 1));
     double sum = 0.0;
     for (int k = -r; k <= r; ++k) {
         const double u = double(k) / sigma;
-        t[size_t(k + r)] = std::exp(-0.5 * u * u);
+        t[size_t(k + r)] = exp(-0.5 * u * u);
         sum += t[size_t(k + r)];
     }
     std::vector<float> f(t.size());
@@ -238,10 +374,52 @@ void ds_samples(uint32_t n, uint32_t m, std::vector<uint32_t>& idx, std::vector<
     }
 }
 
+// ---- the render's parameters as values of the arithmetic type -------------------------------------
+// The plan quantities the differentiable render depends on.  For float and double they are the
+// plan's values as the chain always narrows them; on Dual they carry the tangents of the
+// derivative directions (the mixture sigmas and weights then become Dual too).
+template <class T>
+struct Theta {
+    using S = std::conditional_t<is_dual<T>::value, T, double>;
+    T ec;                                   // exposure-compensation gain
+    T mix_a[3], mix_b[3];                   // bounce mix raw := mix_a raw + mix_b acc
+    S w[SPK_PLAN_MAX_MIX][3], s[SPK_PLAN_MAX_MIX][3];  // bounce mixture weights and sigmas (px)
+    T fg[3], fo[3];                         // film curve argument map
+    T kg;                                   // print gain k_gain
+    T wt[SPK_NL3];                          // enlarger weights
+    T pg[3], po[3];                         // paper argument map
+    T ua, ub;                               // unsharp combine (1 + a), -a
+};
+
+template <class T>
+Theta<T> theta_of(const spk_plan& p) {
+    const bool f32 = sizeof(T) == 4;
+    Theta<T> t{};
+    t.ec = f32 ? T(p.ec_gain) : T(double(p.ec_gain));
+    for (int c = 0; c < 3; ++c) {
+        t.mix_a[c] = T(float(p.mix_a[c]));
+        t.mix_b[c] = T(float(p.mix_b[c]));
+        t.fg[c] = f32 ? T(float(p.film_gamma[c])) : T(p.film_gamma[c]);
+        t.fo[c] = f32 ? T(float(p.film_offset[c])) : T(p.film_offset[c]);
+        t.pg[c] = f32 ? T(float(p.paper_gamma[c])) : T(p.paper_gamma[c]);
+        t.po[c] = f32 ? T(float(p.paper_offset[c])) : T(p.paper_offset[c]);
+    }
+    for (uint32_t k = 0; k < SPK_PLAN_MAX_MIX; ++k)
+        for (int c = 0; c < 3; ++c) {
+            t.w[k][c] = typename Theta<T>::S(p.bounce[k].weight[c]);
+            t.s[k][c] = typename Theta<T>::S(p.bounce[k].sigma[c]);
+        }
+    t.kg = f32 ? T(p.k_gain) : T(p.k_gain64);
+    for (int i = 0; i < SPK_NL3; ++i) t.wt[i] = f32 ? T(p.enl_wt[i]) : T(p.enl_wt64[i]);
+    t.ua = f32 ? T(float(1.0 + p.unsharp_amount)) : T(1.0 + p.unsharp_amount);
+    t.ub = f32 ? T(float(-p.unsharp_amount)) : T(-p.unsharp_amount);
+    return t;
+}
+
 // ---- film side -----------------------------------------------------------------------------------
 template <class T>
 T mitchell(T u) {
-    const T a = std::fabs(u);
+    const T a = fabs(u);
     if (a < T(1)) return (T(1) / T(6)) * (T(7) * a * a * a - T(12) * a * a + T(16) / T(3));
     if (a < T(2)) return (T(1) / T(6)) * ((T(-7) / T(3)) * a * a * a + T(12) * a * a - T(20) * a + T(32) / T(3));
     return T(0);
@@ -258,7 +436,7 @@ void mitchell_axis(T t, uint32_t L, int* idx, T* w) {
         base = int(L) - 2;
         frac = T(1);
     } else {
-        base = int(std::floor(u));
+        base = int(floor(u));
         frac = u - T(base);
     }
     const int n = int(L);
@@ -359,7 +537,7 @@ float lgamma_k(float x) {
 float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t seed) {
     Stream g{pixel, stream, seed};
     if (lam >= 10.0f) {
-        const float slam = std::sqrt(lam);
+        const float slam = sqrt(lam);
         const float loglam = spkm::logf_(lam);
         const float b = 0.931f + (2.53f * slam);
         const float a = -0.059f + (0.02483f * b);
@@ -368,15 +546,15 @@ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t seed) {
         for (int it = 0; it < 64; ++it) {
             const float U = g.uniform() - 0.5f;
             const float V = g.uniform();
-            const float us = 0.5f - std::fabs(U);
-            const float k = std::floor(((((2.0f * a) / us + b) * U + lam) + 0.43f));
+            const float us = 0.5f - fabs(U);
+            const float k = floor(((((2.0f * a) / us + b) * U + lam) + 0.43f));
             if (us >= 0.07f && V <= vr) return k;
             if (k < 0.0f || (us < 0.013f && V > us)) continue;
             const float L = ((spkm::logf_(V) + spkm::logf_(invalpha)) - spkm::logf_(a / (us * us) + b));
             const float Rt = ((-lam + k * loglam) - lgamma_k(k + 1.0f));
             if (L <= Rt) return k;
         }
-        return std::floor(lam);
+        return floor(lam);
     }
     const float E = spkm::expf_(-lam);
     float X = 0.0f, prod = 1.0f;
@@ -388,16 +566,19 @@ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t seed) {
     return X;
 }
 
-float sublayer_draw(float d, float D, float n, float u, uint32_t pixel, uint32_t stream, uint32_t seed) {
+float sublayer_draw(float d, float D, float n, float u, uint32_t pixel, uint32_t stream, uint32_t seed,
+                    float* count_out = nullptr) {
     float q = d / D;
     q = q < 1e-6f ? 1e-6f : (q > 0.999999f ? 0.999999f : q);
     const float sat = 1.0f - ((q * u) * 0.999999f);
     const float lam = (n / sat) * q;
-    return poisson(lam, pixel, stream, seed) * ((D / n) * sat);
+    const float k = poisson(lam, pixel, stream, seed);
+    if (count_out) *count_out = k;
+    return k * ((D / n) * sat);
 }
 
 // Grain SUM draw of one pixel and channel from its density D (binary32 by definition).
-float grain_sum(const spk_plan& p, float D, uint32_t pixel, uint32_t c) {
+float grain_sum(const spk_plan& p, float D, uint32_t pixel, uint32_t c, float* counts9 = nullptr) {
     const float v = p.grain_positive ? -D : D;
     const uint32_t K = p.grain_k;
     const float* axis = p.grain_axis;
@@ -426,28 +607,85 @@ float grain_sum(const spk_plan& p, float D, uint32_t pixel, uint32_t c) {
     float val[3];
     for (uint32_t l = 0; l < 3; ++l) {
         const uint32_t lc = l * 3 + c;
-        val[l] = sublayer_draw(d[l] + p.dmin_l[lc], p.sat_d[lc], p.count[lc], p.uniformity[c], pixel, c + 10u * l, p.grain_seed);
+        val[l] = sublayer_draw(d[l] + p.dmin_l[lc], p.sat_d[lc], p.count[lc], p.uniformity[c], pixel, c + 10u * l, p.grain_seed,
+                               counts9 ? counts9 + 3 * c + l : nullptr);
     }
     float G = ((0.0f + val[0]) + val[1]) + val[2];
     return G * 1.0f + (-p.dmin_c[c]);
 }
 
+// The grain sum with the Poisson counts frozen (k9[3 c + l], as a render drew them): each
+// sub-layer contributes k (D_l / n) sat, sat = 1 - clamp(d / D_l) u 0.999999 (fx_random.h:91-99),
+// so the result depends on the density through sat and the layer curves only.
 template <class T>
-void develop(const spk_plan& p, const float* rgb, T* neg) {
+T grain_frozen(const spk_plan& p, T D, const float* k9, uint32_t c) {
+    const T v = p.grain_positive ? -D : D;
+    const uint32_t K = p.grain_k;
+    const float* axis = p.grain_axis;
+    const float* inv = p.grain_inv;
+    const float* layers = p.grain_layers;
+    T d[3];
+    if (v <= T(axis[c])) {
+        for (uint32_t l = 0; l < 3; ++l) d[l] = T(layers[l * 3 + c]);
+    } else if (v >= T(axis[3 * (K - 1) + c])) {
+        for (uint32_t l = 0; l < 3; ++l) d[l] = T(layers[((K - 1) * 3 + l) * 3 + c]);
+    } else {
+        uint32_t lo = 0, hi = K;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (T(axis[3 * mid + c]) <= v) lo = mid + 1;
+            else hi = mid;
+        }
+        const uint32_t l0 = lo > 0 ? lo - 1 : 0;
+        const T t = (v - T(axis[3 * l0 + c])) * T(inv[3 * l0 + c]);
+        for (uint32_t l = 0; l < 3; ++l) {
+            const T y0 = T(layers[(l0 * 3 + l) * 3 + c]);
+            const T y1 = T(layers[((l0 + 1) * 3 + l) * 3 + c]);
+            d[l] = y0 + t * (y1 - y0);
+        }
+    }
+    T val[3];
+    for (uint32_t l = 0; l < 3; ++l) {
+        const uint32_t lc = l * 3 + c;
+        const T dl = d[l] + T(p.dmin_l[lc]), Dl = T(p.sat_d[lc]), n = T(p.count[lc]);
+        T qq = dl / Dl;
+        qq = qq < T(1e-6f) ? T(1e-6f) : (qq > T(0.999999f) ? T(0.999999f) : qq);
+        const T sat = T(1) - ((qq * T(p.uniformity[c])) * T(0.999999f));
+        val[l] = T(double(k9[3 * c + l])) * ((Dl / n) * sat);
+    }
+    const T G = ((T(0) + val[0]) + val[1]) + val[2];
+    return G * T(1) + (-T(p.dmin_c[c]));
+}
+
+// rgb: the tier image (In = float, or Dual for derivatives with respect to the pixels).
+// thp: the parameters (null: the plan's).  counts_in: frozen Poisson counts, 9 per pixel
+// ([i * 9 + 3 c + l]); counts_out: records the counts the binary32 draw makes.
+template <class T, class In = float>
+void develop(const spk_plan& p, const In* rgb, T* neg, const Theta<T>* thp = nullptr, const float* counts_in = nullptr,
+             float* counts_out = nullptr) {
     const int w = int(p.width), h = int(p.height);
     const size_t np = size_t(w) * h;
+    std::unique_ptr<Theta<T>> own;
+    if (!thp) {
+        own = std::make_unique<Theta<T>>(theta_of<T>(p));
+        thp = own.get();
+    }
+    const Theta<T>& th = *thp;
     std::vector<T> H(3 * np);
-    const T* table = table_of<T>(p);
+    auto tabv = [&](size_t k) -> T {
+        if constexpr (sizeof(T) == 4) return T(p.table[k]);
+        else return T(p.table64[k]);
+    };
     T a[9];
     for (int i = 0; i < 9; ++i) a[i] = sizeof(T) == 4 ? T(p.a[i]) : T(p.a64[i]);
-    const T ec = sizeof(T) == 4 ? T(p.ec_gain) : T(double(p.ec_gain));
+    const T ec = th.ec;
     const bool ec_active = p.ec_gain != 1.0f;
     // auto exposure: the tier image scaled by f32(2^ev) before the upsample (develop.cpp:125-131)
-    const T ae = sizeof(T) == 4 ? T(p.ae_gain) : T(std::exp2(p.ae_ev));
+    const T ae = sizeof(T) == 4 ? T(p.ae_gain) : T(exp2(p.ae_ev));
     const uint32_t L = p.L;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < int64_t(np); ++i) {
-        T r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+        T r = T(rgb[3 * i]), g = T(rgb[3 * i + 1]), b = T(rgb[3 * i + 2]);
         if (p.ae_active) {
             r = r * ae;
             g = g * ae;
@@ -459,9 +697,9 @@ void develop(const spk_plan& p, const float* rgb, T* neg) {
         const T bs = (X + Y) + Z;
         const T d = bs > T(1e-10) ? bs : T(1e-10);
         const T x = X / d, y = Y / d, o = T(1) - x;
-        const T t0 = std::fmin(std::fmax(o * o, T(0)), T(1));
-        const T t1 = std::fmin(std::fmax(y / (o > T(1e-10) ? o : T(1e-10)), T(0)), T(1));
-        const T bp = std::isnan(bs) ? T(0) : bs;
+        const T t0 = fmin(fmax(o * o, T(0)), T(1));
+        const T t1 = fmin(fmax(y / (o > T(1e-10) ? o : T(1e-10)), T(0)), T(1));
+        const T bp = isnan(bs) ? T(0) : bs;
         int ia[4], ib[4];
         T wa[4], wb[4];
         mitchell_axis(t0, L, ia, wa);
@@ -470,11 +708,11 @@ void develop(const spk_plan& p, const float* rgb, T* neg) {
         for (int ii = 0; ii < 4; ++ii)
             for (int jj = 0; jj < 4; ++jj) {
                 const T wgt = wa[ii] * wb[jj];
-                const T* t = table + 3 * (size_t(ia[ii]) * L + size_t(ib[jj]));
+                const size_t tb = 3 * (size_t(ia[ii]) * L + size_t(ib[jj]));
                 s += wgt;
-                a0 += wgt * t[0];
-                a1 += wgt * t[1];
-                a2 += wgt * t[2];
+                a0 += wgt * tabv(tb);
+                a1 += wgt * tabv(tb + 1);
+                a2 += wgt * tabv(tb + 2);
             }
         if (s != T(0)) {
             a0 = a0 / s;
@@ -504,7 +742,13 @@ void develop(const spk_plan& p, const float* rgb, T* neg) {
             for (size_t i = 0; i < np; ++i) H[c * np + i] = sa * H[c * np + i] + sb * acc[c * np + i];
         }
     }
-    if (p.bounce_active) mixture(p.bounce, p.n_bounce);
+    if (p.bounce_active)
+        for (uint32_t k = 0; k < p.n_bounce; ++k) {
+            if constexpr (is_dual<T>::value)
+                blur3_ad(H.data(), acc.data(), th.s[k], k == 0 ? 1 : 2, acc.data(), th.w[k], w, h, p.fir_crossover);
+            else
+                blur3<T>(H.data(), acc.data(), th.s[k], k == 0 ? 1 : 2, acc.data(), th.w[k], w, h, p.fir_crossover);
+        }
     const uint32_t K = p.K;
     std::vector<T> cx(K), cinv(K), cy(3 * K);
     for (uint32_t k = 0; k < K; ++k) {
@@ -513,6 +757,9 @@ void develop(const spk_plan& p, const float* rgb, T* neg) {
                                  : (k + 1 < K && p.curves_x64[k + 1] != p.curves_x64[k] ? T(1.0 / (p.curves_x64[k + 1] - p.curves_x64[k])) : T(0));
     }
     for (uint32_t i = 0; i < 3 * K; ++i) cy[i] = sizeof(T) == 4 ? T(p.curves[2 * K + i]) : T(p.curves_y64[i]);
+    // the per-layer curve argument map fg L + fo (identity by default), rounded as the kernel does
+    const T* fg = th.fg;
+    const T* fo = th.fo;
     // DIR couplers (develop.cpp:182-222, film.metal:264-292): L and the first development D1;
     // chi_m := sum_c s_c M[c][m], s = D1 (dmax - D1 on a positive); chi diffused by the DIR
     // mixture; D := F(L - chi; y0)
@@ -529,10 +776,10 @@ void develop(const spk_plan& p, const float* rgb, T* neg) {
             for (int c = 0; c < 3; ++c) {
                 const size_t o = size_t(c) * np + size_t(i);
                 T v = H[o];
-                if (p.bounce_active) v = T(float(p.mix_a[c])) * v + T(float(p.mix_b[c])) * acc[o];
-                const T Lg = o_log10(std::fmax(v, T(0)) + T(1e-10));
+                if (p.bounce_active) v = th.mix_a[c] * v + th.mix_b[c] * acc[o];
+                const T Lg = o_log10(fmax(v, T(0)) + T(1e-10));
                 Lc[o] = Lg;
-                const T D1 = curve_lookup(cx.data(), cinv.data(), cy.data() + c * K, K, Lg);
+                const T D1 = curve_lookup(cx.data(), cinv.data(), cy.data() + c * K, K, fg[c] * Lg + fo[c]);
                 s[c] = p.coupler_positive ? T(p.coupler_dmax[c]) - D1 : D1;
             }
             for (int m = 0; m < 3; ++m)
@@ -546,24 +793,27 @@ void develop(const spk_plan& p, const float* rgb, T* neg) {
     }
     std::vector<T> G(p.grain_active ? 3 * np : 0), D(p.grain_active ? 3 * np : 0);
     for (int c = 0; c < 3; ++c) {
-        const T ma = T(float(p.mix_a[c])), mb = T(float(p.mix_b[c]));
+        const T ma = th.mix_a[c], mb = th.mix_b[c];
 #pragma omp parallel for schedule(dynamic, 4096)
         for (int64_t i = 0; i < int64_t(np); ++i) {
             const size_t o = size_t(c) * np + size_t(i);
             T Dv;
             if (p.couplers_active) {
-                Dv = curve_lookup(cx.data(), cinv.data(), cy0.data() + c * K, K, Lc[o] - chib[o]);
+                Dv = curve_lookup(cx.data(), cinv.data(), cy0.data() + c * K, K, fg[c] * (Lc[o] - chib[o]) + fo[c]);
             } else {
                 T v = H[o];
                 if (p.bounce_active) v = ma * v + mb * acc[o];
-                const T Lg = o_log10(std::fmax(v, T(0)) + T(1e-10));
-                Dv = curve_lookup(cx.data(), cinv.data(), cy.data() + c * K, K, Lg);
+                const T Lg = o_log10(fmax(v, T(0)) + T(1e-10));
+                Dv = curve_lookup(cx.data(), cinv.data(), cy.data() + c * K, K, fg[c] * Lg + fo[c]);
             }
             if (!p.grain_active) {
                 neg[o] = Dv;
             } else {
                 D[o] = Dv;
-                G[o] = T(grain_sum(p, float(Dv), uint32_t(i), uint32_t(c)));
+                if (counts_in)
+                    G[o] = grain_frozen<T>(p, Dv, counts_in + 9 * size_t(i), uint32_t(c));
+                else
+                    G[o] = T(grain_sum(p, float(Dv), uint32_t(i), uint32_t(c), counts_out ? counts_out + 9 * size_t(i) : nullptr));
             }
         }
     }
@@ -608,12 +858,12 @@ template <class T>
 T knee(T d, T tau, T lim, T pw) {
     if (d <= tau) return d;
     const T x = (d - tau) / (lim - tau);
-    return tau + (lim - tau) * (x / std::pow(T(1) + std::pow(x, pw), T(1) / pw));
+    return tau + (lim - tau) * (x / pow(T(1) + pow(x, pw), T(1) / pw));
 }
 
 template <class T>
 T post_adapt(T v, T f_l) {
-    const T f = std::pow((f_l * std::fabs(v)) / T(100), T(0.42));
+    const T f = pow((f_l * fabs(v)) / T(100), T(0.42));
     return ((T(400) * sgn(v)) * f) / (T(27.13) + f) + T(0.1);
 }
 
@@ -627,36 +877,36 @@ void cam16_forward(const T xyz[3], const Cam<T>& q, T out[3]) {
     const T ba = post_adapt(q.d_rgb[2] * rgb[2], q.f_l);
     const T a = (ra - (T(12) * ga) / T(11)) + ba / T(11);
     const T b = ((ra + ga) - T(2) * ba) / T(9);
-    const T h = std::atan2(b, a);
-    const T et = T(0.25) * (std::cos(h + T(2)) + T(3.8));
+    const T h = atan2(b, a);
+    const T et = T(0.25) * (cos(h + T(2)) + T(3.8));
     const T A = (((T(2) * ra + ga) + ba / T(20)) - T(0.305)) * q.n_bb;
     const T ratio = A / q.a_w;
-    const T J = (T(100) * sgn(ratio)) * std::pow(std::fabs(ratio), q.cz);
+    const T J = (T(100) * sgn(ratio)) * pow(fabs(ratio), q.cz);
     const T den = (ra + ga) + (T(21) * ba) / T(20);
-    const T t = den != T(0) ? (((((T(50000) / T(13)) * q.n_c) * q.n_cb) * et) * std::sqrt(a * a + b * b)) / den : T(0);
-    const T C = t > T(0) ? (std::pow(t, T(0.9)) * std::sqrt(std::fabs(J) / T(100))) * q.chroma_k : T(0);
+    const T t = den != T(0) ? (((((T(50000) / T(13)) * q.n_c) * q.n_cb) * et) * sqrt(a * a + b * b)) / den : T(0);
+    const T C = t > T(0) ? (pow(t, T(0.9)) * sqrt(fabs(J) / T(100))) * q.chroma_k : T(0);
     const T M = C * q.f_l_qrt;
-    out[1] = std::log(T(1) + T(0.0228) * M) / T(0.0228);
+    out[1] = log(T(1) + T(0.0228) * M) / T(0.0228);
     out[0] = (T(1.7) * J) / (T(1) + T(0.007) * J);
     out[2] = h;
 }
 
 template <class T>
 void cam16_inverse(T Jp, T Mp, T h, const Cam<T>& q, T out[3]) {
-    const T M = (std::exp(T(0.0228) * Mp) - T(1)) / T(0.0228);
+    const T M = (exp(T(0.0228) * Mp) - T(1)) / T(0.0228);
     const T J = Jp / (T(1.7) - T(0.007) * Jp);
     const T C = M / q.f_l_qrt;
-    const T A = (q.a_w * sgn(J)) * std::pow(std::fabs(J / T(100)), q.inv_cz);
-    const T sq = std::sqrt(std::fabs(J) / T(100));
-    const T t = (sq > T(0) && C > T(0)) ? std::pow(C / (sq * q.chroma_k), T(1) / T(0.9)) : T(0);
-    const T et = T(0.25) * (std::cos(h + T(2)) + T(3.8));
+    const T A = (q.a_w * sgn(J)) * pow(fabs(J / T(100)), q.inv_cz);
+    const T sq = sqrt(fabs(J) / T(100));
+    const T t = (sq > T(0) && C > T(0)) ? pow(C / (sq * q.chroma_k), T(1) / T(0.9)) : T(0);
+    const T et = T(0.25) * (cos(h + T(2)) + T(3.8));
     const T p2 = A / q.n_bb + T(0.305);
     const T p3 = T(21) / T(20);
     T a = 0, b = 0;
     if (t != T(0)) {
         const T p1 = ((((T(50000) / T(13)) * q.n_c) * q.n_cb) * et) / t;
-        const T sh = std::sin(h), ch = std::cos(h);
-        if (std::fabs(sh) >= std::fabs(ch)) {
+        const T sh = sin(h), ch = cos(h);
+        if (fabs(sh) >= fabs(ch)) {
             const T p4 = p1 / sh;
             b = ((p2 * (T(2) + p3)) * (T(460) / T(1403))) /
                 (((p4 + ((T(2) + p3) * (T(220) / T(1403))) * (ch / sh)) - T(27) / T(1403)) + p3 * (T(6300) / T(1403)));
@@ -673,9 +923,9 @@ void cam16_inverse(T Jp, T Mp, T h, const Cam<T>& q, T out[3]) {
     T rc[3];
     for (int i = 0; i < 3; ++i) {
         const T v = xa[i] - T(0.1);
-        const T av = std::fabs(v);
+        const T av = fabs(v);
         const T beta = av >= T(400) ? T(0) : (T(27.13) * av) / (T(400) - av);
-        rc[i] = ((q.hundred_over_f_l * sgn(v)) * std::pow(beta, T(1) / T(0.42))) / q.d_rgb[i];
+        rc[i] = ((q.hundred_over_f_l * sgn(v)) * pow(beta, T(1) / T(0.42))) / q.d_rgb[i];
     }
     T x100[3];
     mat_apply(q.m16_inv, rc, x100);
@@ -687,25 +937,25 @@ void cam16_inverse(T Jp, T Mp, T h, const Cam<T>& q, T out[3]) {
 template <class T>
 T cctf_encode(T v, uint32_t mode) {
     switch (mode) {
-        case 0: return v <= T(0.0031308) ? T(12.92) * v : T(1.055) * (sgn(v) * std::pow(std::fabs(v), T(0.41666666666666669))) - T(0.055);
-        case 1: return v < T(0.001953125) ? T(16) * v : std::pow(v, T(0.55555555555555558));
-        case 2: return std::pow(v, T(0.45470692717584371));
-        case 3: return v < T(0.018) ? T(4.5) * v : T(1.099) * (sgn(v) * std::pow(std::fabs(v), T(0.45))) - T(0.099);
+        case 0: return v <= T(0.0031308) ? T(12.92) * v : T(1.055) * (sgn(v) * pow(fabs(v), T(0.41666666666666669))) - T(0.055);
+        case 1: return v < T(0.001953125) ? T(16) * v : pow(v, T(0.55555555555555558));
+        case 2: return pow(v, T(0.45470692717584371));
+        case 3: return v < T(0.018) ? T(4.5) * v : T(1.099) * (sgn(v) * pow(fabs(v), T(0.45))) - T(0.099);
         default: return v;
     }
 }
 
 template <class T>
 uint16_t encode16(T v) {
-    if (std::isnan(v)) return 0;
-    const T c = std::fmin(std::fmax(v, T(0)), T(1));
-    return uint16_t(std::floor(c * T(65535) + T(0.5)));
+    if (isnan(v)) return 0;
+    const T c = fmin(fmax(v, T(0)), T(1));
+    return uint16_t(floor(c * T(65535) + T(0.5)));
 }
 
 template <class T>
 T paper_lookup(const spk_plan& p, uint32_t c, T v) {
     const uint32_t K = p.paper_k;
-    if (std::isnan(v)) return v;
+    if (isnan(v)) return v;
     auto X = [&](uint32_t i) { return sizeof(T) == 4 ? T(p.paper[i]) : T(p.paper_x64[i]); };
     auto Y = [&](uint32_t i, uint32_t ch) { return sizeof(T) == 4 ? T(p.paper[2 * K + 3 * i + ch]) : T(p.paper_y64[3 * i + ch]); };
     auto R = [&](uint32_t i) {
@@ -726,16 +976,25 @@ T paper_lookup(const spk_plan& p, uint32_t c, T v) {
     return y0 + ((v - X(j)) * R(j)) * (Y(j + 1, c) - y0);
 }
 
+// thp: the parameters (null: the plan's); enc_out: the encoded value before the 16-bit
+// quantisation (3 planes).  out may be null.
 template <class T>
-void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
+void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out, const Theta<T>* thp = nullptr,
+           T* enc_out = nullptr) {
     const int w = int(p.width), h = int(p.height);
     const size_t np = size_t(w) * h;
     const bool f32 = sizeof(T) == 4;
+    std::unique_ptr<Theta<T>> own;
+    if (!thp) {
+        own = std::make_unique<Theta<T>>(theta_of<T>(p));
+        thp = own.get();
+    }
+    const Theta<T>& th = *thp;
     auto pick = [&](float a, double b) { return f32 ? T(a) : T(b); };
     T eps[SPK_NL3], beta[SPK_NL], wt[SPK_NL3], g[3], o[3], seps[SPK_NL3], sbeta[SPK_NL], swt[SPK_NL3], to_rgb[9], m_nid[9];
     for (int i = 0; i < SPK_NL3; ++i) {
         eps[i] = pick(p.enl_eps[i], p.enl_eps64[i]);
-        wt[i] = pick(p.enl_wt[i], p.enl_wt64[i]);
+        wt[i] = th.wt[i];
         seps[i] = pick(p.scan_eps[i], p.scan_eps64[i]);
         swt[i] = pick(p.scan_wt[i], p.scan_wt64[i]);
     }
@@ -751,7 +1010,9 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
         to_rgb[i] = pick(p.to_rgb[i], p.to_rgb64[i]);
         m_nid[i] = pick(p.m_nid[i], p.m_nid64[i]);
     }
-    const T kg = pick(p.k_gain, p.k_gain64);
+    const T kg = th.kg;
+    const T* pg = th.pg;
+    const T* po = th.po;
     Cam<T> q{};
     for (int i = 0; i < 9; ++i) {
         q.m_to_xyz[i] = pick(p.m_to_xyz[i], p.m_to_xyz64[i]);
@@ -784,8 +1045,8 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
     q.j_white = kk[18];
     q.cz = q.c * q.z;
     q.inv_cz = T(1) / q.cz;
-    q.f_l_qrt = std::pow(q.f_l, T(0.25));
-    q.chroma_k = std::pow(T(1.64) - std::pow(T(0.29), q.n), T(0.73));
+    q.f_l_qrt = pow(q.f_l, T(0.25));
+    q.chroma_k = pow(T(1.64) - pow(T(0.29), q.n), T(0.73));
     q.hundred_over_f_l = T(100) / q.f_l;
     const int rows = SPK_CMAX_ROWS, cols = SPK_CMAX_COLS;
     const T L2 = K<T>::log2_10, tiny = K<T>::tiny;
@@ -799,10 +1060,10 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
         for (int64_t i = 0; i < int64_t(np); ++i) {
             Stream gs{uint32_t(i), 200u, p.glare_seed};
             const float u1 = gs.uniform(), u2 = gs.uniform();
-            const T z = std::sqrt(T(-2) * std::log(T(u1))) * std::cos(T(6.283185307179586f) * T(u2));
-            const T sig = std::sqrt(std::log(T(1) + (sd * sd) / (m * m)));
-            const T mu = std::log(m) - (T(0.5) * sig) * sig;
-            const T v = std::exp(mu + sig * z);
+            const T z = sqrt(T(-2) * log(T(u1))) * cos(T(6.283185307179586f) * T(u2));
+            const T sig = sqrt(log(T(1) + (sd * sd) / (m * m)));
+            const T mu = log(m) - (T(0.5) * sig) * sig;
+            const T v = exp(mu + sig * z);
             f3[i] = v;
             f3[np + i] = v;
             f3[2 * np + i] = v;
@@ -824,15 +1085,16 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
                 r1 += Tl * wt[3 * l + 1];
                 r2 += Tl * wt[3 * l + 2];
             }
-            T l0 = p_log10(std::fmax(r0 * g[0] + o[0], T(0)) + tiny);
-            T l1 = p_log10(std::fmax(r1 * g[1] + o[1], T(0)) + tiny);
-            T l2 = p_log10(std::fmax(r2 * g[2] + o[2], T(0)) + tiny);
-            l0 = p_log10(std::fmax(p_exp2(l0 * L2) * kg, T(0)) + tiny);
-            l1 = p_log10(std::fmax(p_exp2(l1 * L2) * kg, T(0)) + tiny);
-            l2 = p_log10(std::fmax(p_exp2(l2 * L2) * kg, T(0)) + tiny);
-            d0 = paper_lookup<T>(p, 0, l0);
-            d1 = paper_lookup<T>(p, 1, l1);
-            d2 = paper_lookup<T>(p, 2, l2);
+            T l0 = p_log10(fmax(r0 * g[0] + o[0], T(0)) + tiny);
+            T l1 = p_log10(fmax(r1 * g[1] + o[1], T(0)) + tiny);
+            T l2 = p_log10(fmax(r2 * g[2] + o[2], T(0)) + tiny);
+            l0 = p_log10(fmax(p_exp2(l0 * L2) * kg, T(0)) + tiny);
+            l1 = p_log10(fmax(p_exp2(l1 * L2) * kg, T(0)) + tiny);
+            l2 = p_log10(fmax(p_exp2(l2 * L2) * kg, T(0)) + tiny);
+            // the per-layer argument map (identity by default), rounded as the kernel does
+            d0 = paper_lookup<T>(p, 0, pg[0] * l0 + po[0]);
+            d1 = paper_lookup<T>(p, 1, pg[1] * l1 + po[1]);
+            d2 = paper_lookup<T>(p, 2, pg[2] * l2 + po[2]);
         }
         T xyz[3];
         {
@@ -844,9 +1106,9 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
                 r1 += Tl * swt[3 * l + 1];
                 r2 += Tl * swt[3 * l + 2];
             }
-            xyz[0] = std::fmax(r0, T(0)) + tiny;
-            xyz[1] = std::fmax(r1, T(0)) + tiny;
-            xyz[2] = std::fmax(r2, T(0)) + tiny;
+            xyz[0] = fmax(r0, T(0)) + tiny;
+            xyz[1] = fmax(r1, T(0)) + tiny;
+            xyz[2] = fmax(r2, T(0)) + tiny;
         }
         if (p.glare_active)
             for (int c = 0; c < 3; ++c) xyz[c] = xyz[c] + (glare[i] / T(100)) * T(p.glare_tint[c]);
@@ -865,12 +1127,12 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
                     Jp = Ln * q.j_white;
                 }
             }
-            const T Jc = std::fmin(std::fmax(Jp, q.j_first), q.j_last);
+            const T Jc = fmin(fmax(Jp, q.j_first), q.j_last);
             const T u = ((Jc - q.j_first) / (q.j_last - q.j_first)) * T(rows - 1);
-            const int l0 = std::min(std::max(int(std::floor(u)), 0), rows - 2);
+            const int l0 = std::min(std::max(int(floor(u)), 0), rows - 2);
             const T fl = u - T(l0);
             const T v = (hh - q.h_first) / q.h_step;
-            const T vf = std::floor(v);
+            const T vf = floor(v);
             int h0 = int(vf) % cols;
             if (h0 < 0) h0 += cols;
             const int h1 = h0 == cols - 1 ? 0 : h0 + 1;
@@ -879,7 +1141,7 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
             const T t00 = cm(l0 * cols + h0), t01 = cm(l0 * cols + h1), t10 = cm((l0 + 1) * cols + h0),
                     t11 = cm((l0 + 1) * cols + h1);
             const T cmv = (T(1) - fl) * ((T(1) - fh) * t00 + fh * t01) + fl * ((T(1) - fh) * t10 + fh * t11);
-            const T cs = std::fmax(cmv, T(1e-9));
+            const T cs = fmax(cmv, T(1e-9));
             T dd = Mp / cs;
             if (dd > T(0)) dd = knee(dd, q.knee[0], q.knee[1], q.knee[2]);
             T back[3];
@@ -901,8 +1163,7 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
         const double su[3] = {p.unsharp_sigma, p.unsharp_sigma, p.unsharp_sigma};
         std::vector<T> gb(3 * np);
         blur3<T>(rgb.data(), gb.data(), su, 0, nullptr, one, w, h, p.fir_crossover);
-        const T ua = pick(float(1.0 + p.unsharp_amount), 1.0 + p.unsharp_amount);
-        const T ub = pick(float(-p.unsharp_amount), -p.unsharp_amount);
+        const T ua = th.ua, ub = th.ub;
         for (size_t i = 0; i < 3 * np; ++i) rgb[i] = ua * rgb[i] + ub * gb[i];
     }
 #pragma omp parallel for schedule(static)
@@ -915,14 +1176,107 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out) {
             mat_apply(m_nid, v, m);
             for (int c = 0; c < 3; ++c) v[c] = cctf_encode(m[c], p.cctf_mode);
         }
-        for (int c = 0; c < 3; ++c) out[4 * i + c] = encode16(v[c]);
-        out[4 * i + 3] = 65535;
+        if (enc_out)
+            for (int c = 0; c < 3; ++c) enc_out[c * np + i] = v[c];
+        if constexpr (!is_dual<T>::value) {
+            if (out) {
+                for (int c = 0; c < 3; ++c) out[4 * i + c] = encode16(v[c]);
+                out[4 * i + 3] = 65535;
+            }
+        }
     }
+}
+
+// ---- the differentiable (frozen-count) render: float64 value and forward-mode derivatives ----------
+// Quantity index of each derivative direction (the GPU adjoint reports the same quantities):
+//   0 ec gain; 1-3 mix_a; 4-6 mix_b; 7-15 bounce weights [k][c] (k < 3); 16-24 bounce sigmas [k][c];
+//   25-27 film gamma; 28-30 film offset; 31 k_gain; 32-34 paper gamma; 35-37 paper offset;
+//   38 unsharp amount a (the combine (1 + a) x - a G(x)); 39 the enlarger weights along wt_dir;
+//   40 + j the input pixels along in_dirs[j] (j < 8).
+constexpr int kAdN = 48, kAdQuantities = 40, kAdInputDirs = 8;
+using AD = Dual<kAdN>;
+
+template <class T>
+void perturb(Theta<T>& th, int idx, const T& amount, const double* wt_dir) {
+    if (idx == 0) th.ec = th.ec + amount;
+    else if (idx < 4) th.mix_a[idx - 1] = th.mix_a[idx - 1] + amount;
+    else if (idx < 7) th.mix_b[idx - 4] = th.mix_b[idx - 4] + amount;
+    else if (idx < 16) th.w[(idx - 7) / 3][(idx - 7) % 3] = th.w[(idx - 7) / 3][(idx - 7) % 3] + amount;
+    else if (idx < 25) th.s[(idx - 16) / 3][(idx - 16) % 3] = th.s[(idx - 16) / 3][(idx - 16) % 3] + amount;
+    else if (idx < 28) th.fg[idx - 25] = th.fg[idx - 25] + amount;
+    else if (idx < 31) th.fo[idx - 28] = th.fo[idx - 28] + amount;
+    else if (idx == 31) th.kg = th.kg + amount;
+    else if (idx < 35) th.pg[idx - 32] = th.pg[idx - 32] + amount;
+    else if (idx < 38) th.po[idx - 35] = th.po[idx - 35] + amount;
+    else if (idx == 38) {
+        th.ua = th.ua + amount;
+        th.ub = th.ub - amount;
+    } else if (idx == 39) {
+        for (int i = 0; i < SPK_NL3; ++i) th.wt[i] = th.wt[i] + amount * T(wt_dir[i]);
+    }
+}
+
+// mean over pixels and channels of (enc - target)^2; target interleaved w x h x 3
+template <class T>
+T l2_loss(const std::vector<T>& enc, const float* target, size_t np) {
+    T acc = T(0);
+    for (size_t i = 0; i < np; ++i)
+        for (int c = 0; c < 3; ++c) {
+            const T d = enc[size_t(c) * np + i] - T(double(target[3 * i + c]));
+            acc = acc + d * d;
+        }
+    return acc / T(double(3 * np));
 }
 
 }  // namespace
 
 extern "C" {
+
+// The Poisson counts the binary32 develop draws, 9 per pixel ([i * 9 + 3 c + l]), as floats.
+void oracle_grain_counts(const spk_plan* p, const float* rgb, float* counts) {
+    std::vector<float> neg(size_t(p->width) * p->height * 3);
+    develop<float>(*p, rgb, neg.data(), nullptr, nullptr, counts);
+}
+
+// The frozen-count render in float64 with quantity idx moved by h (idx < 0: none; 40 + j moves the
+// pixels along in_dir); returns the L2 loss against target and writes the encoded output (3 planes)
+// when enc is not null.
+double oracle_frozen_f64(const spk_plan* p, const float* rgb, const float* counts, const float* target, int idx,
+                         double h, const double* wt_dir, const double* in_dir, double* enc) {
+    const size_t np = size_t(p->width) * p->height;
+    Theta<double> th = theta_of<double>(*p);
+    if (idx >= 0 && idx < kAdQuantities) perturb(th, idx, h, wt_dir);
+    std::vector<double> in(3 * np);
+    for (size_t i = 0; i < 3 * np; ++i) in[i] = double(rgb[i]) + (idx >= kAdQuantities && in_dir ? h * in_dir[i] : 0.0);
+    std::vector<double> neg(3 * np), e(3 * np);
+    develop<double, double>(*p, in.data(), neg.data(), &th, counts);
+    print<double>(*p, neg.data(), nullptr, nullptr, &th, e.data());
+    if (enc) std::memcpy(enc, e.data(), 3 * np * sizeof(double));
+    return l2_loss(e, target, np);
+}
+
+// Forward-mode derivatives of that loss: grad[q] for the 40 quantities, grad[40 + j] along
+// in_dirs[j] (n_in <= 8 directions of 3 np values each, interleaved like rgb).  Returns the loss.
+double oracle_frozen_grad(const spk_plan* p, const float* rgb, const float* counts, const float* target,
+                          const double* wt_dir, const double* in_dirs, int n_in, double* grad) {
+    const size_t np = size_t(p->width) * p->height;
+    auto th = std::make_unique<Theta<AD>>(theta_of<AD>(*p));
+    for (int q = 0; q < kAdQuantities; ++q) {
+        AD unit(0.0);
+        unit.d[q] = 1.0;
+        perturb(*th, q, unit, wt_dir);
+    }
+    std::vector<AD> in(3 * np), neg(3 * np), e(3 * np);
+    for (size_t i = 0; i < 3 * np; ++i) {
+        in[i] = AD(double(rgb[i]));
+        for (int j = 0; j < n_in && j < kAdInputDirs; ++j) in[i].d[kAdQuantities + j] = in_dirs[size_t(j) * 3 * np + i];
+    }
+    develop<AD, AD>(*p, in.data(), neg.data(), th.get(), counts);
+    print<AD>(*p, neg.data(), nullptr, nullptr, th.get(), e.data());
+    const AD loss = l2_loss(e, target, np);
+    for (int k = 0; k < kAdN; ++k) grad[k] = loss.d[k];
+    return loss.v;
+}
 
 // Tier downscale of an interleaved w x h x 3 image to the long edge L; returns the output size.
 void oracle_downscale(const float* src, uint32_t w, uint32_t h, uint32_t L, float* dst, uint32_t* ow, uint32_t* oh) {
@@ -930,8 +1284,8 @@ void oracle_downscale(const float* src, uint32_t w, uint32_t h, uint32_t L, floa
     uint32_t W = w, Hh = h;
     if (L != 0 && M > L) {
         const double s = double(L) / double(M);
-        W = std::max<uint32_t>(1, uint32_t(std::floor(double(w) * s)));
-        Hh = std::max<uint32_t>(1, uint32_t(std::floor(double(h) * s)));
+        W = std::max<uint32_t>(1, uint32_t(floor(double(w) * s)));
+        Hh = std::max<uint32_t>(1, uint32_t(floor(double(h) * s)));
     }
     *ow = W;
     *oh = Hh;
@@ -951,7 +1305,7 @@ void oracle_downscale(const float* src, uint32_t w, uint32_t h, uint32_t L, floa
                 for (int k = -r; k <= r; ++k) {
                     const int64_t yy = mirror_fold(y + k, h);
                     const float* q = &a[3 * (size_t(yy) * w + x)];
-                    for (int c = 0; c < 3; ++c) acc[c] = std::fma(ty[size_t(k + r)], q[c], acc[c]);
+                    for (int c = 0; c < 3; ++c) acc[c] = fma(ty[size_t(k + r)], q[c], acc[c]);
                 }
                 for (int c = 0; c < 3; ++c) b[3 * (size_t(y) * w + x) + c] = acc[c];
             }
@@ -966,7 +1320,7 @@ void oracle_downscale(const float* src, uint32_t w, uint32_t h, uint32_t L, floa
                 for (int k = -r; k <= r; ++k) {
                     const int64_t xx = mirror_fold(int64_t(x) + k, w);
                     const float* q = &a[3 * (size_t(y) * w + size_t(xx))];
-                    for (int c = 0; c < 3; ++c) acc[c] = std::fma(tx[size_t(k + r)], q[c], acc[c]);
+                    for (int c = 0; c < 3; ++c) acc[c] = fma(tx[size_t(k + r)], q[c], acc[c]);
                 }
                 for (int c = 0; c < 3; ++c) b[3 * (size_t(y) * w + x) + c] = acc[c];
             }
@@ -983,9 +1337,9 @@ void oracle_downscale(const float* src, uint32_t w, uint32_t h, uint32_t L, floa
             const float* pd = &a[3 * (size_t(yi[2 * y + 1]) * w + xi[2 * x])];
             const float* pe = &a[3 * (size_t(yi[2 * y + 1]) * w + xi[2 * x + 1])];
             for (int c = 0; c < 3; ++c) {
-                const float top = std::fma(xt[x], pb[c] - pa[c], pa[c]);
-                const float bot = std::fma(xt[x], pe[c] - pd[c], pd[c]);
-                dst[3 * (size_t(y) * W + x) + c] = std::fma(yt[y], bot - top, top);
+                const float top = fma(xt[x], pb[c] - pa[c], pa[c]);
+                const float bot = fma(xt[x], pe[c] - pd[c], pd[c]);
+                dst[3 * (size_t(y) * W + x) + c] = fma(yt[y], bot - top, top);
             }
         }
 }
@@ -1049,7 +1403,7 @@ long oracle_plan_offset(const char* name) {
     F(scatter_active) F(n_scatter) F(scatter) F(bounce_active) F(n_bounce) F(bounce) F(mix_a) F(mix_b)
     F(grain_active) F(grain_mode) F(grain_seed) F(grain_k) F(dmin_l) F(sat_d) F(count) F(uniformity) F(dmin_c)
     F(ae_active) F(ae_gain) F(ae_ev) F(couplers_active) F(n_dir) F(dir) F(glare_active) F(glare_seed) F(glare_sigma)
-    F(grain_final_blur) F(grain_amount) F(grain_axis) F(grain_inv) F(grain_layers) F(printed) F(paper_k) F(spectral_fp16)
+    F(grain_final_blur) F(grain_amount) F(grain_axis) F(grain_inv) F(grain_layers) F(printed) F(paper_k) F(spectral_fp16) F(film_gamma) F(film_offset) F(paper_gamma) F(paper_offset)
     F(enl_eps64) F(enl_beta64) F(enl_wt64) F(enl_g64)
     F(enl_o64) F(k_gain) F(k_gain64) F(scan_eps64) F(scan_beta64) F(scan_wt64) F(to_rgb64) F(cam64) F(m16_inv64)
     F(m_to_xyz64) F(m_to_rgb64) F(unsharp_sigma) F(unsharp_amount) F(cctf_active) F(cctf_mode) F(m_nid64)

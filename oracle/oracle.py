@@ -48,6 +48,12 @@ def lib():
         l.oracle_downscale.restype = None
         l.oracle_plan_offset.argtypes = [C.c_char_p]
         l.oracle_plan_offset.restype = C.c_long
+        l.oracle_grain_counts.argtypes = [V, F, F]
+        l.oracle_grain_counts.restype = None
+        l.oracle_frozen_f64.argtypes = [V, F, F, F, C.c_int, C.c_double, D, D, D]
+        l.oracle_frozen_f64.restype = C.c_double
+        l.oracle_frozen_grad.argtypes = [V, F, F, F, D, D, C.c_int, D]
+        l.oracle_frozen_grad.restype = C.c_double
         for n in ("oracle_log10f", "oracle_logf", "oracle_expf"):
             getattr(l, n).argtypes = [C.c_float]
             getattr(l, n).restype = C.c_float
@@ -140,6 +146,51 @@ def downscale(image, long_edge):
     out = np.zeros((oh.value, ow.value, 3), np.float32)
     lib().oracle_downscale(_p(img, C.c_float), w, h, long_edge, _p(out, C.c_float), C.byref(ow), C.byref(oh))
     return out
+
+
+# ---- the differentiable (frozen-count) render ---------------------------------------------------
+AD_QUANTITIES = 40   # derivative directions of the parameters (layout in oracle/chain.cpp)
+AD_INPUT_DIRS = 8    # further directions along the input pixels
+
+
+def grain_counts(plan, rgb):
+    """The Poisson counts the binary32 develop draws: float32 H x W x 9 ([3 c + l])."""
+    w, h = plan_size(plan)
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    out = np.zeros((h, w, 9), np.float32)
+    lib().oracle_grain_counts(C.cast(plan, C.c_void_p), _p(rgb, C.c_float), _p(out, C.c_float))
+    return out
+
+
+def frozen_f64(plan, rgb, counts, target, idx=-1, h=0.0, wt_dir=None, in_dir=None, want_enc=False):
+    """Loss (and optionally the encoded output, 3 x H x W) of the float64 frozen-count render with
+    quantity idx moved by h (idx >= 40: the pixels along in_dir)."""
+    w, hh = plan_size(plan)
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    tgt = np.ascontiguousarray(target, dtype=np.float32)
+    cnt = np.ascontiguousarray(counts, dtype=np.float32)
+    wd = np.ascontiguousarray(wt_dir if wt_dir is not None else np.zeros(243), dtype=np.float64)
+    idr = None if in_dir is None else np.ascontiguousarray(in_dir, dtype=np.float64)
+    enc = np.zeros(3 * w * hh) if want_enc else None
+    loss = lib().oracle_frozen_f64(C.cast(plan, C.c_void_p), _p(rgb, C.c_float), _p(cnt, C.c_float), _p(tgt, C.c_float),
+                                   idx, h, _p(wd, C.c_double), None if idr is None else _p(idr, C.c_double),
+                                   None if enc is None else _p(enc, C.c_double))
+    return (loss, enc.reshape(3, hh, w)) if want_enc else loss
+
+
+def frozen_grad(plan, rgb, counts, target, wt_dir, in_dirs):
+    """Forward-mode float64 derivatives of the frozen-count loss: (loss, grad[48])."""
+    w, hh = plan_size(plan)
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    tgt = np.ascontiguousarray(target, dtype=np.float32)
+    cnt = np.ascontiguousarray(counts, dtype=np.float32)
+    wd = np.ascontiguousarray(wt_dir, dtype=np.float64)
+    dirs = np.ascontiguousarray(in_dirs, dtype=np.float64).reshape(-1, 3 * w * hh)
+    assert dirs.shape[0] <= AD_INPUT_DIRS
+    grad = np.zeros(AD_QUANTITIES + AD_INPUT_DIRS)
+    loss = lib().oracle_frozen_grad(C.cast(plan, C.c_void_p), _p(rgb, C.c_float), _p(cnt, C.c_float), _p(tgt, C.c_float),
+                                    _p(wd, C.c_double), _p(dirs, C.c_double), dirs.shape[0], _p(grad, C.c_double))
+    return loss, grad
 
 
 def philox(ctr, key):

@@ -460,6 +460,10 @@ void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* ne
     std::memcpy(da.count, p.count, sizeof da.count);
     std::memcpy(da.uniformity, p.uniformity, sizeof da.uniformity);
     std::memcpy(da.dmin_c, p.dmin_c, sizeof da.dmin_c);
+    for (int c = 0; c < 3; ++c) {
+        da.fg[c] = num::f32(p.film_gamma[c]);
+        da.fo[c] = num::f32(p.film_offset[c]);
+    }
 
     if (p.couplers_active) {
         // DIR couplers (develop.cpp:182-222): stage 1 writes log exposure L and the coupler
@@ -571,6 +575,10 @@ void print_pre(const spk_plan& p, const Tables& t, const float* negative, gpu::P
     std::memcpy(ea.g, p.enl_g, sizeof ea.g);
     std::memcpy(ea.o, p.enl_o, sizeof ea.o);
     ea.k_gain = p.k_gain;
+    for (int c = 0; c < 3; ++c) {
+        ea.pg[c] = num::f32(p.paper_gamma[c]);
+        ea.po[c] = num::f32(p.paper_offset[c]);
+    }
     const k::PrintConsts* dq = gpu::as<const k::PrintConsts>(t.print_consts);
 
     gpu::DevBuf glare;

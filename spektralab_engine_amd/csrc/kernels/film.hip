@@ -331,12 +331,12 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
         float D;
         if (q.coupler_stage == 2u) {  // film.metal:282-292: F(L - chi; y0)
             const float l0 = H[o] - acc[o];
-            D = curve_lookup(x, inv, curves + 5 * K + c * K, K, l0);
+            D = curve_lookup(x, inv, curves + 5 * K + c * K, K, q.fg[c] * l0 + q.fo[c]);
         } else {
             float v = H[o];
             if (q.mix_active) v = q.mix_a[c] * v + q.mix_b[c] * acc[o];
             const float L = spkm::log10f_(fmaxf(v, 0.0f) + 1e-10f);
-            D = curve_lookup(x, inv, y + c * K, K, L);
+            D = curve_lookup(x, inv, y + c * K, K, q.fg[c] * L + q.fo[c]);
             if (q.coupler_stage == 1u) {  // film.metal:264-278
                 out[o] = L;
                 s1[c] = q.coupler_positive ? q.coupler_dmax[c] - D : D;

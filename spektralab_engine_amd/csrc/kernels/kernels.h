@@ -64,6 +64,7 @@ struct DevelopArgs {
     uint32_t coupler_stage;   // 0: no couplers
     uint32_t coupler_positive;
     float coupler_m[9], coupler_dmax[3];
+    float fg[3], fo[3];       // curve argument map fg L + fo (identity: 1, 0)
 };
 
 // Per-run constants of the enlarger (they follow the live print fields), passed by value.
@@ -72,6 +73,7 @@ struct EnlargerArgs {
     float eps[SPK_NL3], beta[SPK_NL], wt[SPK_NL3], g[3], o[3];
     float k_gain;
     float glare_tint[3];      // XYZ += (F / 100) tint with the glare field F when it is given
+    float pg[3], po[3];       // paper argument map pg l + po (identity: 1, 0)
 };
 
 // Pipeline constants of the print side, device-resident (uploaded with the pipeline's tables).

@@ -199,7 +199,8 @@ __device__ inline F3 expose_paper(float r0, float r1, float r2, CEA& e, const fl
     l0 = flog10(fmaxf(fexp2(l0 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
     l1 = flog10(fmaxf(fexp2(l1 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
     l2 = flog10(fmaxf(fexp2(l2 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
-    return {paper_lookup(paper, K, 0, l0), paper_lookup(paper, K, 1, l1), paper_lookup(paper, K, 2, l2)};
+    return {paper_lookup(paper, K, 0, e.pg[0] * l0 + e.po[0]), paper_lookup(paper, K, 1, e.pg[1] * l1 + e.po[1]),
+            paper_lookup(paper, K, 2, e.pg[2] * l2 + e.po[2])};
 }
 
 __device__ inline F3 scan_to_rgb(F3 xyz, CPC& q, const float* __restrict__ cmax);
