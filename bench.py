@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--sweep", type=int, default=32, help="reprints in the precision study")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-rendering measurement")
     ap.add_argument("--batch-frames", type=int, default=64)
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-render measurement")
+    ap.add_argument("--diff-frames", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the render chain runs on a single "
@@ -147,6 +149,10 @@ def main():
         sess.close()
         batched = batch_measure(eng, res, args)
 
+    differentiable = None
+    if not args.no_diff:
+        differentiable = diff_measure(eng, res, args)
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
@@ -192,6 +198,7 @@ def main():
                      "launch_ms": round(dom_ms, 4), "bytes_per_px": STAGE_BYTES[dom]},
         "cpu_baseline": cpu,
         "batched": batched,
+        "differentiable": differentiable,
     }
     print(json.dumps(line), flush=True)
     sess.close()
@@ -279,6 +286,84 @@ def batch_measure(eng, res, args):
             "speedup_vs_spk_render": round(t_render / t_batch, 3),
             "speedup_vs_render_device": round(t_device / t_batch, 3),
             "tuned_config": info["config"], "create_and_tune_s": round(t_create, 2)}
+
+
+def diff_measure(eng, res, args, W=2048, H=1365):
+    """The benchmark's configuration 5: `diff_frames` frames of 2048x1365 through the differentiable
+    render (grain frozen to each forward pass's counts, the chain workload's halation), an L2 loss
+    against targets rendered with the 40 scalar quantities and the enlarger weights perturbed by
+    N(0, 0.05) relative.  Times, inputs resident on the device: the differentiable forward pass of
+    every frame; forward + loss gradient + reverse pass of every frame; the plain render
+    (spkx_render_device) of the same frames.  Returns Mpx/s and the ratios."""
+    import torch
+    import spektralab_engine_amd as spk
+    from spektralab_engine_amd import synth
+
+    n = args.diff_frames
+    delta = dict(film_stock=synth.FILM_STOCK, print_stock=synth.PAPER_STOCK, auto_exposure=False,
+                 dir_couplers_active=False, glare_active=False, grain_active=True, halation_active=True,
+                 film_format_mm=35.0)
+    sess = eng.open(synth.make_frame(W, H, 100), delta)
+    sess.set_overrides({"halation_mixture": [[2.0, 0.6], [8.0, 0.3], [32.0, 0.1]], "halation_scatter": False,
+                        "fir_crossover": 16.5})
+    d = spk.Diff(sess, slots=n)
+    xs = [torch.from_numpy(synth.make_frame(W, H, 100 + i)).cuda() for i in range(n)]
+    rng = np.random.default_rng(0)
+    th0 = d.defaults()
+    th_t = th0 * (1.0 + 0.05 * rng.normal(size=th0.size))
+    th_t[39] = 0.0
+    targets = []
+    for i in range(n):
+        t = torch.empty_like(xs[i])
+        d.forward(i, xs[i].data_ptr(), t.data_ptr(), th_t)
+        targets.append(t)
+    encs = [torch.empty_like(x) for x in xs]
+    gxs = [torch.empty_like(x) for x in xs]
+    gth = torch.empty((n, d.QUANTITIES), dtype=torch.float64, device="cuda")
+    out = torch.empty((H, W, 4), dtype=torch.int16, device="cuda")
+    npx = 3 * W * H
+
+    def fwd():
+        for i in range(n):
+            d.forward(i, xs[i].data_ptr(), encs[i].data_ptr(), th0)
+
+    def fwd_bwd():
+        for i in range(n):
+            d.forward(i, xs[i].data_ptr(), encs[i].data_ptr(), th0)
+            g = (encs[i] - targets[i]) * (2.0 / npx)
+            d.backward(i, xs[i].data_ptr(), g.data_ptr(), gxs[i].data_ptr(), gth[i].data_ptr())
+
+    def plain():
+        for i in range(n):
+            sess.render_device(out.data_ptr(), W)
+        torch.cuda.synchronize()
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1000.0 / reps
+
+    t_f = timed(fwd, 3)
+    t_fb = timed(fwd_bwd, 3)
+    t_p = timed(plain, 3)
+    info = d.info()
+    px = n * W * H
+    log(f"[bench] differentiable x{n} {W}x{H}: forward {t_f:.1f} ms, forward+backward {t_fb:.1f} ms, "
+        f"plain render {t_p:.1f} ms")
+    d.close()
+    sess.close()
+    return {"workload": f"config 5: {n} frames {W}x{H}, grain frozen, 3-Gaussian halation, L2 loss vs targets at "
+                        "quantities perturbed N(0, 0.05) relative; gradients w.r.t. 39 scalars, 243 enlarger weights "
+                        "and the input pixels",
+            "frames": n, "pixels": px, "quantities": d.QUANTITIES,
+            "fwd_bwd_mpx_s": round(px / (t_fb / 1000.0) / 1e6, 1), "fwd_mpx_s": round(px / (t_f / 1000.0) / 1e6, 1),
+            "ms_forward": round(t_f, 3), "ms_forward_backward": round(t_fb, 3), "ms_plain_render": round(t_p, 3),
+            "fwd_bwd_over_fwd": round(t_fb / t_f, 3), "fwd_bwd_over_plain_render": round(t_fb / t_p, 3),
+            "checkpoint_bytes_per_frame": info["checkpoint_bytes_per_slot"]}
 
 
 def precision_study(args):

@@ -80,6 +80,39 @@ SPK_API spk_status spkx_batch_sync(spkx_batch* batch);
 SPK_API spk_status spkx_batch_info(spkx_batch* batch, char** out_json);
 SPK_API void spkx_batch_destroy(spkx_batch* batch);
 
+/* Differentiable render (no counterpart in the reference, which renders forward only).
+ * spkx_diff_create snapshots the session's full-tier parameters and draws one film and one print
+ * seed from its sequence; `slots` forward passes can be held at once (checkpoint memory per slot:
+ * spkx_diff_info's "checkpoint_bytes_per_slot").  The grain is frozen: each forward pass draws the
+ * Poisson counts as spk_render would and the reverse pass differentiates the grain sum with those
+ * counts fixed (the sample's dependence on the density through the sub-layer saturation stays).
+ * Refused (SPK_ERR_USER): scatter halation, DIR couplers, grain_amount != 1, FP16 spectral
+ * precision, more than 3 bounce components, the scanned-film path.
+ * Quantities (float64, SPKX_DIFF_QUANTITIES): 0 exposure-compensation gain; 1-3 bounce mix a_c;
+ * 4-6 mix b_c; 7-15 bounce weights [k][c]; 16-24 bounce sigmas [k][c] (px); 25-27 film curve
+ * gamma_c; 28-30 film offset_c; 31 print gain; 32-34 paper curve gamma_c; 35-37 paper offset_c;
+ * 38 unsharp amount; 39 reserved (0); 40 + 3 l + c enlarger weight wt_c(l), l < 81.
+ * spkx_diff_defaults writes the plan's values of them.
+ * spkx_diff_forward: input (device float32, interleaved h x w x 3) -> encoded output before 16-bit
+ * quantisation (device float32, interleaved h x w x 3), with the quantities theta (host, NULL: the
+ * defaults), into checkpoint `slot`.
+ * spkx_diff_backward: the gradient of a scalar with respect to that output (device, like the
+ * output) -> its gradient with respect to the input (device, like the input) and to the
+ * quantities (device float64 [SPKX_DIFF_QUANTITIES], overwritten); input is the slot's forward
+ * input.  Both run on `stream` (a hipStream_t; NULL = the object's own) and return when done.
+ * spkx_diff_plan copies the plan of the render (spk_plan) as spkx_last_plan does. */
+#define SPKX_DIFF_QUANTITIES 283
+typedef struct spkx_diff spkx_diff;
+SPK_API spkx_diff* spkx_diff_create(spk_session* session, int32_t slots, const char* options_json);
+SPK_API spk_status spkx_diff_defaults(spkx_diff* diff, double* theta);
+SPK_API spk_status spkx_diff_forward(spkx_diff* diff, int32_t slot, const double* theta, const float* input,
+                                     float* output, void* stream);
+SPK_API spk_status spkx_diff_backward(spkx_diff* diff, int32_t slot, const float* input, const float* grad_output,
+                                      float* grad_input, double* grad_theta, void* stream);
+SPK_API size_t spkx_diff_plan(spkx_diff* diff, void* out, size_t cap);
+SPK_API spk_status spkx_diff_info(spkx_diff* diff, char** out_json);
+SPK_API void spkx_diff_destroy(spkx_diff* diff);
+
 /* Self-tests on the engine's GPU:
  *   out[4i..4i+3] = Philox4x32-10(ctr[4i..4i+3], key[2i..2i+1])
  *   out[i] = Poisson(lam[i]) of pixel i on (stream, seed), the grain sampler itself

@@ -679,7 +679,9 @@ void develop(const spk_plan& p, const In* rgb, T* neg, const Theta<T>* thp = nul
     T a[9];
     for (int i = 0; i < 9; ++i) a[i] = sizeof(T) == 4 ? T(p.a[i]) : T(p.a64[i]);
     const T ec = th.ec;
-    const bool ec_active = p.ec_gain != 1.0f;
+    // the frozen-count (differentiable) render always applies the gain: a multiply by 1 is exact,
+    // and the gain then has its derivative
+    const bool ec_active = p.ec_gain != 1.0f || counts_in != nullptr;
     // auto exposure: the tier image scaled by f32(2^ev) before the upsample (develop.cpp:125-131)
     const T ae = sizeof(T) == 4 ? T(p.ae_gain) : T(exp2(p.ae_ev));
     const uint32_t L = p.L;

@@ -10,6 +10,7 @@
 #include <string>
 
 #include "../kernels/kernels.h"
+#include "adjoint.h"
 #include "batch.h"
 #include "engine.h"
 #include "tiers.h"
@@ -607,6 +608,75 @@ SPK_API void spkx_batch_destroy(spkx_batch* batch) {
     le::clear();
     (void)guarded([&] {
         delete reinterpret_cast<spk::Batch*>(batch);
+        return SPK_OK;
+    });
+}
+
+SPK_API spkx_diff* spkx_diff_create(spk_session* session, int32_t slots, const char* options_json) {
+    le::clear();
+    if (!session) {
+        invalid("session is NULL");
+        return nullptr;
+    }
+    spkx_diff* out = nullptr;
+    const int32_t st = guarded([&] {
+        std::unique_ptr<spk::Diff> d = spk::Diff::create(*S(session), slots, options_json);
+        out = reinterpret_cast<spkx_diff*>(d.release());
+        return SPK_OK;
+    });
+    return st == SPK_OK ? out : nullptr;
+}
+
+SPK_API spk_status spkx_diff_defaults(spkx_diff* diff, double* theta) {
+    le::clear();
+    if (!diff || !theta) return invalid("a required argument is NULL");
+    return guarded([&] {
+        reinterpret_cast<spk::Diff*>(diff)->defaults(theta);
+        return SPK_OK;
+    });
+}
+
+SPK_API spk_status spkx_diff_forward(spkx_diff* diff, int32_t slot, const double* theta, const float* input,
+                                     float* output, void* stream) {
+    le::clear();
+    if (!diff || !input || !output) return invalid("a required argument is NULL");
+    return guarded([&] {
+        reinterpret_cast<spk::Diff*>(diff)->forward(slot, theta, input, output, static_cast<hipStream_t>(stream));
+        return SPK_OK;
+    });
+}
+
+SPK_API spk_status spkx_diff_backward(spkx_diff* diff, int32_t slot, const float* input, const float* grad_output,
+                                      float* grad_input, double* grad_theta, void* stream) {
+    le::clear();
+    if (!diff || !input || !grad_output || !grad_input || !grad_theta) return invalid("a required argument is NULL");
+    return guarded([&] {
+        reinterpret_cast<spk::Diff*>(diff)->backward(slot, input, grad_output, grad_input, grad_theta,
+                                                      static_cast<hipStream_t>(stream));
+        return SPK_OK;
+    });
+}
+
+SPK_API size_t spkx_diff_plan(spkx_diff* diff, void* out, size_t cap) {
+    le::clear();
+    if (!diff) return 0;
+    if (out && cap >= sizeof(spk_plan)) std::memcpy(out, &reinterpret_cast<spk::Diff*>(diff)->plan(), sizeof(spk_plan));
+    return sizeof(spk_plan);
+}
+
+SPK_API spk_status spkx_diff_info(spkx_diff* diff, char** out_json) {
+    le::clear();
+    if (!diff || !out_json) return invalid("a required argument is NULL");
+    return guarded([&] {
+        put_json(out_json, reinterpret_cast<spk::Diff*>(diff)->info());
+        return SPK_OK;
+    });
+}
+
+SPK_API void spkx_diff_destroy(spkx_diff* diff) {
+    le::clear();
+    (void)guarded([&] {
+        delete reinterpret_cast<spk::Diff*>(diff);
         return SPK_OK;
     });
 }

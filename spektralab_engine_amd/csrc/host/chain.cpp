@@ -183,13 +183,15 @@ void fir_planes(const float* in, float* out, double sigma, uint32_t mode, const 
 
 // Recursive branch: a row pass into tmp, then the column pass (fused accumulation) into out;
 // columns taller than one workgroup's 32 chunks run as transpose, row pass, transpose.
+// adjoint: the transposed passes (the filter's transpose; rows and columns commute).
 void iir_planes(const float* in, float* out, double sigma, uint32_t mode, const float* acc, double weight, uint32_t w,
-                uint32_t h, uint32_t np, float* tmp, float* tmp2, hipStream_t s) {
+                uint32_t h, uint32_t np, float* tmp, float* tmp2, hipStream_t s, bool adjoint = false) {
     const size_t npx = size_t(w) * h;
     {
         const int m = k::iir_row_chunk(w);
         if (!m) fail_internal("recursive blur: rows longer than 16384 samples are not supported");
-        const k::IirArgs a = iir_args(sigma, w, h, w, uint32_t(m), (w + uint32_t(m) - 1) / uint32_t(m));
+        k::IirArgs a = iir_args(sigma, w, h, w, uint32_t(m), (w + uint32_t(m) - 1) / uint32_t(m));
+        a.adjoint = adjoint ? 1u : 0u;
         k::iir_row(in, tmp, a, h * np, s);
     }
     const int mc = k::iir_col_chunk(h);
@@ -197,12 +199,14 @@ void iir_planes(const float* in, float* out, double sigma, uint32_t mode, const 
         k::IirArgs a = iir_args(sigma, w, h, h, uint32_t(mc), (h + uint32_t(mc) - 1) / uint32_t(mc));
         a.acc_mode = mode;
         a.weight = num::f32(weight);
+        a.adjoint = adjoint ? 1u : 0u;
         k::iir_col(tmp, out, acc, a, np, s);
         return;
     }
     const int m = k::iir_row_chunk(h);
     if (!m) fail_internal("recursive blur: columns taller than 16384 samples are not supported");
-    const k::IirArgs a = iir_args(sigma, h, w, h, uint32_t(m), (h + uint32_t(m) - 1) / uint32_t(m));
+    k::IirArgs a = iir_args(sigma, h, w, h, uint32_t(m), (h + uint32_t(m) - 1) / uint32_t(m));
+    a.adjoint = adjoint ? 1u : 0u;
     for (uint32_t p = 0; p < np; ++p) {
         k::transpose(tmp + p * npx, tmp2, nullptr, w, h, 0, 1.0f, s);  // tmp2: h wide, w tall
         k::iir_row(tmp2, tmp2, a, w, s);
@@ -233,6 +237,56 @@ void blur1(const float* in, float* out, double sigma, double crossover, uint32_t
 }
 
 }  // namespace
+
+void blur_planes(const float* in, float* out, uint32_t nplanes, double sigma, double branch_sigma, double crossover,
+                 bool adjoint, uint32_t w, uint32_t h, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep) {
+    const size_t np = size_t(w) * h;
+    const int r = setup::fir_radius(branch_sigma);
+    if (!(branch_sigma > 0.0) || (branch_sigma < crossover && r == 0)) {
+        k::copy(in, out, nplanes * np, s);
+        return;
+    }
+    if (branch_sigma < crossover) {
+        // The FIR pass with half-sample reflected edges and symmetric taps is a symmetric operator
+        // (entry (i, j) sums w_t over t = j - i and t = -1 - i - j, mod 2n, and w_t = w_-t), so the
+        // adjoint runs the same passes.
+        if (r > k::kFirMaxR) fail_internal("FIR radius above the kernel maximum");
+        k::FirArgs a{};
+        a.w = w;
+        a.h = h;
+        a.r = uint32_t(r);
+        if (sigma == branch_sigma) {
+            const std::vector<float> taps = setup::fir_taps(sigma);
+            for (size_t i = 0; i < taps.size(); ++i) a.taps[i] = taps[i];
+        } else {  // the taps at sigma over the branch's radius
+            std::vector<double> d(size_t(2 * r + 1));
+            double sum = 0.0;
+            for (int t = -r; t <= r; ++t) {
+                const double u = double(t) / sigma;
+                d[size_t(t + r)] = std::exp(-0.5 * u * u);
+                sum += d[size_t(t + r)];
+            }
+            for (size_t i = 0; i < d.size(); ++i) a.taps[i] = num::f32(d[i] / sum);
+        }
+        if (r <= k::kFir2dMaxR && in != out) {
+            k::fir2d(in, out, nullptr, a, nplanes, s);
+            return;
+        }
+        gpu::DevBuf tmp = pool.acquire(nplanes * np * sizeof(float));
+        keep.push_back(tmp);
+        k::fir_v(in, gpu::as<float>(tmp), a, nplanes, s);
+        k::fir_h(gpu::as<float>(tmp), out, nullptr, a, nplanes, s);
+        return;
+    }
+    gpu::DevBuf tmp = pool.acquire(nplanes * np * sizeof(float)), tmp2;
+    keep.push_back(tmp);
+    if (!k::iir_col_chunk(h)) {
+        tmp2 = pool.acquire(np * sizeof(float));
+        keep.push_back(tmp2);
+    }
+    iir_planes(in, out, sigma, 0, nullptr, 1.0, w, h, nplanes, gpu::as<float>(tmp),
+               tmp2 ? gpu::as<float>(tmp2) : nullptr, s, adjoint);
+}
 
 // Per-channel branch (reference src/fx/gaussian.cpp:19-22): identity for sigma <= 0 or a zero FIR
 // radius, FIR below the crossover, recursive otherwise.  The three planes run as one launch per

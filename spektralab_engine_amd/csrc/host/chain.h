@@ -86,4 +86,11 @@ void print_post(const spk_plan& p, const Tables& t, const PrintStage& st, uint16
 void blur3(const float* in, float* out, const double sigma[3], uint32_t mode, const float* acc, const double weight[3],
            uint32_t w, uint32_t h, double crossover, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep);
 
+// The Gaussian of nplanes contiguous w x h planes with the branch (identity, FIR of its radius, or
+// recursive) that branch_sigma takes, evaluated at sigma (the FIR taps at sigma over branch_sigma's
+// radius; the recursion's coefficients at sigma); adjoint: the transposed filter.  In place only
+// for the recursive branch and radii above kFir2dMaxR.
+void blur_planes(const float* in, float* out, uint32_t nplanes, double sigma, double branch_sigma, double crossover,
+                 bool adjoint, uint32_t w, uint32_t h, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep);
+
 }  // namespace spk::chain

@@ -159,6 +159,8 @@ public:
     void batch_frame(BatchFrame* f, hipStream_t s);
     // The seeds of one fresh render from the session's sequence, in the reference's order.
     void draw_seeds(uint32_t* film, uint32_t* print);
+    // Fills the table section of a plan made by batch_frame (it is only filled for the first upload).
+    void fill_tables(spk_plan* p);
 
 private:
     Session() = default;

@@ -243,7 +243,11 @@ __device__ inline void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, 
 // backward start is y[n..n+2] = u[n-1]: the last chunk's padding is set to u[n-1] before the
 // backward pass, which the unit-DC-gain recursion carries unchanged to sample n-1.  Called by
 // every thread of the workgroup (it synchronises).
-template <int MC>
+// With kAdj the pass is the transpose of that line filter: both directions start from zero,
+// the forward start's weight on x0 moves to the last output of the forward direction
+// (b[n-1] += (S1 a[n-1] + S2 a[n-2] + S3 a[n-3]) / B, S1 = B1 + B2 + B3, S2 = B2 + B3, S3 = B3) and
+// the backward start's to the first output (y[0] += (S1 c[0] + S2 c[1] + S3 c[2]) / B).
+template <int MC, bool kAdj>
 __device__ inline void yvv_chunks(float (&v)[MC], uint32_t j, uint32_t nch, uint32_t len, double x0,
                                   const IirArgs& q, double* V, uint32_t vs) {
     const bool has = j < nch;
@@ -270,7 +274,8 @@ __device__ inline void yvv_chunks(float (&v)[MC], uint32_t j, uint32_t nch, uint
     // chunk live across the passes.
 #pragma unroll
     for (int k = 0; k < MC; ++k) asm volatile("" : "+v"(v[k]));
-    const St e0{x0, x0, x0};
+    const double xs = kAdj ? 0.0 : x0;
+    const St e0{xs, xs, xs};
     if (j == 0) z = st_add(mat_apply(q.pw[0], e0), z);
     if (has) put(j, z);
     __syncthreads();
@@ -288,6 +293,7 @@ __device__ inline void yvv_chunks(float (&v)[MC], uint32_t j, uint32_t nch, uint
     St e = e0;
     if (j > 0 && has) e = get(j - 1);
     __syncthreads();
+    const St ein = e;
 #pragma unroll
     for (int k = 0; k < MC; ++k) {
         const double t = yvv_step(q, v[k], e);
@@ -297,7 +303,22 @@ __device__ inline void yvv_chunks(float (&v)[MC], uint32_t j, uint32_t nch, uint
 
     // ---- backward ----
     St eb{0.0, 0.0, 0.0};
-    if (j == last) {
+    if constexpr (kAdj) {
+        if (j == last) {
+            // a[n-1], a[n-2], a[n-3]: from the chunk, or from its entry state for len < 3
+            double a1 = 0.0, a2 = len == 1 ? ein.a : 0.0, a3 = len == 2 ? ein.a : (len == 1 ? ein.b : 0.0);
+#pragma unroll
+            for (int k = 0; k < MC; ++k) {
+                if (uint32_t(k) + 1 == len) a1 = v[k];
+                if (uint32_t(k) + 2 == len) a2 = v[k];
+                if (uint32_t(k) + 3 == len) a3 = v[k];
+            }
+            const double S1 = (q.B1 + q.B2) + q.B3, S2 = q.B2 + q.B3, S3 = q.B3;
+            const double end = a1 + ((S1 * a1 + S2 * a2) + S3 * a3) / q.B;
+#pragma unroll
+            for (int k = 0; k < MC; ++k) v[k] = uint32_t(k) >= len ? 0.0f : (uint32_t(k) + 1 == len ? float(end) : v[k]);
+        }
+    } else if (j == last) {
         float un = 0.0f;
 #pragma unroll
         for (int k = 0; k < MC; ++k) un = (uint32_t(k) + 1 == len) ? v[k] : un;
@@ -336,6 +357,10 @@ __device__ inline void yvv_chunks(float (&v)[MC], uint32_t j, uint32_t nch, uint
         eb = {t, eb.a, eb.b};
         v[k] = float(t);
     }
+    if (kAdj && j == 0) {
+        const double S1 = (q.B1 + q.B2) + q.B3, S2 = q.B2 + q.B3, S3 = q.B3;
+        v[0] = float(eb.a + ((S1 * eb.a + S2 * eb.b) + S3 * eb.c) / q.B);
+    }
 }
 
 constexpr uint32_t kRowThreads = 128;
@@ -347,7 +372,7 @@ __host__ inline size_t row_lds_floats(uint32_t n) { return size_t(n) + n / 32 + 
 // One line of n = q.n samples per workgroup (lines of all planes are contiguous: line b starts at
 // b n); 128 threads, chunks of MC (nch <= 128).  The line is staged through LDS (one pad word per
 // 32), each thread's chunk held in registers.
-template <int MC>
+template <int MC, bool kAdj>
 __global__ __launch_bounds__(128) void k_iir_row(const float* __restrict__ in, float* __restrict__ out, IirArgs q) {
     extern __shared__ float srow[];
     __shared__ double V[kRowThreads * 3];
@@ -378,7 +403,7 @@ __global__ __launch_bounds__(128) void k_iir_row(const float* __restrict__ in, f
 #pragma unroll
     for (int k = 0; k < MC; ++k) v[k] = srow[rpad(start + min(uint32_t(k), kmax))];
     const double x0 = double(srow[0]);
-    yvv_chunks<MC>(v, j, nch, len, x0, q, V, 3u);
+    yvv_chunks<MC, kAdj>(v, j, nch, len, x0, q, V, 3u);
 #pragma unroll
     for (int k = 0; k < MC; ++k)
         if (uint32_t(k) < len) srow[rpad(start + uint32_t(k))] = v[k];
@@ -405,7 +430,7 @@ constexpr uint32_t kColW = 16, kColChunks = 32;
 // the row to the chunk's last valid row (lanes past the frame edge read column w-1), so every load
 // stays inside the plane; stores use a wave-uniform row offset and only rows < n.  The mixture
 // accumulation (out = acc + w y, or 0 + w y) is fused into the store.
-template <int MC>
+template <int MC, bool kAdj>
 __global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, float* __restrict__ out,
                                                  const float* __restrict__ accp, IirArgs q, uint32_t nplanes) {
     __shared__ double V[kColChunks * kColW * 3];
@@ -437,7 +462,7 @@ __global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, f
 #pragma unroll
     for (int k = 0; k < MC; ++k) v[k] = buf_ld(rin, lbase + min(uint32_t(k), kmax) * rowb, 0u);
     const double x0 = double(buf_ld(rin, xc * 4u, 0u));
-    yvv_chunks<MC>(v, j, nch, len, x0, q, V + tx * 3, kColW * 3);
+    yvv_chunks<MC, kAdj>(v, j, nch, len, x0, q, V + tx * 3, kColW * 3);
     if (!colok || !has) return;
     const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc(out, (short)0, plane_bytes, 0x00020000);
     if (q.acc_mode == 2u) {
@@ -528,17 +553,23 @@ int iir_row_chunk(uint32_t n) {
     return 0;
 }
 
-void iir_row(const float* in, float* out, const IirArgs& a, uint32_t lines, hipStream_t s) {
+template <bool kAdj>
+void iir_row_launch(const float* in, float* out, const IirArgs& a, uint32_t lines, hipStream_t s) {
     const size_t lds = row_lds_floats(a.n) * sizeof(float);
     const dim3 grid(lines), block(kRowThreads);
     switch (a.m) {
-        case 16: hipLaunchKernelGGL(k_iir_row<16>, grid, block, lds, s, in, out, a); break;
-        case 32: hipLaunchKernelGGL(k_iir_row<32>, grid, block, lds, s, in, out, a); break;
-        case 48: hipLaunchKernelGGL(k_iir_row<48>, grid, block, lds, s, in, out, a); break;
-        case 64: hipLaunchKernelGGL(k_iir_row<64>, grid, block, lds, s, in, out, a); break;
-        case 96: hipLaunchKernelGGL(k_iir_row<96>, grid, block, lds, s, in, out, a); break;
-        default: hipLaunchKernelGGL(k_iir_row<128>, grid, block, lds, s, in, out, a); break;
+        case 16: hipLaunchKernelGGL((k_iir_row<16, kAdj>), grid, block, lds, s, in, out, a); break;
+        case 32: hipLaunchKernelGGL((k_iir_row<32, kAdj>), grid, block, lds, s, in, out, a); break;
+        case 48: hipLaunchKernelGGL((k_iir_row<48, kAdj>), grid, block, lds, s, in, out, a); break;
+        case 64: hipLaunchKernelGGL((k_iir_row<64, kAdj>), grid, block, lds, s, in, out, a); break;
+        case 96: hipLaunchKernelGGL((k_iir_row<96, kAdj>), grid, block, lds, s, in, out, a); break;
+        default: hipLaunchKernelGGL((k_iir_row<128, kAdj>), grid, block, lds, s, in, out, a); break;
     }
+}
+
+void iir_row(const float* in, float* out, const IirArgs& a, uint32_t lines, hipStream_t s) {
+    if (a.adjoint) iir_row_launch<true>(in, out, a, lines, s);
+    else iir_row_launch<false>(in, out, a, lines, s);
 }
 
 int iir_col_chunk(uint32_t h) {
@@ -547,14 +578,20 @@ int iir_col_chunk(uint32_t h) {
     return 0;
 }
 
-void iir_col(const float* in, float* out, const float* acc, const IirArgs& a, uint32_t nplanes, hipStream_t s) {
+template <bool kAdj>
+void iir_col_launch(const float* in, float* out, const float* acc, const IirArgs& a, uint32_t nplanes, hipStream_t s) {
     const dim3 grid(xcd_grid(((a.w + kColW - 1) / kColW) * nplanes)), block(kColW, kColChunks);
     switch (a.m) {
-        case 16: hipLaunchKernelGGL(k_iir_col<16>, grid, block, 0, s, in, out, acc, a, nplanes); break;
-        case 32: hipLaunchKernelGGL(k_iir_col<32>, grid, block, 0, s, in, out, acc, a, nplanes); break;
-        case 64: hipLaunchKernelGGL(k_iir_col<64>, grid, block, 0, s, in, out, acc, a, nplanes); break;
-        default: hipLaunchKernelGGL(k_iir_col<128>, grid, block, 0, s, in, out, acc, a, nplanes); break;
+        case 16: hipLaunchKernelGGL((k_iir_col<16, kAdj>), grid, block, 0, s, in, out, acc, a, nplanes); break;
+        case 32: hipLaunchKernelGGL((k_iir_col<32, kAdj>), grid, block, 0, s, in, out, acc, a, nplanes); break;
+        case 64: hipLaunchKernelGGL((k_iir_col<64, kAdj>), grid, block, 0, s, in, out, acc, a, nplanes); break;
+        default: hipLaunchKernelGGL((k_iir_col<128, kAdj>), grid, block, 0, s, in, out, acc, a, nplanes); break;
     }
+}
+
+void iir_col(const float* in, float* out, const float* acc, const IirArgs& a, uint32_t nplanes, hipStream_t s) {
+    if (a.adjoint) iir_col_launch<true>(in, out, acc, a, nplanes, s);
+    else iir_col_launch<false>(in, out, acc, a, nplanes, s);
 }
 
 void transpose(const float* in, float* out, const float* acc, uint32_t w, uint32_t h, uint32_t acc_mode, float weight,

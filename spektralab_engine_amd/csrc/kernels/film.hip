@@ -415,6 +415,11 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
     drain<false>(gs, n_small, lane, base, seed, hl2p, s_lgam);
     wave_sync();
     if (!valid) return;
+    if (q.counts) {
+        float* cp = q.counts + size_t(i) * 9;
+#pragma unroll
+        for (uint32_t jd = 0; jd < 9; ++jd) cp[jd] = gs.v[jd][lane];
+    }
 #pragma unroll
     for (uint32_t c = 0; c < 3; ++c) {
         float val[3];

@@ -36,6 +36,7 @@ struct IirArgs {
     uint32_t n, m, nch;  // line length, chunk length, chunk count
     uint32_t acc_mode;   // column pass only (rows: NONE)
     float weight;
+    uint32_t adjoint;    // 1: the transposed pass (zero starts, edge terms moved to the far ends)
     double B, B1, B2, B3;
     double pw[7][9];
     double last[9];
@@ -65,6 +66,7 @@ struct DevelopArgs {
     uint32_t coupler_positive;
     float coupler_m[9], coupler_dmax[3];
     float fg[3], fo[3];       // curve argument map fg L + fo (identity: 1, 0)
+    float* counts;            // grain: the Poisson counts drawn, 9 per pixel [9 i + 3 c + l] (or null)
 };
 
 // Per-run constants of the enlarger (they follow the live print fields), passed by value.
@@ -186,6 +188,49 @@ void unsharp_encode2d(const float* rgb, uint16_t* out, uint32_t stride, const Pr
                       uint32_t w, uint32_t h, hipStream_t s);
 void unsharp_encode(const float* rgb, const float* blurred, uint16_t* out, uint32_t stride, const PrintConsts* pc,
                     uint32_t w, uint32_t h, hipStream_t s);
+
+// ---- the adjoint of the differentiable render (adjoint.hip) -------------------------------------
+// Gradient slots of the plan quantities (float64, device): the quantity layout of the
+// differentiable render.
+inline constexpr uint32_t kAdjEcGain = 0;        // exposure-compensation gain
+inline constexpr uint32_t kAdjMixA = 1;          // bounce mix a_c (raw := a raw + b acc)
+inline constexpr uint32_t kAdjMixB = 4;          // bounce mix b_c
+inline constexpr uint32_t kAdjBounceW = 7;       // bounce component weights [k][c], k < 3
+inline constexpr uint32_t kAdjBounceS = 16;      // bounce component sigmas [k][c] (pixels)
+inline constexpr uint32_t kAdjFilmGamma = 25;    // film curve argument map: gamma_c, offset_c
+inline constexpr uint32_t kAdjFilmOffset = 28;
+inline constexpr uint32_t kAdjKGain = 31;        // print gain
+inline constexpr uint32_t kAdjPaperGamma = 32;   // paper curve argument map: gamma_c, offset_c
+inline constexpr uint32_t kAdjPaperOffset = 35;
+inline constexpr uint32_t kAdjUnsharp = 38;      // unsharp amount a: (1 + a) x - a G(x)
+inline constexpr uint32_t kAdjReserved = 39;     // (zero)
+inline constexpr uint32_t kAdjWt = 40;           // enlarger weights wt[3 l + c], 243 of them
+inline constexpr uint32_t kAdjQuantities = kAdjWt + SPK_NL3;
+
+// enc (interleaved) := transfer(m_nid (ua lin + ub blur)) (blur null: lin), pre-quantisation.
+void adj_encode_fwd(const float* lin, const float* blur, float* enc, const PrintConsts* pc, float ua, float ub,
+                    uint32_t n, hipStream_t s);
+// g_v (3 planes) from g_enc (interleaved); gq[kAdjUnsharp] += d/da.
+void adj_encode_bwd(const float* lin, const float* blur, const float* g_enc, float* g_v, const PrintConsts* pc, float ua,
+                    float ub, uint32_t n, double* gq, hipStream_t s);
+// g_neg, g_R (3 planes each) from g_lin; gq gains k_gain and the paper maps.
+void adj_print_bwd(const float* neg, const float* g_lin, const float* glare, float* g_neg, float* g_R,
+                   const PrintConsts* pc, const EnlargerArgs& ea, const float* paper, const float* cmax, uint32_t n,
+                   double* gq, hipStream_t s);
+// g_wt[3 l + c] += sum_px g_R_c T_l (g_wt: 243 doubles).
+void adj_wt(const float* neg, const float* g_R, const EnlargerArgs& ea, uint32_t n, double* g_wt, hipStream_t s);
+// g_H, g_acc (acc null: g_acc unused) from the gradient of the grain sum (or of the density when
+// grain is off); gq gains the mix and film maps.
+void adj_develop_bwd(const float* H, const float* acc, const float* counts, const float* g_G, float* g_H, float* g_acc,
+                     const float* curves, const float* gaxis, const float* ginv, const float* glayers,
+                     const DevelopArgs& a, uint32_t n, double* gq, hipStream_t s);
+// g_rgb (interleaved) from g_H; gq[kAdjEcGain] += d/d ec.
+void adj_upsample_bwd(const float* rgb, const float* g_H, float* g_rgb, const float* table, const UpsampleArgs& a,
+                      double* gq, hipStream_t s);
+// *dst += sum_i a_i b_i.
+void adj_dot(const float* a, const float* b, uint32_t n, double* dst, hipStream_t s);
+// dst[0..n) := 0 (a kernel, so that callers may capture it).
+void adj_zero(double* dst, uint32_t n, hipStream_t s);
 
 // Tier downscale passes on interleaved 3-channel images (session.hip).
 void ds_blur_v(const float* src, float* dst, const float* taps, uint32_t w, uint32_t h, int r, hipStream_t s);

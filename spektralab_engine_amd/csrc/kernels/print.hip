@@ -17,161 +17,11 @@
 #include "fir_tile.h"
 #include "kernels.h"
 #include "philox.h"
+#include "print_math.h"
 #include "xcd.h"
 
 namespace spk::k {
 namespace {
-
-constexpr float kLog2Of10 = 3.321928094887362f;
-constexpr float kTiny = 1e-10f;
-
-__device__ inline float sgn(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
-
-// Hardware transcendentals: v_log_f32 (log2), v_exp_f32 (exp2), v_sin_f32 / v_cos_f32 (argument
-// in revolutions), each within a few ulp over the ranges met here; the libm versions' special-case
-// and wide-range reduction code is most of the kernel otherwise.  fpow is for x >= 0 (0 -> 0).
-__device__ inline float flog2(float x) { return __builtin_amdgcn_logf(x); }
-__device__ inline float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
-__device__ inline float fpow(float x, float y) { return fexp2(y * flog2(x)); }
-__device__ inline float flog10(float x) { return flog2(x) * 0.30102999566398120f; }
-__device__ inline float fln(float x) { return flog2(x) * 0.69314718055994531f; }
-__device__ inline float fexp(float x) { return fexp2(x * 1.4426950408889634f); }
-__device__ inline float fsin(float x) { return __builtin_amdgcn_sinf(x * 0.15915494309189535f); }
-__device__ inline float fcos(float x) { return __builtin_amdgcn_cosf(x * 0.15915494309189535f); }
-
-struct F3 {
-    float x, y, z;
-};
-
-// Per-frame constants are read through the constant address space, so that their uniform loads
-// become scalar loads however the constants reached the kernel (kernel argument or a batch's job
-// table in memory).
-#define SPK_CONST __attribute__((address_space(4)))
-using CPC = const PrintConsts SPK_CONST;
-using CEA = const EnlargerArgs SPK_CONST;
-template <class T>
-__device__ inline const T SPK_CONST& cref(const T& x) {
-    return *(const T SPK_CONST*)(&x);
-}
-
-template <class P>
-__device__ inline F3 mat_apply(P m, F3 v) {
-    return {(m[0] * v.x + m[1] * v.y) + m[2] * v.z, (m[3] * v.x + m[4] * v.y) + m[5] * v.z,
-            (m[6] * v.x + m[7] * v.y) + m[8] * v.z};
-}
-
-__device__ inline float knee(float d, float tau, float lim, float p) {
-    if (d <= tau) return d;
-    const float x = (d - tau) / (lim - tau);
-    return tau + (lim - tau) * (x / fpow(1.0f + fpow(x, p), 1.0f / p));
-}
-
-__device__ inline float post_adapt(float v, float f_l) {
-    const float f = fpow((f_l * fabsf(v)) / 100.0f, 0.42f);
-    return ((400.0f * sgn(v)) * f) / (27.13f + f) + 0.1f;
-}
-
-struct Cam16Derived {
-    float cz, inv_cz, f_l_qrt, chroma_k, hundred_over_f_l;
-};
-
-__device__ inline F3 cam16_forward(F3 xyz, CPC& q, const Cam16Derived& k) {
-    const F3 x100{100.0f * xyz.x, 100.0f * xyz.y, 100.0f * xyz.z};
-    const F3 rgb = mat_apply(q.m16, x100);
-    const float ra = post_adapt(q.d_rgb[0] * rgb.x, q.f_l);
-    const float ga = post_adapt(q.d_rgb[1] * rgb.y, q.f_l);
-    const float ba = post_adapt(q.d_rgb[2] * rgb.z, q.f_l);
-    const float a = (ra - (12.0f * ga) / 11.0f) + ba / 11.0f;
-    const float b = ((ra + ga) - 2.0f * ba) / 9.0f;
-    const float h = atan2f(b, a);
-    const float et = 0.25f * (fcos(h + 2.0f) + 3.8f);
-    const float A = (((2.0f * ra + ga) + ba / 20.0f) - 0.305f) * q.n_bb;
-    const float ratio = A / q.a_w;
-    const float J = (100.0f * sgn(ratio)) * fpow(fabsf(ratio), k.cz);
-    const float den = (ra + ga) + (21.0f * ba) / 20.0f;
-    const float t = den != 0.0f ? (((((50000.0f / 13.0f) * q.n_c) * q.n_cb) * et) * sqrtf(a * a + b * b)) / den : 0.0f;
-    const float C = t > 0.0f ? (fpow(t, 0.9f) * sqrtf(fabsf(J) / 100.0f)) * k.chroma_k : 0.0f;
-    const float M = C * k.f_l_qrt;
-    const float Mp = fln(1.0f + 0.0228f * M) * (1.0f / 0.0228f);
-    const float Jp = (1.7f * J) / (1.0f + 0.007f * J);
-    return {Jp, Mp, h};
-}
-
-__device__ inline F3 cam16_inverse(float Jp, float Mp, float h, CPC& q, const Cam16Derived& k) {
-    const float M = (fexp(0.0228f * Mp) - 1.0f) * (1.0f / 0.0228f);
-    const float J = Jp / (1.7f - 0.007f * Jp);
-    const float C = M / k.f_l_qrt;
-    const float A = (q.a_w * sgn(J)) * fpow(fabsf(J / 100.0f), k.inv_cz);
-    const float sq = sqrtf(fabsf(J) / 100.0f);
-    const float t = (sq > 0.0f && C > 0.0f) ? fpow(C / (sq * k.chroma_k), 1.0f / 0.9f) : 0.0f;
-    const float et = 0.25f * (fcos(h + 2.0f) + 3.8f);
-    const float p2 = A / q.n_bb + 0.305f;
-    const float p3 = 21.0f / 20.0f;
-    float a = 0.0f, b = 0.0f;
-    if (t != 0.0f) {
-        const float p1 = ((((50000.0f / 13.0f) * q.n_c) * q.n_cb) * et) / t;
-        const float sh = fsin(h), ch = fcos(h);
-        if (fabsf(sh) >= fabsf(ch)) {
-            const float p4 = p1 / sh;
-            b = ((p2 * (2.0f + p3)) * (460.0f / 1403.0f)) /
-                (((p4 + ((2.0f + p3) * (220.0f / 1403.0f)) * (ch / sh)) - 27.0f / 1403.0f) + p3 * (6300.0f / 1403.0f));
-            a = (b * ch) / sh;
-        } else {
-            const float p5 = p1 / ch;
-            a = ((p2 * (2.0f + p3)) * (460.0f / 1403.0f)) /
-                ((p5 + (2.0f + p3) * (220.0f / 1403.0f)) - (27.0f / 1403.0f - p3 * (6300.0f / 1403.0f)) * (sh / ch));
-            b = (a * sh) / ch;
-        }
-    }
-    const float xa[3] = {((460.0f * p2 + 451.0f * a) + 288.0f * b) / 1403.0f,
-                         ((460.0f * p2 - 891.0f * a) - 261.0f * b) / 1403.0f,
-                         ((460.0f * p2 - 220.0f * a) - 6300.0f * b) / 1403.0f};
-    float rc[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const float v = xa[i] - 0.1f;
-        const float av = fabsf(v);
-        const float beta = av >= 400.0f ? 0.0f : (27.13f * av) / (400.0f - av);
-        rc[i] = ((k.hundred_over_f_l * sgn(v)) * fpow(beta, 1.0f / 0.42f)) / q.d_rgb[i];
-    }
-    const F3 x100 = mat_apply(q.m16_inv, F3{rc[0], rc[1], rc[2]});
-    return {x100.x / 100.0f, x100.y / 100.0f, x100.z / 100.0f};
-}
-
-// Clamped paper-curve lookup with a right bisection; NaN passes (print.metal:201-232).
-__device__ inline float paper_lookup(const float* __restrict__ tab, uint32_t K, uint32_t c, float v) {
-    const float* x = tab;
-    const float* r = tab + K;
-    const float* y = tab + 2 * K;
-    if (isnan(v)) return v;
-    if (v <= x[0]) return y[c];
-    if (v >= x[K - 1]) return y[3 * (K - 1) + c];
-    uint32_t lo = 0, hi = K;
-    while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (x[mid] <= v) lo = mid + 1;
-        else hi = mid;
-    }
-    const uint32_t j = lo > 0 ? lo - 1 : 0;  // lo == 0 only on a non-monotone axis
-    const float y0 = y[3 * j + c];
-    return y0 + ((v - x[j]) * r[j]) * (y[3 * (j + 1) + c] - y0);
-}
-
-__device__ inline float cctf_encode(float v, uint32_t mode) {
-    switch (mode) {
-        case 0: return v <= 0.0031308f ? 12.92f * v : 1.055f * (sgn(v) * fpow(fabsf(v), 0.41666666666666669f)) - 0.055f;
-        case 1: return v < 0.001953125f ? 16.0f * v : fpow(v, 0.55555555555555558f);
-        case 2: return v >= 0.0f ? fpow(v, 0.45470692717584371f) : __builtin_nanf("");
-        case 3: return v < 0.018f ? 4.5f * v : 1.099f * (sgn(v) * fpow(fabsf(v), 0.45f)) - 0.099f;
-        default: return v;
-    }
-}
-
-__device__ inline uint16_t encode16(float v) {
-    if (isnan(v)) return 0;
-    const float c = fminf(fmaxf(v, 0.0f), 1.0f);
-    return uint16_t(floorf(c * 65535.0f + 0.5f));
-}
 
 // Print glare field (reference kernels/fx_random.h:100-115, fx_grain.metal:102-116 with same_all):
 // one lognormal value per pixel from words x, y of block 0 of (pixel, stream 200, seed),
@@ -203,7 +53,6 @@ __device__ inline F3 expose_paper(float r0, float r1, float r2, CEA& e, const fl
             paper_lookup(paper, K, 2, e.pg[2] * l2 + e.po[2])};
 }
 
-__device__ inline F3 scan_to_rgb(F3 xyz, CPC& q, const float* __restrict__ cmax);
 
 // Pixel i (< n) of one frame: negative densities -> enlarger -> paper -> scan -> output RGB.
 __device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __restrict__ neg, float* __restrict__ rgb,
@@ -350,47 +199,6 @@ __global__ __launch_bounds__(256) void k_print16(const float* __restrict__ neg, 
     rgb[i] = o.x;
     rgb[size_t(n) + i] = o.y;
     rgb[2 * size_t(n) + i] = o.z;
-}
-
-// Scanned XYZ -> output RGB (print.metal:249-258), then the CAM16-UCS gamut compression
-// (:267-317) when it is on.
-__device__ inline F3 scan_to_rgb(F3 xyz, CPC& q, const float* __restrict__ cmax) {
-    F3 o = mat_apply(q.to_rgb, xyz);
-
-    if (q.gamut_active) {
-        const F3 gx = mat_apply(q.m_to_xyz, o);
-        const Cam16Derived k{q.cz, q.inv_cz, q.f_l_qrt, q.chroma_k, q.hundred_over_f_l};
-        const F3 f = cam16_forward(gx, q, k);
-        float Jp = f.x;
-        const float Mp = f.y, hh = f.z;
-        if (q.lc_active) {
-            float Ln = Jp / q.j_white;
-            if (Ln > q.lc[0]) {
-                Ln = knee(Ln, q.lc[0], q.lc[1], q.lc[2]);
-                Jp = Ln * q.j_white;
-            }
-        }
-        const float Jc = fminf(fmaxf(Jp, q.j_first), q.j_last);
-        const float u = ((Jc - q.j_first) / (q.j_last - q.j_first)) * float(q.rows - 1);
-        const int l0 = min(max(int(floorf(u)), 0), int(q.rows) - 2);
-        const float fl = u - float(l0);
-        const float v = (hh - q.h_first) / q.h_step;
-        const float vf = floorf(v);
-        const int cols = int(q.cols);
-        int h0 = int(vf) % cols;
-        if (h0 < 0) h0 += cols;
-        const int h1 = h0 == cols - 1 ? 0 : h0 + 1;
-        const float fh = v - vf;
-        const float t00 = cmax[l0 * cols + h0], t01 = cmax[l0 * cols + h1];
-        const float t10 = cmax[(l0 + 1) * cols + h0], t11 = cmax[(l0 + 1) * cols + h1];
-        const float cm = (1.0f - fl) * ((1.0f - fh) * t00 + fh * t01) + fl * ((1.0f - fh) * t10 + fh * t11);
-        const float cs = fmaxf(cm, 1e-9f);
-        float dd = Mp / cs;
-        if (dd > 0.0f) dd = knee(dd, q.knee[0], q.knee[1], q.knee[2]);
-        const F3 back = cam16_inverse(Jp, dd * cs, hh, q, k);
-        o = mat_apply(q.m_to_rgb, back);
-    }
-    return o;
 }
 
 __global__ __launch_bounds__(256) void k_unsharp_encode(const float* __restrict__ rgb, const float* __restrict__ blurred,
