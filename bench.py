@@ -144,6 +144,23 @@ def main():
     log(f"[bench] {ms:.3f} ms/frame, {mpxs:.1f} Mpx/s; stages (ms): " +
         ", ".join(f"{k} {v:.3f}" for k, v in per_stage.items()))
 
+    # the reference ABI's own call: spk_render into a pinned host result (the device-to-host copy of
+    # the RGBA16 image over PCIe included); reported beside the device-resident value
+    import ctypes
+    lib = eng.lib
+    r = spk.Result()
+    lib.spk_render(sess.h, b"full", ctypes.byref(r))
+    lib.spk_result_free(ctypes.byref(r))
+    a0 = time.perf_counter()
+    for _ in range(3):
+        r = spk.Result()
+        st = lib.spk_render(sess.h, b"full", ctypes.byref(r))
+        if st != 0:
+            raise spk.SpkError(st, eng.error())
+        lib.spk_result_free(ctypes.byref(r))
+    abi_ms = (time.perf_counter() - a0) * 1000.0 / 3
+    log(f"[bench] spk_render (host result): {abi_ms:.3f} ms/frame")
+
     batched = None
     if not args.no_batch:
         sess.close()
@@ -196,6 +213,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "launch_ms": round(dom_ms, 4), "bytes_per_px": STAGE_BYTES[dom]},
+        "spk_render_host_result": {"ms_per_frame": round(abi_ms, 3), "mpx_s": round(px / (abi_ms / 1000.0) / 1e6, 1),
+                                   "note": "PCIe-inclusive: RGBA16 copied into pinned host memory; not the value"},
         "cpu_baseline": cpu,
         "batched": batched,
         "differentiable": differentiable,

@@ -58,6 +58,53 @@ std::unique_ptr<Diff> Diff::create(Session& s, int frames, const char* options) 
     d->unsharp_ = p.unsharp_sigma > 0.0 && p.unsharp_amount > 0.0;
     d->lens_ = p.scanner_lens_blur_px > 0.0;
     d->slots_.resize(size_t(frames));
+
+    // partial-sum regions and the reduction table
+    const uint32_t n = d->bf_.w * d->bf_.h;
+    const size_t nb = k::adj_blocks(n), wb = k::adj_wt_blocks(n);
+    Regions& r = d->regions_;
+    size_t off = 0;
+    auto take = [&](size_t count) {
+        const size_t o = off;
+        off += count;
+        return o;
+    };
+    r.enc = take(nb);
+    r.print = take(nb * 7);
+    r.wt = take(wb * SPK_NL3);
+    r.dev = take(nb * 12);
+    r.up = take(nb);
+    for (uint32_t kk = 0; kk < d->n_bounce_; ++kk) {
+        r.bw[kk] = take(nb * 3);
+        r.bs[kk] = take(nb * 3);
+    }
+    r.total = off;
+    std::vector<k::AdjTerm> terms(kQuantities);  // nblocks 0: zero
+    auto term = [&](uint32_t q, size_t offset, size_t rows, uint32_t stride) {
+        terms[q].offset = offset;
+        terms[q].nblocks = uint32_t(rows);
+        terms[q].stride = stride;
+    };
+    term(k::kAdjEcGain, r.up, nb, 1);
+    for (uint32_t c = 0; c < 3; ++c) {
+        term(k::kAdjMixA + c, r.dev + c, nb, 12);
+        term(k::kAdjMixB + c, r.dev + 3 + c, nb, 12);
+        term(k::kAdjFilmGamma + c, r.dev + 6 + c, nb, 12);
+        term(k::kAdjFilmOffset + c, r.dev + 9 + c, nb, 12);
+        term(k::kAdjPaperGamma + c, r.print + 1 + c, nb, 7);
+        term(k::kAdjPaperOffset + c, r.print + 4 + c, nb, 7);
+        for (uint32_t kk = 0; kk < d->n_bounce_; ++kk) {
+            term(k::kAdjBounceW + 3 * kk + c, r.bw[kk] + c, nb, 3);
+            term(k::kAdjBounceS + 3 * kk + c, r.bs[kk] + c, nb, 3);
+        }
+    }
+    term(k::kAdjKGain, r.print, nb, 7);
+    term(k::kAdjUnsharp, r.enc, nb, 1);
+    for (uint32_t i = 0; i < SPK_NL3; ++i) term(k::kAdjWt + i, r.wt + i, wb, SPK_NL3);
+    d->part_ = d->e_->pool().acquire(r.total * sizeof(double));
+    d->terms_ = d->e_->pool().acquire(terms.size() * sizeof(k::AdjTerm));
+    gpu::check(hipMemcpy(d->terms_->ptr, terms.data(), terms.size() * sizeof(k::AdjTerm), hipMemcpyHostToDevice),
+               "hipMemcpy(reduction table)");
     return d;
 }
 
@@ -258,15 +305,15 @@ void Diff::backward(int slot, const float* input, const float* g_enc, float* g_i
     float* neg = gpu::as<float>(sl.neg);
     float* lin = gpu::as<float>(sl.lin);
     float* ub = gpu::as<float>(sl.ublur);
-    double* gq = g_theta;
-    k::adj_zero(gq, kQuantities, s);
+    double* part = gpu::as<double>(part_);
+    const Regions& rg = regions_;
 
     // encode and unsharp combine
     const double a = th[k::kAdjUnsharp];
     const float ua = num::f32(1.0 + a), ubw = num::f32(-a);
     float* gA = buf(3 * np);  // the gradient plane, passed down the chain
     float* gB = buf(3 * np);  // a second plane
-    k::adj_encode_bwd(lin, ub, g_enc, gA, pc, ua, ubw, n, gq, s);
+    k::adj_encode_bwd(lin, ub, g_enc, gA, pc, ua, ubw, n, part + rg.enc, s);
     if (unsharp_) {
         chain::blur_planes(gA, gB, 3, p.unsharp_sigma, p.unsharp_sigma, p.fir_crossover, true, w, h, pool, s, keep);
         k::lincomb(gA, gB, gA, uint32_t(3 * np), ua, ubw, s);
@@ -280,8 +327,8 @@ void Diff::backward(int slot, const float* input, const float* g_enc, float* g_i
     const k::EnlargerArgs ea = enlarger_args(th);
     float* gR = buf(3 * np);
     k::adj_print_bwd(neg, gA, gpu::as<float>(sl.glare), gB, gR, pc, ea, gpu::as<float>(t.paper), gpu::as<float>(t.cmax),
-                     n, gq, s);
-    k::adj_wt(neg, gR, ea, n, gq + k::kAdjWt, s);
+                     n, part + rg.print, s);
+    k::adj_wt(neg, gR, ea, n, part + rg.wt, s);
     std::swap(gA, gB);  // gA: g_neg
     // grain final blur
     if (grain_ && p.grain_final_blur > 0.0) {
@@ -294,30 +341,42 @@ void Diff::backward(int slot, const float* input, const float* g_enc, float* g_i
     float* gAcc = n_bounce_ ? gR : nullptr;  // g_R is no longer needed
     const k::DevelopArgs da = develop_args(th);
     k::adj_develop_bwd(H, acc, gpu::as<float>(sl.counts), gA, gH, gAcc, gpu::as<float>(t.curves),
-                       gpu::as<float>(t.gaxis), gpu::as<float>(t.ginv), gpu::as<float>(t.glayers), da, n, gq, s);
-    // bounce mixture: acc_c = sum_k w_kc G_{s_kc}(H_c)
+                       gpu::as<float>(t.gaxis), gpu::as<float>(t.ginv), gpu::as<float>(t.glayers), da, n,
+                       part + rg.dev, s);
+    // bounce mixture: acc_c = sum_k w_kc G_{s_kc}(H_c); the three channels in one pass when they
+    // share sigma
     if (n_bounce_) {
         float* tp = gA;  // g_G is no longer needed
-        float* fp = buf(np);
-        float* fm = buf(np);
-        for (uint32_t kk = 0; kk < n_bounce_; ++kk)
+        float* fp = buf(3 * np);
+        float* fm = buf(3 * np);
+        for (uint32_t kk = 0; kk < n_bounce_; ++kk) {
+            double sg[3];
+            float wg[3], sc[3];
             for (int c = 0; c < 3; ++c) {
-                const double sg = th[k::kAdjBounceS + 3 * kk + c], wg = th[k::kAdjBounceW + 3 * kk + c];
-                const float* gac = gAcc + c * np;
-                const float* Hc = H + c * np;
-                chain::blur_planes(gac, tp, 1, sg, sg, p.fir_crossover, true, w, h, pool, s, keep);
-                k::adj_dot(tp, Hc, n, gq + k::kAdjBounceW + 3 * kk + c, s);
-                k::lincomb(gH + c * np, tp, gH + c * np, n, 1.0f, num::f32(wg), s);
-                const double eps = kSigmaStep * sg;
-                chain::blur_planes(Hc, fp, 1, sg + eps, sg, p.fir_crossover, false, w, h, pool, s, keep);
-                chain::blur_planes(Hc, fm, 1, sg - eps, sg, p.fir_crossover, false, w, h, pool, s, keep);
-                const float sc = num::f32(wg / (2.0 * eps));
-                k::lincomb(fp, fm, fp, n, sc, -sc, s);
-                k::adj_dot(gac, fp, n, gq + k::kAdjBounceS + 3 * kk + c, s);
+                sg[c] = th[k::kAdjBounceS + 3 * kk + c];
+                wg[c] = num::f32(th[k::kAdjBounceW + 3 * kk + c]);
+                sc[c] = num::f32(th[k::kAdjBounceW + 3 * kk + c] / (2.0 * kSigmaStep * sg[c]));
             }
+            auto blur = [&](const float* in, float* out, double step, bool adjoint) {
+                if (sg[0] == sg[1] && sg[1] == sg[2]) {
+                    chain::blur_planes(in, out, 3, sg[0] * (1.0 + step), sg[0], p.fir_crossover, adjoint, w, h, pool, s,
+                                       keep);
+                    return;
+                }
+                for (int c = 0; c < 3; ++c)
+                    chain::blur_planes(in + c * np, out + c * np, 1, sg[c] * (1.0 + step), sg[c], p.fir_crossover,
+                                       adjoint, w, h, pool, s, keep);
+            };
+            blur(gAcc, tp, 0.0, true);
+            k::adj_bounce(tp, H, gH, wg, n, part + rg.bw[kk], s);
+            blur(H, fp, kSigmaStep, false);
+            blur(H, fm, -kSigmaStep, false);
+            k::adj_dot_diff(gAcc, fp, fm, sc, n, part + rg.bs[kk], s);
+        }
     }
     // upsample
-    k::adj_upsample_bwd(input, gH, g_input, gpu::as<float>(t.table), upsample_args(th), gq, s);
+    k::adj_upsample_bwd(input, gH, g_input, gpu::as<float>(t.table), upsample_args(th), part + rg.up, s);
+    k::adj_reduce(gpu::as<const k::AdjTerm>(terms_), kQuantities, part, g_theta, s);
     gpu::check(hipStreamSynchronize(s), "hipStreamSynchronize(differentiable backward)");
 }
 

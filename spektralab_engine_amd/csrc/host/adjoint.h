@@ -70,7 +70,13 @@ private:
     std::vector<Slot> slots_;
     uint32_t n_bounce_ = 0;
     bool grain_ = false, unsharp_ = false, lens_ = false;
-    std::vector<gpu::DevBuf> scratch_;  // reverse-pass planes, kept between calls
+    // The reverse pass's partial sums: one region per producing kernel, and the table that reduces
+    // them into the quantities (built once: it depends on the frame size and the stages only).
+    struct Regions {
+        size_t enc = 0, print = 0, wt = 0, dev = 0, up = 0, bw[3] = {0, 0, 0}, bs[3] = {0, 0, 0}, total = 0;
+    };
+    Regions regions_;
+    gpu::DevBuf part_, terms_;
 };
 
 }  // namespace spk

@@ -23,12 +23,27 @@ namespace {
 
 constexpr float kLn10 = 2.302585092994046f;
 
-// Sum of v over the wavefront, added to *dst by lane 0.
-__device__ inline void wave_add(double* dst, float v) {
-    double x = v;
+// Per-block partial sums of the quantity gradients: each pointwise kernel reduces its NQ
+// per-thread values over the workgroup (wavefront shuffles in float32, then the waves in float64)
+// and writes them to part[block * NQ + j]; k_adj_reduce then sums each column over the blocks in
+// a fixed order.  No atomics: the gradients are deterministic run to run.
+template <int NQ>
+__device__ inline void block_sums(const float (&v)[NQ], double* __restrict__ part) {
+    __shared__ double red[NQ][kAdjBlock / 64];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    if ((threadIdx.x & 63u) == 0) atomicAdd(dst, x);
+    for (int j = 0; j < NQ; ++j) {
+        float x = v[j];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) red[j][wv] = double(x);
+    }
+    __syncthreads();
+    if (threadIdx.x < uint32_t(NQ)) {
+        double t = 0.0;
+        for (uint32_t w = 0; w < blockDim.x / 64; ++w) t += red[threadIdx.x][w];
+        part[size_t(blockIdx.x) * NQ + threadIdx.x] = t;
+    }
 }
 
 // ---- output stage: encoded value of the unsharp-masked RGB and its adjoint ---------------------
@@ -61,7 +76,7 @@ __global__ __launch_bounds__(256) void k_adj_encode_fwd(const float* __restrict_
 __global__ __launch_bounds__(256) void k_adj_encode_bwd(const float* __restrict__ lin, const float* __restrict__ blur,
                                                         const float* __restrict__ g_enc, float* __restrict__ g_v,
                                                         const PrintConsts* __restrict__ pcp, float ua, float ub,
-                                                        uint32_t n, double* __restrict__ gq) {
+                                                        uint32_t n, double* __restrict__ part) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float ga = 0.0f;
     if (i < n) {
@@ -86,7 +101,8 @@ __global__ __launch_bounds__(256) void k_adj_encode_bwd(const float* __restrict_
         g_v[2 * size_t(n) + i] = g.z;
         if (blur) ga = (g.x * (l.x - b.x) + g.y * (l.y - b.y)) + g.z * (l.z - b.z);
     }
-    wave_add(gq + kAdjUnsharp, ga);
+    const float q[1] = {ga};
+    block_sums<1>(q, part);
 }
 
 // ---- print: negative densities -> linear RGB, transposed ---------------------------------------
@@ -99,7 +115,7 @@ __global__ __launch_bounds__(256) void k_adj_print_bwd(const float* __restrict__
                                                        float* __restrict__ g_R, const PrintConsts* __restrict__ pcp,
                                                        EnlargerArgs ea, const float* __restrict__ paper,
                                                        const float* __restrict__ cmax, uint32_t n,
-                                                       double* __restrict__ gq) {
+                                                       double* __restrict__ part) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float g_kg = 0.0f, g_pg[3] = {0.0f, 0.0f, 0.0f}, g_po[3] = {0.0f, 0.0f, 0.0f};
     if (i < n) {
@@ -181,43 +197,55 @@ __global__ __launch_bounds__(256) void k_adj_print_bwd(const float* __restrict__
             g_R[size_t(c) * n + i] = gRr[c];
         }
     }
-    wave_add(gq + kAdjKGain, g_kg);
-    for (int c = 0; c < 3; ++c) {
-        wave_add(gq + kAdjPaperGamma + c, g_pg[c]);
-        wave_add(gq + kAdjPaperOffset + c, g_po[c]);
-    }
+    const float q[7] = {g_kg, g_pg[0], g_pg[1], g_pg[2], g_po[0], g_po[1], g_po[2]};
+    block_sums<7>(q, part);
 }
 
-// g_wt[3 l + c] += sum over pixels of g_R_c T_l.  Block b of 256 threads takes a run of pixels;
-// thread t < 81 handles bin t for all of them, the pixel values staged in LDS.
+// part[b * 243 + 3 l + c] := sum over workgroup b's pixels of g_R_c T_l (T_l the enlarger
+// transmittance of bin l).  Workgroups stride over tiles of kWtPx pixels staged in LDS; thread
+// t < 81 handles bin t.
 constexpr uint32_t kWtPx = 256;
 __global__ __launch_bounds__(128) void k_adj_wt(const float* __restrict__ neg, const float* __restrict__ g_R,
-                                                EnlargerArgs ea, uint32_t n, double* __restrict__ g_wt) {
+                                                EnlargerArgs ea, uint32_t n, double* __restrict__ part) {
     __shared__ float sd[3][kWtPx], sg[3][kWtPx];
-    const uint32_t p0 = blockIdx.x * kWtPx;
-    for (uint32_t t = threadIdx.x; t < kWtPx; t += blockDim.x) {
-        const uint32_t p = p0 + t;
-        for (int c = 0; c < 3; ++c) {
-            sd[c][t] = p < n ? neg[size_t(c) * n + p] : 0.0f;
-            sg[c][t] = p < n ? g_R[size_t(c) * n + p] : 0.0f;
+    const uint32_t l = threadIdx.x;
+    CEA& e = cref(ea);
+    const bool act = l < SPK_NL;
+    const uint32_t lb = act ? l : 0u;
+    const float e0 = e.eps[3 * lb], e1 = e.eps[3 * lb + 1], e2 = e.eps[3 * lb + 2], bb = e.beta[lb];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const uint32_t ntiles = (n + kWtPx - 1) / kWtPx;
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint32_t p0 = tile * kWtPx;
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < kWtPx; t += blockDim.x) {
+            const uint32_t p = p0 + t;
+            for (int c = 0; c < 3; ++c) {
+                sd[c][t] = p < n ? neg[size_t(c) * n + p] : 0.0f;
+                sg[c][t] = p < n ? g_R[size_t(c) * n + p] : 0.0f;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+            const uint32_t cnt = min(kWtPx, n - p0);
+            for (uint32_t t = 0; t < cnt; ++t) {
+                const float T = fexp2(((sd[0][t] * e0 + sd[1][t] * e1) + sd[2][t] * e2) + bb);
+                f0 += sg[0][t] * T;
+                f1 += sg[1][t] * T;
+                f2 += sg[2][t] * T;
+            }
+            a0 += f0;
+            a1 += f1;
+            a2 += f2;
         }
     }
-    __syncthreads();
-    const uint32_t l = threadIdx.x;
-    if (l >= SPK_NL) return;
-    CEA& e = cref(ea);
-    const float e0 = e.eps[3 * l], e1 = e.eps[3 * l + 1], e2 = e.eps[3 * l + 2], b = e.beta[l];
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-    const uint32_t cnt = min(kWtPx, n - p0);
-    for (uint32_t t = 0; t < cnt; ++t) {
-        const float T = fexp2(((sd[0][t] * e0 + sd[1][t] * e1) + sd[2][t] * e2) + b);
-        a0 += sg[0][t] * T;
-        a1 += sg[1][t] * T;
-        a2 += sg[2][t] * T;
+    if (act) {
+        double* dst = part + size_t(blockIdx.x) * SPK_NL3 + 3 * l;
+        dst[0] = a0;
+        dst[1] = a1;
+        dst[2] = a2;
     }
-    atomicAdd(g_wt + 3 * l, double(a0));
-    atomicAdd(g_wt + 3 * l + 1, double(a1));
-    atomicAdd(g_wt + 3 * l + 2, double(a2));
 }
 
 // ---- develop (frozen counts): exposure + halation accumulator -> grain sum, transposed ---------
@@ -243,7 +271,7 @@ __global__ __launch_bounds__(256) void k_adj_develop_bwd(const float* __restrict
                                                          const float* __restrict__ curves, const float* __restrict__ gaxis,
                                                          const float* __restrict__ ginv,
                                                          const float* __restrict__ glayers, DevelopArgs da, uint32_t n,
-                                                         double* __restrict__ gq) {
+                                                         double* __restrict__ part) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float gma[3] = {0.0f, 0.0f, 0.0f}, gmb[3] = {0.0f, 0.0f, 0.0f}, gfg[3] = {0.0f, 0.0f, 0.0f},
           gfo[3] = {0.0f, 0.0f, 0.0f};
@@ -312,12 +340,9 @@ __global__ __launch_bounds__(256) void k_adj_develop_bwd(const float* __restrict
             }
         }
     }
-    for (int c = 0; c < 3; ++c) {
-        wave_add(gq + kAdjMixA + c, gma[c]);
-        wave_add(gq + kAdjMixB + c, gmb[c]);
-        wave_add(gq + kAdjFilmGamma + c, gfg[c]);
-        wave_add(gq + kAdjFilmOffset + c, gfo[c]);
-    }
+    const float q[12] = {gma[0], gma[1], gma[2], gmb[0], gmb[1], gmb[2],
+                         gfg[0], gfg[1], gfg[2], gfo[0], gfo[1], gfo[2]};
+    block_sums<12>(q, part);
 }
 
 // ---- upsample: input RGB -> exposure H, transposed (forward-mode Jacobian per pixel) ----------
@@ -357,7 +382,7 @@ __device__ inline void mitchell_axis_d(D3 t, uint32_t L, int* idx, D3* w) {
 // The upsample of k_upsample (film.hip) on D3, the pixel's three channels seeded.
 __global__ __launch_bounds__(256) void k_adj_upsample_bwd(const float* __restrict__ rgb, const float* __restrict__ g_H,
                                                           float* __restrict__ g_rgb, const float* __restrict__ table,
-                                                          UpsampleArgs qa, double* __restrict__ gq) {
+                                                          UpsampleArgs qa, double* __restrict__ part) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float gec = 0.0f;
     if (i < qa.n) {
@@ -408,22 +433,73 @@ __global__ __launch_bounds__(256) void k_adj_upsample_bwd(const float* __restric
         g_rgb[3 * size_t(i) + 1] = (gh[0] * h[0].d1 + gh[1] * h[1].d1) + gh[2] * h[2].d1;
         g_rgb[3 * size_t(i) + 2] = (gh[0] * h[0].d2 + gh[1] * h[1].d2) + gh[2] * h[2].d2;
     }
-    wave_add(gq + kAdjEcGain, gec);
+    const float q[1] = {gec};
+    block_sums<1>(q, part);
 }
 
-__global__ __launch_bounds__(256) void k_adj_dot(const float* __restrict__ a, const float* __restrict__ b, uint32_t n,
-                                                 double* __restrict__ dst) {
-    float s = 0.0f;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += a[i] * b[i];
-    wave_add(dst, s);
-}
-
-__global__ __launch_bounds__(256) void k_adj_zero(double* __restrict__ dst, uint32_t n) {
+// ---- bounce mixture: acc_c = sum_k w_kc G_kc(H_c), one component k at a time ------------------
+// With t = G_k^T(g_acc): g_H_c += w_kc t_c and d/dw_kc = <t_c, H_c> (3 partial columns).
+__global__ __launch_bounds__(256) void k_adj_bounce(const float* __restrict__ t, const float* __restrict__ H,
+                                                    float* __restrict__ gH, float w0, float w1, float w2, uint32_t n,
+                                                    double* __restrict__ part) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = 0.0;
+    float q[3] = {0.0f, 0.0f, 0.0f};
+    if (i < n) {
+        const float w[3] = {w0, w1, w2};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const size_t o = size_t(c) * n + i;
+            const float tv = t[o];
+            gH[o] = gH[o] + w[c] * tv;
+            q[c] = tv * H[o];
+        }
+    }
+    block_sums<3>(q, part);
+}
+
+// d/dsigma_kc ~ scale_c <g_acc_c, G_{sigma + eps}(H_c) - G_{sigma - eps}(H_c)> (3 partial columns).
+__global__ __launch_bounds__(256) void k_adj_dot_diff(const float* __restrict__ g, const float* __restrict__ fp,
+                                                      const float* __restrict__ fm, float s0, float s1, float s2,
+                                                      uint32_t n, double* __restrict__ part) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float q[3] = {0.0f, 0.0f, 0.0f};
+    if (i < n) {
+        const float sc[3] = {s0, s1, s2};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const size_t o = size_t(c) * n + i;
+            q[c] = sc[c] * (g[o] * (fp[o] - fm[o]));
+        }
+    }
+    block_sums<3>(q, part);
+}
+
+// gq[q] := sum over b < terms[q].nblocks of part[terms[q].offset + b * terms[q].stride] (0 when
+// nblocks is 0); one workgroup per quantity, a fixed summation order.
+__global__ __launch_bounds__(256) void k_adj_reduce(const AdjTerm* __restrict__ terms, const double* __restrict__ part,
+                                                    double* __restrict__ gq) {
+    __shared__ double red[256];
+    const AdjTerm tm = terms[blockIdx.x];
+    double t = 0.0;
+    for (uint32_t b = threadIdx.x; b < tm.nblocks; b += blockDim.x) t += part[tm.offset + size_t(b) * tm.stride];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gq[blockIdx.x] = red[0];
 }
 
 }  // namespace
+
+namespace {
+
+uint32_t blocks_of(uint32_t n) { return (n + kAdjBlock - 1) / kAdjBlock; }
+
+}  // namespace
+
+uint32_t adj_blocks(uint32_t n) { return blocks_of(n); }
 
 void adj_encode_fwd(const float* lin, const float* blur, float* enc, const PrintConsts* pc, float ua, float ub,
                     uint32_t n, hipStream_t s) {
@@ -431,39 +507,49 @@ void adj_encode_fwd(const float* lin, const float* blur, float* enc, const Print
 }
 
 void adj_encode_bwd(const float* lin, const float* blur, const float* g_enc, float* g_v, const PrintConsts* pc, float ua,
-                    float ub, uint32_t n, double* gq, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_encode_bwd, dim3((n + 255) / 256), dim3(256), 0, s, lin, blur, g_enc, g_v, pc, ua, ub, n, gq);
+                    float ub, uint32_t n, double* part, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_encode_bwd, dim3(blocks_of(n)), dim3(kAdjBlock), 0, s, lin, blur, g_enc, g_v, pc, ua, ub,
+                       n, part);
 }
 
 void adj_print_bwd(const float* neg, const float* g_lin, const float* glare, float* g_neg, float* g_R,
                    const PrintConsts* pc, const EnlargerArgs& ea, const float* paper, const float* cmax, uint32_t n,
-                   double* gq, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_print_bwd, dim3((n + 255) / 256), dim3(256), 0, s, neg, g_lin, glare, g_neg, g_R, pc, ea,
-                       paper, cmax, n, gq);
+                   double* part, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_print_bwd, dim3(blocks_of(n)), dim3(kAdjBlock), 0, s, neg, g_lin, glare, g_neg, g_R, pc,
+                       ea, paper, cmax, n, part);
 }
 
-void adj_wt(const float* neg, const float* g_R, const EnlargerArgs& ea, uint32_t n, double* g_wt, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_wt, dim3((n + kWtPx - 1) / kWtPx), dim3(128), 0, s, neg, g_R, ea, n, g_wt);
+uint32_t adj_wt_blocks(uint32_t n) { return std::min<uint32_t>(kAdjWtBlocks, (n + kWtPx - 1) / kWtPx); }
+
+void adj_wt(const float* neg, const float* g_R, const EnlargerArgs& ea, uint32_t n, double* part, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_wt, dim3(adj_wt_blocks(n)), dim3(128), 0, s, neg, g_R, ea, n, part);
 }
 
 void adj_develop_bwd(const float* H, const float* acc, const float* counts, const float* g_G, float* g_H, float* g_acc,
                      const float* curves, const float* gaxis, const float* ginv, const float* glayers,
-                     const DevelopArgs& a, uint32_t n, double* gq, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_develop_bwd, dim3((n + 255) / 256), dim3(256), 0, s, H, acc, counts, g_G, g_H, g_acc, curves,
-                       gaxis, ginv, glayers, a, n, gq);
+                     const DevelopArgs& a, uint32_t n, double* part, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_develop_bwd, dim3(blocks_of(n)), dim3(kAdjBlock), 0, s, H, acc, counts, g_G, g_H, g_acc,
+                       curves, gaxis, ginv, glayers, a, n, part);
 }
 
 void adj_upsample_bwd(const float* rgb, const float* g_H, float* g_rgb, const float* table, const UpsampleArgs& a,
-                      double* gq, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_upsample_bwd, dim3((a.n + 255) / 256), dim3(256), 0, s, rgb, g_H, g_rgb, table, a, gq);
+                      double* part, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_upsample_bwd, dim3(blocks_of(a.n)), dim3(kAdjBlock), 0, s, rgb, g_H, g_rgb, table, a,
+                       part);
 }
 
-void adj_dot(const float* a, const float* b, uint32_t n, double* dst, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_dot, dim3(std::min<uint32_t>((n + 255) / 256, 2048u)), dim3(256), 0, s, a, b, n, dst);
+void adj_bounce(const float* t, const float* H, float* gH, const float w[3], uint32_t n, double* part, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_bounce, dim3(blocks_of(n)), dim3(kAdjBlock), 0, s, t, H, gH, w[0], w[1], w[2], n, part);
 }
 
-void adj_zero(double* dst, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_zero, dim3((n + 255) / 256), dim3(256), 0, s, dst, n);
+void adj_dot_diff(const float* g, const float* fp, const float* fm, const float scale[3], uint32_t n, double* part,
+                  hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_dot_diff, dim3(blocks_of(n)), dim3(kAdjBlock), 0, s, g, fp, fm, scale[0], scale[1],
+                       scale[2], n, part);
+}
+
+void adj_reduce(const AdjTerm* terms, uint32_t nq, const double* part, double* gq, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_reduce, dim3(nq), dim3(256), 0, s, terms, part, gq);
 }
 
 }  // namespace spk::k

@@ -207,30 +207,47 @@ inline constexpr uint32_t kAdjReserved = 39;     // (zero)
 inline constexpr uint32_t kAdjWt = 40;           // enlarger weights wt[3 l + c], 243 of them
 inline constexpr uint32_t kAdjQuantities = kAdjWt + SPK_NL3;
 
+// The reverse-pass kernels write per-workgroup partial sums of their quantities into `part`
+// (device float64): a kernel with NQ quantities over n pixels writes adj_blocks(n) rows of NQ
+// (part[b * NQ + j]); adj_wt writes adj_wt_blocks(n) rows of 243.  adj_reduce then sums, for every
+// quantity q, the column terms[q] names, in a fixed order: the gradients are deterministic.
+inline constexpr uint32_t kAdjBlock = 256;      // threads per workgroup of the pointwise kernels
+inline constexpr uint32_t kAdjWtBlocks = 1024;  // workgroups of the enlarger-weight reduction
+struct AdjTerm {
+    uint64_t offset;   // first partial of the column in `part`
+    uint32_t nblocks;  // rows (0: the quantity's gradient is 0)
+    uint32_t stride;   // doubles between rows
+};
+uint32_t adj_blocks(uint32_t n);
+uint32_t adj_wt_blocks(uint32_t n);
+
 // enc (interleaved) := transfer(m_nid (ua lin + ub blur)) (blur null: lin), pre-quantisation.
 void adj_encode_fwd(const float* lin, const float* blur, float* enc, const PrintConsts* pc, float ua, float ub,
                     uint32_t n, hipStream_t s);
-// g_v (3 planes) from g_enc (interleaved); gq[kAdjUnsharp] += d/da.
+// g_v (3 planes) from g_enc (interleaved).  Partials (1): d/d unsharp amount.
 void adj_encode_bwd(const float* lin, const float* blur, const float* g_enc, float* g_v, const PrintConsts* pc, float ua,
-                    float ub, uint32_t n, double* gq, hipStream_t s);
-// g_neg, g_R (3 planes each) from g_lin; gq gains k_gain and the paper maps.
+                    float ub, uint32_t n, double* part, hipStream_t s);
+// g_neg, g_R (3 planes each) from g_lin.  Partials (7): k_gain, paper gamma 0-2, paper offset 0-2.
 void adj_print_bwd(const float* neg, const float* g_lin, const float* glare, float* g_neg, float* g_R,
                    const PrintConsts* pc, const EnlargerArgs& ea, const float* paper, const float* cmax, uint32_t n,
-                   double* gq, hipStream_t s);
-// g_wt[3 l + c] += sum_px g_R_c T_l (g_wt: 243 doubles).
-void adj_wt(const float* neg, const float* g_R, const EnlargerArgs& ea, uint32_t n, double* g_wt, hipStream_t s);
+                   double* part, hipStream_t s);
+// Partials (243): sum_px g_R_c T_l at [3 l + c].
+void adj_wt(const float* neg, const float* g_R, const EnlargerArgs& ea, uint32_t n, double* part, hipStream_t s);
 // g_H, g_acc (acc null: g_acc unused) from the gradient of the grain sum (or of the density when
-// grain is off); gq gains the mix and film maps.
+// grain is off).  Partials (12): mix a 0-2, mix b 0-2, film gamma 0-2, film offset 0-2.
 void adj_develop_bwd(const float* H, const float* acc, const float* counts, const float* g_G, float* g_H, float* g_acc,
                      const float* curves, const float* gaxis, const float* ginv, const float* glayers,
-                     const DevelopArgs& a, uint32_t n, double* gq, hipStream_t s);
-// g_rgb (interleaved) from g_H; gq[kAdjEcGain] += d/d ec.
+                     const DevelopArgs& a, uint32_t n, double* part, hipStream_t s);
+// g_rgb (interleaved) from g_H.  Partials (1): d/d EC gain.
 void adj_upsample_bwd(const float* rgb, const float* g_H, float* g_rgb, const float* table, const UpsampleArgs& a,
-                      double* gq, hipStream_t s);
-// *dst += sum_i a_i b_i.
-void adj_dot(const float* a, const float* b, uint32_t n, double* dst, hipStream_t s);
-// dst[0..n) := 0 (a kernel, so that callers may capture it).
-void adj_zero(double* dst, uint32_t n, hipStream_t s);
+                      double* part, hipStream_t s);
+// One bounce component (3 planes each): g_H_c += w_c t_c.  Partials (3): <t_c, H_c>.
+void adj_bounce(const float* t, const float* H, float* gH, const float w[3], uint32_t n, double* part, hipStream_t s);
+// Partials (3): scale_c <g_c, fp_c - fm_c>.
+void adj_dot_diff(const float* g, const float* fp, const float* fm, const float scale[3], uint32_t n, double* part,
+                  hipStream_t s);
+// gq[q] := the reduction of terms[q] (device table of nq entries), q < nq.
+void adj_reduce(const AdjTerm* terms, uint32_t nq, const double* part, double* gq, hipStream_t s);
 
 // Tier downscale passes on interleaved 3-channel images (session.hip).
 void ds_blur_v(const float* src, float* dst, const float* taps, uint32_t w, uint32_t h, int r, hipStream_t s);
