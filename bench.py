@@ -67,9 +67,10 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 1 GiB device-to-device copy (a reference dispatch of known "
                          "traffic for hardware byte counters)")
-    ap.add_argument("--study", choices=["precision"], default=None,
+    ap.add_argument("--study", choices=["precision", "diff"], default=None,
                     help="precision: one developed negative, a reprint sweep of print_exposure 0.5..2.0 in FP32 and "
-                         "FP16, each against the float64 oracle (16-bit LSB and CAM16-UCS dE)")
+                         "FP16, each against the float64 oracle (16-bit LSB and CAM16-UCS dE); diff: only the "
+                         "differentiable-render measurement (configuration 5), one JSON line")
     ap.add_argument("--sweep", type=int, default=32, help="reprints in the precision study")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-rendering measurement")
     ap.add_argument("--batch-frames", type=int, default=64)
@@ -82,6 +83,13 @@ def main():
         sys.exit(2)
     if args.study == "precision":
         precision_study(args)
+        return
+    if args.study == "diff":
+        import spektralab_engine_amd as spk
+        from spektralab_engine_amd import synth
+        eng = spk.Engine(synth.default_resources_dir(192))
+        print(json.dumps(diff_measure(eng, None, args)), flush=True)
+        eng.close()
         return
 
     import torch

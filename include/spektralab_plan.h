@@ -24,7 +24,7 @@ extern "C" {
 #endif
 
 #define SPK_PLAN_MAGIC 0x504B5053u /* "SPKP" */
-#define SPK_PLAN_VERSION 4u
+#define SPK_PLAN_VERSION 5u
 
 #define SPK_NL 81        /* wavelength samples 380..780 nm, 5 nm */
 #define SPK_NL3 243      /* [lambda][channel] */
@@ -34,6 +34,7 @@ extern "C" {
 #define SPK_CMAX_ROWS 64
 #define SPK_CMAX_COLS 720
 #define SPK_CMAX_N (SPK_CMAX_ROWS * SPK_CMAX_COLS)
+#define SPK_PLAN_MAX_EDR 256  /* EDR tone-map nodes */
 
 /* Grain draw modes (kernels/fx_shared.h:70-72). */
 #define SPK_GRAIN_SUM 0u
@@ -63,6 +64,22 @@ typedef struct spk_plan {
     float ae_gain;         /* f32(2^ae_ev) */
     uint32_t pad0;
     double ae_ev;          /* the metered EV of the session's auto_exposure_method */
+
+    /* input transfer decode of the tier image (first stage; mode: 0 sRGB, 1 ProPhoto, 2 Adobe, 3 BT.709) */
+    uint32_t decode_active, decode_mode;
+    float bt709_bp, bt709_exp;
+    /* geometry: the tier image (src_w x src_h) resampled to width x height (crop, rotation, quarter
+     * turns, flips; bilinear at float64 source coordinates) */
+    uint32_t geometry_active, src_w, src_h, quarter_turns, flip_h, flip_v;
+    double crop[4];        /* x, y, w, h as fractions of the source */
+    double rotation_deg;
+    /* highlight boost (boost_ev != 0) and back-reflection headroom, both on the exposure after EC,
+     * each from the frame maximum at that point */
+    uint32_t boost_active, headroom_active;
+    double boost_ev;
+    float headroom_ev;
+    uint32_t pad3;
+    double lens_blur_px;   /* camera lens blur sigma (lens_blur_um / pitch_um), 0 = none */
 
     uint32_t scatter_active, n_scatter;
     spk_mix_component scatter[4];
@@ -131,6 +148,14 @@ typedef struct spk_plan {
     uint32_t cctf_active, cctf_mode;
     float m_nid[9];
     double m_nid64[9];
+
+    /* scanner levels (after the scan integral): Y' := clamp(m Y + q, 0, 1); XYZ *= Y' / (Y + 1e-10) */
+    /* (levels_active, levels_m, levels_q above) */
+    /* EDR tone map on luminance after the unsharp mask: y := max(r_y . rgb, 0); outside [toe, shoulder]
+     * y' := 2^(interpolated target_log2_y at (log2 y - lo) / span); rgb *= y' / (y + 1e-10) */
+    uint32_t edr_active, edr_n;
+    float edr_r_y[3], edr_domain_lo, edr_domain_span, edr_toe_y, edr_shoulder_y;
+    float edr_table[SPK_PLAN_MAX_EDR];
 
     /* ---- tables ---- */
     float table[SPK_PLAN_MAX_L * SPK_PLAN_MAX_L * 3];      /* L x L x 3 [i][j][c] */

@@ -25,6 +25,14 @@
 //   DIR couplers         src/film/develop.cpp:182-222, kernels/film.metal:264-292, src/fx/dir_blur.cpp
 //   glare                src/fx/glare.cpp:22-51, kernels/fx_random.h:100-115, kernels/fx_glare.metal
 //   tier downscale       src/session/tiers.cpp:48-132, :185-227, kernels/session.metal
+//   input decode         kernels/film.metal:104-146, src/film/geometry.cpp:160-169
+//   geometry             src/film/geometry.cpp:30-152 (plan; bilinear at float64 coordinates)
+//   highlight boost      src/film/develop.cpp:40-62, kernels/film.metal:229-238
+//   headroom             src/fx/halation.cpp:64-87, kernels/fx_halation.metal:13-39
+//   camera lens blur     src/fx/gaussian.cpp:307-312
+//   scanner levels       kernels/print.metal:234-246 (m, q from the plan)
+//   EDR tone map         kernels/print.metal:319-347
+//   print-preview table  kernels/print.metal:368-407, src/print/run.cpp:190-232
 // Arithmetic conventions (binary32 mode): every float operation rounds as written (compiled with
 // -ffp-contract=off); FIR taps accumulate with fused multiply-add in ascending order (the HIP
 // kernels do the same); film-side log10 and the sampler's log / exp use the shared reproducible
@@ -36,6 +44,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <type_traits>
 #include <vector>
@@ -416,6 +425,143 @@ Theta<T> theta_of(const spk_plan& p) {
     return t;
 }
 
+// ---- input decode and geometry (the tier image before the upsample) -------------------------------
+// Decode of one value: mode 0 sRGB, 1 ProPhoto, 2 Adobe, 3 BT.709.  The binary32 powers are the
+// shared reproducible powf_; float64 uses the C library's pow.
+template <class T>
+T powx(T x, T y) {
+    if constexpr (std::is_same_v<T, float>) return spkm::powf_(x, y);
+    else return pow(x, y);
+}
+
+template <class T>
+T decode_value(T v, uint32_t mode, T bp, T ex) {
+    auto sg = [](T u) { return u > T(0) ? T(1) : (u < T(0) ? T(-1) : T(0)); };
+    switch (mode) {
+        case 0:
+            if (v <= T(0.040449936f)) return v / T(12.92f);
+            {
+                const T u = (v + T(0.055f)) / T(1.055f);
+                return sg(u) * powx(fabs(u), T(2.4f));
+            }
+        case 1: return v < T(16.0f / 512.0f) ? v / T(16.0f) : powx(v, T(1.8f));
+        case 2: return powx(v, T(563.0f / 256.0f));
+        case 3:
+            if (v < bp) return v / T(4.5f);
+            {
+                const T u = (v + T(0.099f)) / T(1.099f);
+                return sg(u) * powx(fabs(u), ex);
+            }
+        default: return v;
+    }
+}
+
+// geometry.cpp:30-107: per output column / row, the source offsets (x, y), float64.
+struct Geo {
+    uint32_t sw = 0, sh = 0, dw = 0, dh = 0;
+    double cx = 0, cy = 0;
+    std::vector<double> colx, coly, rowx, rowy;
+};
+
+Geo geometry_of(const spk_plan& p) {
+    Geo g;
+    g.sw = p.src_w;
+    g.sh = p.src_h;
+    const double cx = p.crop[0], cy = p.crop[1], cw = p.crop[2], ch = p.crop[3];
+    const uint32_t q = p.quarter_turns & 3u;
+    const bool fh = p.flip_h != 0, fv = p.flip_v != 0;
+    const double CW = std::max(1.0, std::round(cw * double(g.sw)));
+    const double CH = std::max(1.0, std::round(ch * double(g.sh)));
+    const double OW = (q & 1u) ? CH : CW, OH = (q & 1u) ? CW : CH;
+    g.dw = uint32_t(OW);
+    g.dh = uint32_t(OH);
+    const double sx = cw * double(g.sw) / CW, sy = ch * double(g.sh) / CH;
+    const double hx = cw * double(g.sw) / 2.0, hy = ch * double(g.sh) / 2.0;
+    g.cx = (cx + cw / 2.0) * double(g.sw) - 0.5;
+    g.cy = (cy + ch / 2.0) * double(g.sh) - 0.5;
+    const double th = p.rotation_deg * 3.14159265358979323846 / 180.0;
+    const double ca = std::cos(th), sa = std::sin(th);
+    auto U = [&](double u, double& x, double& y) {
+        const double P = sx * u - hx;
+        x = P * ca;
+        y = P * sa;
+    };
+    auto V = [&](double v, double& x, double& y) {
+        const double Q = sy * v - hy;
+        x = -(Q * sa);
+        y = Q * ca;
+    };
+    g.colx.resize(g.dw);
+    g.coly.resize(g.dw);
+    g.rowx.resize(g.dh);
+    g.rowy.resize(g.dh);
+    for (uint32_t o = 0; o < g.dw; ++o) {
+        const double u0 = fh ? OW - (double(o) + 0.5) : double(o) + 0.5;
+        if (q == 0) U(u0, g.colx[o], g.coly[o]);
+        else if (q == 1) V(OW - u0, g.colx[o], g.coly[o]);
+        else if (q == 2) U(OW - u0, g.colx[o], g.coly[o]);
+        else V(u0, g.colx[o], g.coly[o]);
+    }
+    for (uint32_t o = 0; o < g.dh; ++o) {
+        const double v0 = fv ? OH - (double(o) + 0.5) : double(o) + 0.5;
+        if (q == 0) V(v0, g.rowx[o], g.rowy[o]);
+        else if (q == 1) U(v0, g.rowx[o], g.rowy[o]);
+        else if (q == 2) V(OH - v0, g.rowx[o], g.rowy[o]);
+        else U(OH - v0, g.rowx[o], g.rowy[o]);
+    }
+    return g;
+}
+
+// The tier image after decode and geometry (geometry.cpp:111-152: bilinear, clamped, float
+// weights), as the arithmetic type; the plan's width x height pixels.
+template <class T, class In>
+std::vector<T> preprocess(const spk_plan& p, const In* rgb) {
+    const uint32_t mode = p.decode_active ? p.decode_mode : 99u;
+    const T bp = std::is_same_v<T, float> ? T(p.bt709_bp) : T(1.099 * std::pow(0.018, 0.45) - 0.099);
+    const T ex = std::is_same_v<T, float> ? T(p.bt709_exp) : T(1.0 / 0.45);
+    auto dec = [&](const In& v) { return decode_value(T(v), mode, bp, ex); };
+    if (!p.geometry_active) {
+        const size_t n = size_t(p.width) * p.height * 3;
+        std::vector<T> out(n);
+        for (size_t i = 0; i < n; ++i) out[i] = dec(rgb[i]);
+        return out;
+    }
+    const Geo g = geometry_of(p);
+    std::vector<T> out(size_t(g.dw) * g.dh * 3);
+    auto axis = [](double f, uint32_t n, uint32_t& i0, uint32_t& i1, float& t) {
+        if (f <= 0.0) {
+            i0 = 0;
+            t = 0.0f;
+        } else if (f >= double(n - 1)) {
+            i0 = n - 1;
+            t = 0.0f;
+        } else {
+            const double fl = std::floor(f);
+            i0 = uint32_t(fl);
+            t = float(f - fl);
+            t = std::fmin(std::fmax(t, 0.0f), 1.0f);
+        }
+        i1 = std::min(i0 + 1, n - 1);
+    };
+    for (uint32_t oy = 0; oy < g.dh; ++oy)
+        for (uint32_t ox = 0; ox < g.dw; ++ox) {
+            const double fx = g.cx + (g.colx[ox] + g.rowx[oy]);
+            const double fy = g.cy + (g.coly[ox] + g.rowy[oy]);
+            uint32_t x0, x1, y0, y1;
+            float tx, ty;
+            axis(fx, g.sw, x0, x1, tx);
+            axis(fy, g.sh, y0, y1, ty);
+            for (int c = 0; c < 3; ++c) {
+                const T a = dec(rgb[3 * (size_t(y0) * g.sw + x0) + c]), b = dec(rgb[3 * (size_t(y0) * g.sw + x1) + c]);
+                const T d = dec(rgb[3 * (size_t(y1) * g.sw + x0) + c]), e = dec(rgb[3 * (size_t(y1) * g.sw + x1) + c]);
+                const T top = a * (T(1) - T(tx)) + b * T(tx);
+                const T bot = d * (T(1) - T(tx)) + e * T(tx);
+                out[3 * (size_t(oy) * g.dw + ox) + c] = top * (T(1) - T(ty)) + bot * T(ty);
+            }
+        }
+    return out;
+}
+
 // ---- film side -----------------------------------------------------------------------------------
 template <class T>
 T mitchell(T u) {
@@ -685,9 +831,16 @@ void develop(const spk_plan& p, const In* rgb, T* neg, const Theta<T>* thp = nul
     // auto exposure: the tier image scaled by f32(2^ev) before the upsample (develop.cpp:125-131)
     const T ae = sizeof(T) == 4 ? T(p.ae_gain) : T(exp2(p.ae_ev));
     const uint32_t L = p.L;
+    // input decode and geometry (develop.cpp:117-129); the stages off: the tier image itself
+    std::vector<T> pre;
+    if (p.decode_active || p.geometry_active) pre = preprocess<T, In>(p, rgb);
+    auto src = [&](size_t k) -> T {
+        if (!pre.empty()) return pre[k];
+        return T(rgb[k]);
+    };
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < int64_t(np); ++i) {
-        T r = T(rgb[3 * i]), g = T(rgb[3 * i + 1]), b = T(rgb[3 * i + 2]);
+        T r = src(3 * i), g = src(3 * i + 1), b = src(3 * i + 2);
         if (p.ae_active) {
             r = r * ae;
             g = g * ae;
@@ -730,6 +883,77 @@ void develop(const spk_plan& p, const In* rgb, T* neg, const Theta<T>* thp = nul
         H[i] = h0;
         H[np + i] = h1;
         H[2 * np + i] = h2;
+    }
+    // highlight boost (develop.cpp:40-62) and back-reflection headroom (halation.cpp:64-87), each
+    // from the frame maximum at its point (NaN ignored)
+    auto frame_max = [&]() {
+        float m = -std::numeric_limits<float>::infinity();
+        for (size_t k = 0; k < 3 * np; ++k)
+            if (float(H[k]) > m) m = float(H[k]);
+        return m;
+    };
+    constexpr bool f32 = std::is_same_v<T, float>;
+    if (p.boost_active) {
+        const float M = frame_max();
+        if (M == 0.0f) {
+            for (auto& v : H) v = v * T(0);
+        } else {
+            const double Md = double(M);
+            const double x0 = std::min(std::max(0.184 * std::pow(2.0, 4.0), 0.0), Md);
+            if (x0 != Md) {
+                const double a = std::pow(28.0, 1.0 - 0.3);
+                const double q = a * (1.0 - x0 / Md);
+                const double den = std::exp(q) - q - 1.0;
+                const double kk = (std::pow(2.0, p.boost_ev) - 1.0) / den;
+                const T tx0 = f32 ? T(float(x0)) : T(x0), tr = f32 ? T(float(1.0 / Md)) : T(1.0 / Md);
+                const T ta = f32 ? T(float(a)) : T(a), tk = f32 ? T(float(kk * Md)) : T(kk * Md);
+                for (auto& v : H) {
+                    if (!(v > tx0)) continue;
+                    const T z = ta * ((v - tx0) * tr);
+                    T e;
+                    if constexpr (f32) e = spkm::expf_(z);
+                    else e = exp(z);
+                    v = v + tk * ((e - z) - T(1));
+                }
+            }
+        }
+    }
+    if (p.headroom_active) {
+        const float clip = frame_max();
+        if (clip > 0.0f && std::isfinite(clip)) {
+            auto lg2 = [&](T x) -> T {
+                if constexpr (f32) return spkm::log2f_(x);
+                else return log(x) / T(0.69314718055994530942);
+            };
+            auto ex2 = [&](T x) -> T {
+                if constexpr (f32) return spkm::exp2f_(x);
+                else return exp2(x);
+            };
+            auto smooth01 = [](T t) {
+                const T u = fmin(fmax(t, T(0)), T(1));
+                return u * u * (T(3) - T(2) * u);
+            };
+            const T ev = T(p.headroom_ev);
+            const T lclip = lg2(T(clip)), lmid = lg2(T(0.184f));
+            for (size_t i = 0; i < np; ++i) {
+                T* v[3] = {&H[i], &H[np + i], &H[2 * np + i]};
+                const T m = fmax(fmax(*v[0], *v[1]), *v[2]);
+                T g = T(1);
+                if (m > T(0)) {
+                    const T lm = lg2(m);
+                    const T s1 = smooth01(((lm - lclip) - T(-1)) / T(0.75f));
+                    const T s2 = smooth01(((lm - lmid) - T(1.5f)) / T(1));
+                    g = ex2((ev * s1) * s2);
+                }
+                for (int c = 0; c < 3; ++c) *v[c] = *v[c] * g;
+            }
+        }
+    }
+    // camera lens blur (gaussian.cpp:307-312)
+    if (p.lens_blur_px > 0.0) {
+        const double sb[3] = {p.lens_blur_px, p.lens_blur_px, p.lens_blur_px}, one[3] = {1, 1, 1};
+        std::vector<T> t(H);
+        blur3<T>(t.data(), H.data(), sb, 0, nullptr, one, w, h, p.fir_crossover);
     }
     std::vector<T> acc;
     if (p.scatter_active || p.bounce_active) acc.assign(3 * np, T(0));
@@ -1112,6 +1336,14 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out, const The
             xyz[1] = fmax(r1, T(0)) + tiny;
             xyz[2] = fmax(r2, T(0)) + tiny;
         }
+        if (p.levels_active) {  // print.metal:234-246
+            const T Y = xyz[1];
+            const T Yp = fmin(fmax(T(p.levels_m) * Y + T(p.levels_q), T(0)), T(1));
+            const T sc = Yp / (Y + tiny);
+            xyz[0] = sc * xyz[0];
+            xyz[1] = sc * Y;
+            xyz[2] = sc * xyz[2];
+        }
         if (p.glare_active)
             for (int c = 0; c < 3; ++c) xyz[c] = xyz[c] + (glare[i] / T(100)) * T(p.glare_tint[c]);
         T c3[3];
@@ -1171,6 +1403,20 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out, const The
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < int64_t(np); ++i) {
         T v[3] = {rgb[i], rgb[np + i], rgb[2 * np + i]};
+        if (p.edr_active) {  // print.metal:319-347
+            const T y = fmax((T(p.edr_r_y[0]) * v[0] + T(p.edr_r_y[1]) * v[1]) + T(p.edr_r_y[2]) * v[2], T(0));
+            T yp = y;
+            if (y > T(0) && (y < T(p.edr_toe_y) || y > T(p.edr_shoulder_y))) {
+                const uint32_t N = std::max(p.edr_n, 2u);
+                const T u = fmin(fmax((log(y) / T(0.69314718055994530942) - T(p.edr_domain_lo)) / T(p.edr_domain_span), T(0)),
+                                 T(1)) * T(float(N - 1));
+                const uint32_t i0 = std::min(uint32_t(floor(u)), N - 2);
+                const T t = u - T(float(i0));
+                yp = exp2((T(1) - t) * T(p.edr_table[i0]) + t * T(p.edr_table[i0 + 1]));
+            }
+            const T den = y + tiny;
+            for (int c = 0; c < 3; ++c) v[c] = (v[c] * yp) / den;
+        }
         if (rgb_out)
             for (int c = 0; c < 3; ++c) rgb_out[c * np + i] = v[c];
         if (p.cctf_active) {
@@ -1278,6 +1524,38 @@ double oracle_frozen_grad(const spk_plan* p, const float* rgb, const float* coun
     const AD loss = l2_loss(e, target, np);
     for (int k = 0; k < kAdN; ++k) grad[k] = loss.d[k];
     return loss.v;
+}
+
+// The print-preview table (kernels/print.metal:368-407) on a planar w x h negative: per channel
+// u = clamp((d - lo) s, 0, 1) (S - 1), trilinear along k, then j, then i; 16-bit encode.
+void oracle_print_lut(const float* neg, uint32_t w, uint32_t h, const float* lut, uint32_t S, const float* lo,
+                      const float* s, uint16_t* out) {
+    const size_t n = size_t(w) * h;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < int64_t(n); ++ii) {
+        const size_t i = size_t(ii);
+        uint32_t i0[3], i1[3];
+        float f[3];
+        for (int c = 0; c < 3; ++c) {
+            const float u = std::fmin(std::fmax((neg[size_t(c) * n + i] - lo[c]) * s[c], 0.0f), 1.0f) * float(S - 1);
+            const float fl = std::floor(u);
+            i0[c] = uint32_t(fl);
+            f[c] = u - fl;
+            i1[c] = std::min(i0[c] + 1, S - 1);
+        }
+        for (int ch = 0; ch < 3; ++ch) {
+            float cc[2][2];
+            for (int di = 0; di < 2; ++di)
+                for (int dj = 0; dj < 2; ++dj) {
+                    const size_t base = (size_t(di ? i1[0] : i0[0]) * S + (dj ? i1[1] : i0[1])) * S;
+                    cc[di][dj] = lut[(base + i0[2]) * 3 + ch] * (1.0f - f[2]) + lut[(base + i1[2]) * 3 + ch] * f[2];
+                }
+            const float v = (cc[0][0] * (1.0f - f[1]) + cc[0][1] * f[1]) * (1.0f - f[0]) +
+                            (cc[1][0] * (1.0f - f[1]) + cc[1][1] * f[1]) * f[0];
+            out[4 * i + ch] = encode16(v);
+        }
+        out[4 * i + 3] = 65535;
+    }
 }
 
 // Tier downscale of an interleaved w x h x 3 image to the long edge L; returns the output size.
@@ -1409,6 +1687,8 @@ long oracle_plan_offset(const char* name) {
     F(enl_eps64) F(enl_beta64) F(enl_wt64) F(enl_g64)
     F(enl_o64) F(k_gain) F(k_gain64) F(scan_eps64) F(scan_beta64) F(scan_wt64) F(to_rgb64) F(cam64) F(m16_inv64)
     F(m_to_xyz64) F(m_to_rgb64) F(unsharp_sigma) F(unsharp_amount) F(cctf_active) F(cctf_mode) F(m_nid64)
+    F(decode_active) F(decode_mode) F(geometry_active) F(src_w) F(src_h) F(boost_active) F(headroom_active)
+    F(lens_blur_px) F(levels_active) F(levels_m) F(levels_q) F(edr_active) F(edr_n) F(edr_table) F(edr_r_y)
     F(table) F(table64) F(curves) F(curves_x64) F(curves_y64) F(paper) F(paper_x64) F(paper_y64) F(cmax) F(cmax64)
 #undef F
     return -1;

@@ -44,6 +44,8 @@ def lib():
         l.oracle_fir_radius.argtypes = [C.c_double]
         l.oracle_yvv.argtypes = [C.c_double, D]
         l.oracle_reflect.argtypes = [C.c_int, C.c_int]
+        l.oracle_print_lut.argtypes = [F, C.c_uint32, C.c_uint32, F, C.c_uint32, F, F, U16]
+        l.oracle_print_lut.restype = None
         l.oracle_downscale.argtypes = [F, C.c_uint32, C.c_uint32, C.c_uint32, F, U, U]
         l.oracle_downscale.restype = None
         l.oracle_plan_offset.argtypes = [C.c_char_p]
@@ -74,7 +76,10 @@ _DT = {"width": "u4", "height": "u4", "L": "u4", "K": "u4", "scatter_active": "u
        "a": "f4", "ec_gain": "f4", "dmin_l": "f4", "sat_d": "f4", "count": "f4", "uniformity": "f4",
        "dmin_c": "f4", "k_gain": "f4", "table": "f4", "curves": "f4", "paper": "f4", "cmax": "f4",
        "grain_axis": "f4", "grain_inv": "f4", "grain_layers": "f4", "ae_active": "u4", "ae_gain": "f4",
-       "couplers_active": "u4", "glare_active": "u4", "glare_seed": "u4", "n_dir": "u4", "spectral_fp16": "u4"}
+       "couplers_active": "u4", "glare_active": "u4", "glare_seed": "u4", "n_dir": "u4", "spectral_fp16": "u4",
+       "decode_active": "u4", "decode_mode": "u4", "geometry_active": "u4", "src_w": "u4", "src_h": "u4",
+       "boost_active": "u4", "headroom_active": "u4", "levels_active": "u4", "levels_m": "f4", "levels_q": "f4",
+       "edr_active": "u4", "edr_n": "u4", "edr_table": "f4", "edr_r_y": "f4"}
 
 
 def field(plan, name, count=1):
@@ -93,11 +98,19 @@ def plan_size(plan) -> tuple[int, int]:
     return int(head[2]), int(head[3])
 
 
+def source_size(plan) -> tuple[int, int]:
+    """(width, height) of the tier image a plan develops (the geometry's source when it is on)."""
+    if int(field(plan, "geometry_active")[0]):
+        return int(field(plan, "src_w")[0]), int(field(plan, "src_h")[0])
+    return plan_size(plan)
+
+
 def develop(plan, rgb, f64=False):
-    """rgb: H x W x 3 float32 (the session's source).  Returns the negative, 3 x H x W."""
+    """rgb: the tier image, float32 (source_size(plan)[1] x [0] x 3).  Returns the negative, 3 x H x W."""
     w, h = plan_size(plan)
     rgb = np.ascontiguousarray(rgb, dtype=np.float32)
-    assert rgb.shape == (h, w, 3)
+    sw, sh = source_size(plan)
+    assert rgb.shape == (sh, sw, 3)
     out = np.zeros(3 * w * h, dtype=np.float64 if f64 else np.float32)
     if f64:
         lib().oracle_develop_f64(C.cast(plan, C.c_void_p), _p(rgb, C.c_float), _p(out, C.c_double))
@@ -134,6 +147,20 @@ def gaussian(image, sigma3, crossover=3.0):
     out = np.zeros_like(img)
     s = (C.c_double * 3)(*[float(v) for v in sigma3])
     lib().oracle_gaussian_f32(_p(img, C.c_float), _p(out, C.c_float), w, h, s, crossover)
+    return out
+
+
+def print_lut(negative, table, axes):
+    """negative: 3 x H x W float32; table: (S, S, S, 3); axes: (3, S).  Returns H x W x 4 uint16."""
+    neg = np.ascontiguousarray(negative, dtype=np.float32)
+    _, h, w = neg.shape
+    t = np.ascontiguousarray(table, dtype=np.float32)
+    S = t.shape[0]
+    lo = np.ascontiguousarray(axes[:, 0], dtype=np.float32)
+    s = np.array([np.float32(1.0 / (float(axes[c, -1]) - float(axes[c, 0]))) for c in range(3)], np.float32)
+    out = np.zeros((h, w, 4), np.uint16)
+    lib().oracle_print_lut(_p(neg, C.c_float), w, h, _p(t, C.c_float), S, _p(lo, C.c_float), _p(s, C.c_float),
+                           _p(out, C.c_uint16))
     return out
 
 

@@ -310,8 +310,12 @@ SPK_API spk_status spk_print_lut_table(spk_engine* engine, const char* print_sto
                                        uint32_t* out_size) {
     le::clear();
     if (!engine || !print_stock || !out_table || !out_size) return invalid("a required argument is NULL");
-    le::set("no print-preview table for '" + std::string(print_stock) + "'");
-    return SPK_ERR_USER;
+    return guarded([&] {
+        const spk::res::PrintLutTable t = E(engine)->resources().lut_table(print_stock);
+        *out_table = t.table;  // inside the engine's blob mapping: valid for the engine's life
+        *out_size = t.size;
+        return SPK_OK;
+    });
 }
 
 SPK_API spk_status spk_preview_stock_lut(spk_session* session, const char* print_stock, const char* tier,
@@ -321,8 +325,12 @@ SPK_API spk_status spk_preview_stock_lut(spk_session* session, const char* print
     if (!out) return invalid("out is NULL");
     std::memset(out, 0, sizeof *out);
     if (!session) return invalid("session is NULL");
-    le::set(std::string("no print-preview table for '") + (print_stock && *print_stock ? print_stock : "(session)") + "'");
-    return SPK_ERR_USER;
+    return guarded([&] {
+        std::string reply;
+        S(session)->preview_stock_lut(print_stock, tier, out, out_json ? &reply : nullptr);
+        if (out_json) put_json(out_json, reply);
+        return SPK_OK;
+    });
 }
 
 SPK_API spk_status spk_output_transform(spk_engine* engine, const char* src_cs, const char* dst_cs,

@@ -53,6 +53,10 @@ std::unique_ptr<Diff> Diff::create(Session& s, int frames, const char* options) 
     if (p.grain_active && p.grain_amount != 1.0)
         fail_user("differentiable render: grain_amount must be 1 (the blend with the density is not differentiated)");
     if (p.bounce_active && p.n_bounce > 3) fail_user("differentiable render: at most 3 bounce components");
+    if (p.decode_active || p.geometry_active || p.boost_active || p.headroom_active || p.lens_blur_px > 0.0 ||
+        p.levels_active || p.edr_active)
+        fail_user("differentiable render: input decode, geometry, highlight boost and headroom, camera lens blur, "
+                  "scanner levels and EDR are not differentiated; set them off");
     d->n_bounce_ = p.bounce_active ? p.n_bounce : 0u;
     d->grain_ = p.grain_active != 0u;
     d->unsharp_ = p.unsharp_sigma > 0.0 && p.unsharp_amount > 0.0;

@@ -88,6 +88,11 @@ std::unique_ptr<Batch> Batch::create(Engine& e, Session** sessions, int n, void*
         Frame& f = b->frames_[size_t(i)];
         f.session = s;
         s->batch_frame(&f.bf, b->origin_->get());
+        const spk_plan& pl = *f.bf.plan;
+        if (pl.boost_active || pl.headroom_active)
+            fail_user("session " + std::to_string(i) +
+                      ": highlight boost and headroom read the frame maximum on the host; not in a batch");
+        if (pl.geometry_active) fail_user("session " + std::to_string(i) + ": geometry is not built for batches");
         if (strides[i] < f.bf.w) fail(SPK_ERR_INVALID_ARG, "stride " + std::to_string(i) + " is smaller than the width");
         f.out = static_cast<uint16_t*>(outs[i]);
         f.stride = strides[i];

@@ -447,6 +447,14 @@ Tables upload_tables(const spk_plan& p, gpu::Pool& pool) {
     q.cctf_active = p.cctf_active;
     q.cctf_mode = p.cctf_mode;
     std::memcpy(q.m_nid, p.m_nid, sizeof q.m_nid);
+    q.edr_active = p.edr_active;
+    q.edr_n = p.edr_n;
+    std::memcpy(q.edr_r_y, p.edr_r_y, sizeof q.edr_r_y);
+    q.edr_domain_lo = p.edr_domain_lo;
+    q.edr_domain_span = p.edr_domain_span;
+    q.edr_toe_y = p.edr_toe_y;
+    q.edr_shoulder_y = p.edr_shoulder_y;
+    std::memcpy(q.edr_table, p.edr_table, sizeof q.edr_table);
     q.unsharp_active = (p.unsharp_sigma > 0.0 && p.unsharp_amount > 0.0) ? 1u : 0u;
     q.unsharp_a = num::f32(1.0 + p.unsharp_amount);
     q.unsharp_b = num::f32(-p.unsharp_amount);
@@ -454,6 +462,62 @@ Tables upload_tables(const spk_plan& p, gpu::Pool& pool) {
     const k::SpecMfma sm = spec_mfma(p.scan_eps64, p.scan_beta64, p.scan_wt64);
     t.scan_mfma = upload(&sm, sizeof sm, pool);
     return t;
+}
+
+const float* preprocess(const float* src, uint32_t sw, uint32_t sh, const setup::GeometryPlan& g, uint32_t decode,
+                        float bt709_bp, float bt709_exp, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep,
+                        uint32_t* ow, uint32_t* oh) {
+    *ow = sw;
+    *oh = sh;
+    if (g.identity && !decode) return src;
+    if (g.identity) {
+        gpu::DevBuf b = pool.acquire(size_t(sw) * sh * 3 * sizeof(float));
+        keep.push_back(b);
+        k::decode(src, gpu::as<float>(b), uint32_t(size_t(sw) * sh * 3), decode - 1u, bt709_bp, bt709_exp, s);
+        return gpu::as<const float>(b);
+    }
+    // the per-column and per-row source offsets (x, y pairs), uploaded with a blocking copy
+    std::vector<double> cr(2 * (size_t(g.dst_w) + g.dst_h));
+    for (uint32_t i = 0; i < g.dst_w; ++i) {
+        cr[2 * i] = g.col_x[i];
+        cr[2 * i + 1] = g.col_y[i];
+    }
+    for (uint32_t i = 0; i < g.dst_h; ++i) {
+        cr[2 * (size_t(g.dst_w) + i)] = g.row_x[i];
+        cr[2 * (size_t(g.dst_w) + i) + 1] = g.row_y[i];
+    }
+    gpu::DevBuf crb = pool.acquire(cr.size() * sizeof(double));
+    gpu::DevBuf b = pool.acquire(size_t(g.dst_w) * g.dst_h * 3 * sizeof(float));
+    keep.push_back(crb);
+    keep.push_back(b);
+    gpu::check(hipMemcpy(crb->ptr, cr.data(), cr.size() * sizeof(double), hipMemcpyHostToDevice),
+               "hipMemcpy(geometry offsets)");
+    k::GeometryArgs ga{};
+    ga.src_w = sw;
+    ga.src_h = sh;
+    ga.dst_w = g.dst_w;
+    ga.dst_h = g.dst_h;
+    ga.cx = g.cx;
+    ga.cy = g.cy;
+    ga.decode = decode;
+    ga.bt709_bp = bt709_bp;
+    ga.bt709_exp = bt709_exp;
+    const double* col = gpu::as<const double>(crb);
+    k::geometry(src, gpu::as<float>(b), col, col + 2 * size_t(g.dst_w), ga, s);
+    *ow = g.dst_w;
+    *oh = g.dst_h;
+    return gpu::as<const float>(b);
+}
+
+float device_max(const float* p, size_t n, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep) {
+    gpu::DevBuf b = pool.acquire((k::kMaxBlocks + 1) * sizeof(float));
+    keep.push_back(b);
+    float* part = gpu::as<float>(b);
+    k::plane_max(p, n, part, part + k::kMaxBlocks, s);
+    float m = 0.0f;
+    gpu::check(hipMemcpyAsync(&m, part + k::kMaxBlocks, sizeof m, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(max)");
+    gpu::check(hipStreamSynchronize(s), "hipStreamSynchronize(max)");
+    return m;
 }
 
 void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
@@ -469,6 +533,18 @@ void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* ne
     float* H = gpu::as<float>(Hb);
 
     k::UpsampleArgs ua{};
+    const uint32_t decode = p.decode_active ? 1u + p.decode_mode : 0u;
+    if (p.geometry_active) {  // decode (on the taps) and geometry in one pass (develop.cpp:117-129)
+        uint32_t gw = 0, gh = 0;
+        rgb = preprocess(rgb, p.src_w, p.src_h, setup::plan_geometry(p), decode, p.bt709_bp, p.bt709_exp, pool, s,
+                         keep, &gw, &gh);
+        if (gw != w || gh != h) fail_internal("geometry: output size differs from the plan");
+        mark("geometry");
+    } else {
+        ua.decode = decode;  // fused into the upsample
+        ua.bt709_bp = p.bt709_bp;
+        ua.bt709_exp = p.bt709_exp;
+    }
     ua.n = uint32_t(np);
     ua.L = p.L;
     std::memcpy(ua.a, p.a, sizeof ua.a);
@@ -478,6 +554,38 @@ void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* ne
     ua.pre_active = p.ae_active;
     k::upsample(rgb, H, gpu::as<float>(t.table), ua, s);
     mark("upsample");
+
+    // highlight boost (develop.cpp:40-62): M := the frame maximum after EC
+    if (p.boost_active) {
+        const float M = device_max(H, 3 * np, pool, s, keep);
+        if (M == 0.0f) {
+            k::lincomb(H, nullptr, H, uint32_t(3 * np), 0.0f, 0.0f, s);
+        } else {
+            const double Md = double(M);
+            const double x0 = std::min(std::max(0.184 * std::pow(2.0, 4.0), 0.0), Md);
+            if (x0 != Md) {
+                const double a = std::pow(28.0, 1.0 - 0.3);
+                const double q = a * (1.0 - x0 / Md);
+                const double den = std::exp(q) - q - 1.0;
+                if (!(den > 0.0)) fail_gpu("highlight boost: the frame maximum leaves no range to lift (den <= 0)");
+                const double kk = (std::pow(2.0, p.boost_ev) - 1.0) / den;
+                k::boost(H, 3 * np, num::f32(x0), num::f32(1.0 / Md), num::f32(a), num::f32(kk * Md), s);
+            }
+        }
+        mark("boost");
+    }
+    // back-reflection headroom (halation.cpp:68-87): clip := the frame maximum now
+    if (p.headroom_active) {
+        const float clip = device_max(H, 3 * np, pool, s, keep);
+        if (clip > 0.0f && std::isfinite(clip)) k::headroom(H, uint32_t(np), clip, p.headroom_ev, s);
+        mark("headroom");
+    }
+    // camera lens blur (gaussian.cpp:307-312)
+    if (p.lens_blur_px > 0.0) {
+        const double sg[3] = {p.lens_blur_px, p.lens_blur_px, p.lens_blur_px}, one[3] = {1.0, 1.0, 1.0};
+        blur3(H, H, sg, 0, nullptr, one, w, h, p.fir_crossover, pool, s, keep);
+        mark("lens_blur");
+    }
 
     gpu::DevBuf accb;
     if (p.scatter_active || p.bounce_active) {
@@ -612,16 +720,8 @@ gpu::DevBuf upload_enlarger_mfma(const spk_plan& p, gpu::Pool& pool) {
     return upload(&em, sizeof em, pool);
 }
 
-void print_pre(const spk_plan& p, const Tables& t, const float* negative, gpu::Pool& pool, hipStream_t s,
-               std::vector<gpu::DevBuf>& keep, PrintStage* st, const uint32_t* glare_seed, Profiler* prof,
-               const k::SpecMfma* enl_mfma) {
-    auto mark = [&](const char* l) {
-        if (prof && prof->on) prof->mark(l, s);
-    };
-    mark("start");
-    const uint32_t w = p.width, h = p.height;
-    const size_t np = size_t(w) * h;
-    if (!p.printed) fail_internal("print: the scanned-film path is not built");
+// The per-run enlarger constants of a plan (print/run.cpp:118-150, the levels of :153-159).
+k::EnlargerArgs enlarger_args(const spk_plan& p) {
     k::EnlargerArgs ea{};
     for (int i = 0; i < SPK_NL3; ++i) ea.eps[i] = num::f32(-p.enl_eps64[i] * kLog2Of10);
     for (int i = 0; i < SPK_NL; ++i) ea.beta[i] = num::f32(-p.enl_beta64[i] * kLog2Of10);
@@ -633,6 +733,23 @@ void print_pre(const spk_plan& p, const Tables& t, const float* negative, gpu::P
         ea.pg[c] = num::f32(p.paper_gamma[c]);
         ea.po[c] = num::f32(p.paper_offset[c]);
     }
+    ea.levels_active = p.levels_active;
+    ea.levels_m = p.levels_m;
+    ea.levels_q = p.levels_q;
+    return ea;
+}
+
+void print_pre(const spk_plan& p, const Tables& t, const float* negative, gpu::Pool& pool, hipStream_t s,
+               std::vector<gpu::DevBuf>& keep, PrintStage* st, const uint32_t* glare_seed, Profiler* prof,
+               const k::SpecMfma* enl_mfma) {
+    auto mark = [&](const char* l) {
+        if (prof && prof->on) prof->mark(l, s);
+    };
+    mark("start");
+    const uint32_t w = p.width, h = p.height;
+    const size_t np = size_t(w) * h;
+    if (!p.printed) fail_internal("print: the scanned-film path is not built");
+    k::EnlargerArgs ea = enlarger_args(p);
     const k::PrintConsts* dq = gpu::as<const k::PrintConsts>(t.print_consts);
 
     gpu::DevBuf glare;

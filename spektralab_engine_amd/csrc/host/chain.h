@@ -17,6 +17,7 @@
 
 #include "../kernels/kernels.h"
 #include "gpu.h"
+#include "setup.h"
 #include "spektralab_plan.h"
 
 namespace spk::chain {
@@ -80,6 +81,13 @@ gpu::DevBuf upload_enlarger_mfma(const spk_plan& p, gpu::Pool& pool);
 void print_kernel(const PrintStage& st, hipStream_t s);
 void print_post(const spk_plan& p, const Tables& t, const PrintStage& st, uint16_t* out, uint32_t stride,
                 gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr);
+
+// Input decode (decode: 0 none, 1 + mode) and geometry of an interleaved 3-channel image: the
+// processed image (src itself when both are off) and its size.
+const float* preprocess(const float* src, uint32_t sw, uint32_t sh, const setup::GeometryPlan& g, uint32_t decode,
+                        float bt709_bp, float bt709_exp, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep,
+                        uint32_t* ow, uint32_t* oh);
+k::EnlargerArgs enlarger_args(const spk_plan& p);
 
 // One per-channel separable Gaussian on 3 planar planes (exposed for the blur tests):
 // out_c = fuse(mode, acc_c, weight_c, G_{sigma_c}(in_c)); mode 0 none, 1 zero, 2 plane.

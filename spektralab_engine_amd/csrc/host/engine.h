@@ -133,6 +133,9 @@ public:
     std::string get_params();
     // spk_render / spk_reprint into a fresh output image.
     void render(const char* tier, bool reprint, spk_result* out);
+    // spk_preview_stock_lut (session.cpp:503-575): the tier's negative (developed when not cached)
+    // through a print stock's preview table.
+    void preview_stock_lut(const char* print_stock, const char* tier, spk_result* out, std::string* out_json);
     // The same into a caller device buffer on a caller stream.
     void render_device(const char* tier, bool reprint, void* dev_out, uint32_t stride, hipStream_t stream, uint32_t* w,
                        uint32_t* h);
@@ -168,12 +171,16 @@ private:
                             std::shared_ptr<const res::Profile> print) const;
     void digest(params::ParamSet& p, const res::Profile& film) const;
     int tier_index(const char* tier) const;
+    // The output (and negative) size of tier t: its tier image size through the geometry.
+    tiers::Size out_size(int t) const;
     uint32_t tier_long_edge(int t) const;
     const TierImage& tier_image(int t, hipStream_t s);
     // auto_exposure: the EV of the configured method (meters once, session.cpp:296-319)
     std::optional<double> session_ev(hipStream_t s);
     // Develops (unless reusing) and prints into `out`; returns the plan's reprint flag.
-    bool run(int tier, bool reprint, uint16_t* out, uint32_t stride, hipStream_t s, std::vector<gpu::DevBuf>& keep);
+    // lut: the print-preview table to apply instead of the print (no print seed is drawn).
+    bool run(int tier, bool reprint, uint16_t* out, uint32_t stride, hipStream_t s, std::vector<gpu::DevBuf>& keep,
+             const res::PrintLutTable* lut = nullptr);
     void node(const char* label);
 
     Engine* engine_ = nullptr;

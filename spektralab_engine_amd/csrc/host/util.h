@@ -91,6 +91,9 @@ inline double nanmax_strided(const double* p, size_t n, size_t stride) {
     }
     return m;
 }
+// Rounds half away from zero (C round()).
+inline double round_half_away(double x) { return std::round(x); }
+
 inline double nanmean(const double* p, size_t n) {
     double s = 0.0;
     size_t c = 0;

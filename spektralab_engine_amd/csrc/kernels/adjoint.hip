@@ -203,22 +203,24 @@ __global__ __launch_bounds__(256) void k_adj_print_bwd(const float* __restrict__
 
 // part[b * 243 + 3 l + c] := sum over workgroup b's pixels of g_R_c T_l (T_l the enlarger
 // transmittance of bin l).  Workgroups stride over tiles of kWtPx pixels staged in LDS; thread
-// t < 81 handles bin t.
+// t < 243 takes bin t % 81 over every third pixel of the tile (phase t / 81), and the phases are
+// summed at the end.
 constexpr uint32_t kWtPx = 256;
-__global__ __launch_bounds__(128) void k_adj_wt(const float* __restrict__ neg, const float* __restrict__ g_R,
+__global__ __launch_bounds__(256) void k_adj_wt(const float* __restrict__ neg, const float* __restrict__ g_R,
                                                 EnlargerArgs ea, uint32_t n, double* __restrict__ part) {
     __shared__ float sd[3][kWtPx], sg[3][kWtPx];
-    const uint32_t l = threadIdx.x;
+    __shared__ double red[3][SPK_NL3];
+    const uint32_t tid = threadIdx.x;
     CEA& e = cref(ea);
-    const bool act = l < SPK_NL;
-    const uint32_t lb = act ? l : 0u;
-    const float e0 = e.eps[3 * lb], e1 = e.eps[3 * lb + 1], e2 = e.eps[3 * lb + 2], bb = e.beta[lb];
+    const bool act = tid < 3 * SPK_NL;
+    const uint32_t l = act ? tid % SPK_NL : 0u, ph = act ? tid / SPK_NL : 0u;
+    const float e0 = e.eps[3 * l], e1 = e.eps[3 * l + 1], e2 = e.eps[3 * l + 2], bb = e.beta[l];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     const uint32_t ntiles = (n + kWtPx - 1) / kWtPx;
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint32_t p0 = tile * kWtPx;
         __syncthreads();
-        for (uint32_t t = threadIdx.x; t < kWtPx; t += blockDim.x) {
+        for (uint32_t t = tid; t < kWtPx; t += blockDim.x) {
             const uint32_t p = p0 + t;
             for (int c = 0; c < 3; ++c) {
                 sd[c][t] = p < n ? neg[size_t(c) * n + p] : 0.0f;
@@ -229,7 +231,7 @@ __global__ __launch_bounds__(128) void k_adj_wt(const float* __restrict__ neg, c
         if (act) {
             float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
             const uint32_t cnt = min(kWtPx, n - p0);
-            for (uint32_t t = 0; t < cnt; ++t) {
+            for (uint32_t t = ph; t < cnt; t += 3) {
                 const float T = fexp2(((sd[0][t] * e0 + sd[1][t] * e1) + sd[2][t] * e2) + bb);
                 f0 += sg[0][t] * T;
                 f1 += sg[1][t] * T;
@@ -241,11 +243,13 @@ __global__ __launch_bounds__(128) void k_adj_wt(const float* __restrict__ neg, c
         }
     }
     if (act) {
-        double* dst = part + size_t(blockIdx.x) * SPK_NL3 + 3 * l;
-        dst[0] = a0;
-        dst[1] = a1;
-        dst[2] = a2;
+        red[ph][3 * l] = a0;
+        red[ph][3 * l + 1] = a1;
+        red[ph][3 * l + 2] = a2;
     }
+    __syncthreads();
+    for (uint32_t j = tid; j < SPK_NL3; j += blockDim.x)
+        part[size_t(blockIdx.x) * SPK_NL3 + j] = (red[0][j] + red[1][j]) + red[2][j];
 }
 
 // ---- develop (frozen counts): exposure + halation accumulator -> grain sum, transposed ---------
@@ -319,7 +323,7 @@ __global__ __launch_bounds__(256) void k_adj_develop_bwd(const float* __restrict
                     const float Dl = q.sat_d[lc], nl = q.count[lc];
                     const float qq = (dv[l] + q.dmin_l[lc]) / Dl;
                     if (qq > 1e-6f && qq < 0.999999f) {
-                        const float k = counts[size_t(i) * 9 + 3 * c + l];
+                        const float k = counts[size_t(3 * c + l) * n + i];
                         dGdD += k * (Dl / nl) * (-(q.uniformity[c] * 0.999999f) / Dl) * ds[l] * sg;
                     }
                 }
@@ -522,7 +526,7 @@ void adj_print_bwd(const float* neg, const float* g_lin, const float* glare, flo
 uint32_t adj_wt_blocks(uint32_t n) { return std::min<uint32_t>(kAdjWtBlocks, (n + kWtPx - 1) / kWtPx); }
 
 void adj_wt(const float* neg, const float* g_R, const EnlargerArgs& ea, uint32_t n, double* part, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_wt, dim3(adj_wt_blocks(n)), dim3(128), 0, s, neg, g_R, ea, n, part);
+    hipLaunchKernelGGL(k_adj_wt, dim3(adj_wt_blocks(n)), dim3(256), 0, s, neg, g_R, ea, n, part);
 }
 
 void adj_develop_bwd(const float* H, const float* acc, const float* counts, const float* g_G, float* g_H, float* g_acc,

@@ -6,6 +6,8 @@
 // oracle's.  log / exp / log10 come from spk_math.h (binary64 evaluation, one final rounding).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 #include "philox.h"
 #include "spk_math.h"
@@ -249,6 +251,138 @@ __device__ __forceinline__ void drain(GrainScratch& gs, uint32_t count, uint32_t
     }
 }
 
+// ---- input decode, geometry, boost, headroom ---------------------------------------------------
+
+// Input transfer decode of one value (kernels/film.metal:104-146): 0 sRGB, 1 ProPhoto, 2 Adobe,
+// 3 BT.709, else identity.  The powers are spkm::powf_ (binary64, rounded once), as in the oracle.
+__device__ inline float decode_value(float v, uint32_t mode, float bp, float ex) {
+    switch (mode) {
+        case 0:
+            if (v <= 0.040449936f) return v / 12.92f;
+            {
+                const float u = (v + 0.055f) / 1.055f;
+                return (u > 0.0f ? 1.0f : (u < 0.0f ? -1.0f : 0.0f)) * spkm::powf_(fabsf(u), 2.4f);
+            }
+        case 1: return v < 16.0f / 512.0f ? v / 16.0f : spkm::powf_(v, 1.8f);
+        case 2: return spkm::powf_(v, 563.0f / 256.0f);
+        case 3:
+            if (v < bp) return v / 4.5f;
+            {
+                const float u = (v + 0.099f) / 1.099f;
+                return (u > 0.0f ? 1.0f : (u < 0.0f ? -1.0f : 0.0f)) * spkm::powf_(fabsf(u), ex);
+            }
+        default: return v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_decode(const float* __restrict__ in, float* __restrict__ out, uint32_t n,
+                                                uint32_t mode, float bp, float ex) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = decode_value(in[i], mode, bp, ex);
+}
+
+// Bilinear resample of an interleaved 3-channel image at float64 source coordinates
+// f = c + (col[ox] + row[oy]) per axis (film/geometry.cpp:111-152); taps decoded first when asked.
+__device__ inline void geometry_axis(double f, uint32_t n, uint32_t& i0, uint32_t& i1, float& t) {
+    if (f <= 0.0) {
+        i0 = 0;
+        t = 0.0f;
+    } else if (f >= double(n - 1)) {
+        i0 = n - 1;
+        t = 0.0f;
+    } else {
+        const double fl = floor(f);
+        i0 = uint32_t(fl);
+        t = float(f - fl);
+        t = fminf(fmaxf(t, 0.0f), 1.0f);
+    }
+    i1 = min(i0 + 1, n - 1);
+}
+
+__global__ __launch_bounds__(256) void k_geometry(const float* __restrict__ src, float* __restrict__ dst,
+                                                  const double* __restrict__ col, const double* __restrict__ row,
+                                                  GeometryArgs q) {
+    const uint32_t ox = blockIdx.x * blockDim.x + threadIdx.x, oy = blockIdx.y;
+    if (ox >= q.dst_w) return;
+    const double fx = q.cx + (col[2 * ox] + row[2 * oy]);
+    const double fy = q.cy + (col[2 * ox + 1] + row[2 * oy + 1]);
+    uint32_t x0, x1, y0, y1;
+    float tx, ty;
+    geometry_axis(fx, q.src_w, x0, x1, tx);
+    geometry_axis(fy, q.src_h, y0, y1, ty);
+    const float* pa = src + 3 * (size_t(y0) * q.src_w + x0);
+    const float* pb = src + 3 * (size_t(y0) * q.src_w + x1);
+    const float* pd = src + 3 * (size_t(y1) * q.src_w + x0);
+    const float* pe = src + 3 * (size_t(y1) * q.src_w + x1);
+    float* o = dst + 3 * (size_t(oy) * q.dst_w + ox);
+    for (int c = 0; c < 3; ++c) {
+        float a = pa[c], b = pb[c], d = pd[c], e = pe[c];
+        if (q.decode) {
+            a = decode_value(a, q.decode - 1u, q.bt709_bp, q.bt709_exp);
+            b = decode_value(b, q.decode - 1u, q.bt709_bp, q.bt709_exp);
+            d = decode_value(d, q.decode - 1u, q.bt709_bp, q.bt709_exp);
+            e = decode_value(e, q.decode - 1u, q.bt709_bp, q.bt709_exp);
+        }
+        const float top = a * (1.0f - tx) + b * tx;
+        const float bot = d * (1.0f - tx) + e * tx;
+        o[c] = top * (1.0f - ty) + bot * ty;
+    }
+}
+
+// Largest value of n floats, NaN ignored, -inf when none (film/geometry.cpp:208-215): per-block
+// maxima, then one block over them.  max is exact in any order.
+__global__ __launch_bounds__(256) void k_plane_max(const float* __restrict__ in, size_t n, float* __restrict__ out) {
+    __shared__ float red[256];
+    float m = -__builtin_inff();
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        m = fmaxf(m, in[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// Highlight boost (kernels/film.metal:229-238): v > x0: v + k (exp(z) - z - 1), z = a (v - x0) r.
+__global__ __launch_bounds__(256) void k_boost(float* __restrict__ p, size_t n, float x0, float r, float a, float k) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = p[i];
+    if (v <= x0) return;
+    const float delta = (v - x0) * r;
+    const float z = a * delta;
+    const float w = (spkm::expf_(z) - z) - 1.0f;
+    p[i] = v + k * w;
+}
+
+// Back-reflection headroom (kernels/fx_halation.metal:13-39) on planar planes: m := max channel;
+// m > 0: g := 2^(ev s1 s2), s1 = smooth01((log2 m - log2 clip + 1) / 0.75),
+// s2 = smooth01((log2 m - log2 0.184 - 1.5) / 1); every channel times g.
+__device__ inline float smooth01(float t) {
+    const float u = fminf(fmaxf(t, 0.0f), 1.0f);
+    return u * u * (3.0f - 2.0f * u);
+}
+
+__global__ __launch_bounds__(256) void k_headroom(float* __restrict__ p, uint32_t n, float clip, float ev) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v[3] = {p[i], p[size_t(n) + i], p[2 * size_t(n) + i]};
+    const float m = fmaxf(fmaxf(v[0], v[1]), v[2]);
+    float g = 1.0f;
+    if (m > 0.0f) {
+        const float lm = spkm::log2f_(m);
+        const float s1 = smooth01(((lm - spkm::log2f_(clip)) - (-1.0f)) / 0.75f);
+        const float s2 = smooth01(((lm - spkm::log2f_(0.184f)) - 1.5f) / 1.0f);
+        g = spkm::exp2f_((ev * s1) * s2);
+    }
+    p[i] = v[0] * g;
+    p[size_t(n) + i] = v[1] * g;
+    p[2 * size_t(n) + i] = v[2] * g;
+}
+
 // ---- kernels ---------------------------------------------------------------------------------
 
 // RGB -> layer exposure H (kernels/film.metal:189-227), then x f32(2^EC) (film.metal:176-180).
@@ -257,6 +391,11 @@ __global__ __launch_bounds__(256) void k_upsample(const float* __restrict__ rgb,
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= q.n) return;
     float r = rgb[3 * size_t(i)], g = rgb[3 * size_t(i) + 1], b = rgb[3 * size_t(i) + 2];
+    if (q.decode) {  // the input decode (develop.cpp:119-123), when no geometry pass ran
+        r = decode_value(r, q.decode - 1u, q.bt709_bp, q.bt709_exp);
+        g = decode_value(g, q.decode - 1u, q.bt709_bp, q.bt709_exp);
+        b = decode_value(b, q.decode - 1u, q.bt709_bp, q.bt709_exp);
+    }
     if (q.pre_active) {  // the auto-exposure scale of the tier image (develop.cpp:125-131)
         r = r * q.pre_gain;
         g = g * q.pre_gain;
@@ -416,9 +555,8 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
     wave_sync();
     if (!valid) return;
     if (q.counts) {
-        float* cp = q.counts + size_t(i) * 9;
 #pragma unroll
-        for (uint32_t jd = 0; jd < 9; ++jd) cp[jd] = gs.v[jd][lane];
+        for (uint32_t jd = 0; jd < 9; ++jd) q.counts[size_t(jd) * n + i] = gs.v[jd][lane];
     }
 #pragma unroll
     for (uint32_t c = 0; c < 3; ++c) {
@@ -514,6 +652,29 @@ void philox_test(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32
 void poisson_test(const float* lam, float* out, uint32_t n, uint32_t stream, uint32_t seed, const float* lgam,
                   hipStream_t s) {
     hipLaunchKernelGGL(k_poisson_test, dim3((n + 255) / 256), dim3(256), 0, s, lam, out, n, stream, seed, lgam);
+}
+
+void decode(const float* in, float* out, uint32_t n, uint32_t mode, float bp, float ex, hipStream_t s) {
+    hipLaunchKernelGGL(k_decode, dim3((n + 255) / 256), dim3(256), 0, s, in, out, n, mode, bp, ex);
+}
+
+void geometry(const float* src, float* dst, const double* col, const double* row, const GeometryArgs& a,
+              hipStream_t s) {
+    hipLaunchKernelGGL(k_geometry, dim3((a.dst_w + 255) / 256, a.dst_h), dim3(256), 0, s, src, dst, col, row, a);
+}
+
+void plane_max(const float* in, size_t n, float* partial, float* out, hipStream_t s) {
+    const uint32_t blocks = uint32_t(std::min<size_t>((n + 255) / 256, kMaxBlocks));
+    hipLaunchKernelGGL(k_plane_max, dim3(blocks), dim3(256), 0, s, in, n, partial);
+    hipLaunchKernelGGL(k_plane_max, dim3(1), dim3(256), 0, s, partial, size_t(blocks), out);
+}
+
+void boost(float* p, size_t n, float x0, float r, float a, float k, hipStream_t s) {
+    hipLaunchKernelGGL(k_boost, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s, p, n, x0, r, a, k);
+}
+
+void headroom(float* p, uint32_t n, float clip, float ev, hipStream_t s) {
+    hipLaunchKernelGGL(k_headroom, dim3((n + 255) / 256), dim3(256), 0, s, p, n, clip, ev);
 }
 
 void upsample(const float* rgb, float* H, const float* table, const UpsampleArgs& a, hipStream_t s) {

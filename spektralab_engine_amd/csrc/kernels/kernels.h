@@ -49,6 +49,18 @@ struct UpsampleArgs {
     uint32_t ec_active;
     float pre_gain;           // auto exposure: the input is multiplied by pre_gain first
     uint32_t pre_active;
+    uint32_t decode;          // input decode of each value first: 0 none, 1 + mode (mode 0 sRGB, 1 ProPhoto,
+                              // 2 Adobe, 3 BT.709)
+    float bt709_bp, bt709_exp;
+};
+
+// Geometry resample (interleaved 3-channel): output (ox, oy) samples the source at
+// (cx + col[2 ox] + row[2 oy], cy + col[2 ox + 1] + row[2 oy + 1]), float64, bilinear, clamped.
+struct GeometryArgs {
+    uint32_t src_w, src_h, dst_w, dst_h;
+    double cx, cy;
+    uint32_t decode;          // the taps decoded first (as UpsampleArgs)
+    float bt709_bp, bt709_exp;
 };
 
 // The fused develop kernel: halation mix, log10, density curves, grain draw.
@@ -66,7 +78,8 @@ struct DevelopArgs {
     uint32_t coupler_positive;
     float coupler_m[9], coupler_dmax[3];
     float fg[3], fo[3];       // curve argument map fg L + fo (identity: 1, 0)
-    float* counts;            // grain: the Poisson counts drawn, 9 per pixel [9 i + 3 c + l] (or null)
+    float* counts;            // grain: the Poisson counts drawn, 9 planes: draw 3 c + l of pixel i at
+                              // [(3 c + l) n + i] (or null)
 };
 
 // Per-run constants of the enlarger (they follow the live print fields), passed by value.
@@ -76,6 +89,8 @@ struct EnlargerArgs {
     float k_gain;
     float glare_tint[3];      // XYZ += (F / 100) tint with the glare field F when it is given
     float pg[3], po[3];       // paper argument map pg l + po (identity: 1, 0)
+    uint32_t levels_active;   // scanner levels on the scanned XYZ: Y' := clamp(m Y + q, 0, 1)
+    float levels_m, levels_q;
 };
 
 // Pipeline constants of the print side, device-resident (uploaded with the pipeline's tables).
@@ -92,6 +107,9 @@ struct PrintConsts {
     float unsharp_a;          // (1 + a), the unsharp combine's first weight; second is -a
     float unsharp_b;
     uint32_t unsharp_active;
+    uint32_t edr_active, edr_n;  // EDR tone map before the output matrix (table of edr_n log2 targets)
+    float edr_r_y[3], edr_domain_lo, edr_domain_span, edr_toe_y, edr_shoulder_y;
+    float edr_table[SPK_PLAN_MAX_EDR];
 };
 
 // One 81-bin spectral integral R_c = sum_l exp2(e(l) . d + beta(l)) wt_c(l) with the products and
@@ -142,6 +160,16 @@ inline constexpr uint32_t kBatchTile = 256;
 
 // Launchers (stream-ordered, no synchronisation). Planes as documented above.
 void upsample(const float* rgb, float* H, const float* table, const UpsampleArgs& a, hipStream_t s);
+// Input decode of n values (the meter image).
+void decode(const float* in, float* out, uint32_t n, uint32_t mode, float bp, float ex, hipStream_t s);
+void geometry(const float* src, float* dst, const double* col, const double* row, const GeometryArgs& a,
+              hipStream_t s);
+// *out := the largest of n floats (NaN ignored; -inf when none); partial: kMaxBlocks floats.
+inline constexpr uint32_t kMaxBlocks = 1024;
+void plane_max(const float* in, size_t n, float* partial, float* out, hipStream_t s);
+void boost(float* p, size_t n, float x0, float r, float a, float k, hipStream_t s);
+// planar 3-channel planes of n pixels
+void headroom(float* p, uint32_t n, float clip, float ev, hipStream_t s);
 // The passes run on nplanes contiguous planes of a.w x a.h with the same arguments.
 void fir_v(const float* in, float* out, const FirArgs& a, uint32_t nplanes, hipStream_t s);
 void fir_h(const float* in, float* out, const float* acc, const FirArgs& a, uint32_t nplanes, hipStream_t s);
@@ -248,6 +276,13 @@ void adj_dot_diff(const float* g, const float* fp, const float* fm, const float 
                   hipStream_t s);
 // gq[q] := the reduction of terms[q] (device table of nq entries), q < nq.
 void adj_reduce(const AdjTerm* terms, uint32_t nq, const double* part, double* gq, hipStream_t s);
+
+// The print-preview table lookup (session.hip) on the planar negative into RGBA16.
+struct LutArgs {
+    uint32_t w, h, size;
+    float lo[3], s[3];  // s = f32(1 / (hi - lo))
+};
+void print_lut(const float* neg, const float* lut, uint16_t* out, uint32_t stride, const LutArgs& a, hipStream_t s);
 
 // Tier downscale passes on interleaved 3-channel images (session.hip).
 void ds_blur_v(const float* src, float* dst, const float* taps, uint32_t w, uint32_t h, int r, hipStream_t s);

@@ -54,6 +54,41 @@ __device__ inline F3 expose_paper(float r0, float r1, float r2, CEA& e, const fl
 }
 
 
+// Scanner levels (print.metal:234-246): Y' := clamp(m Y + q, 0, 1), XYZ *= Y' / (Y + 1e-10); then the
+// glare field (fx_glare.metal:13-22): XYZ += (F / 100) tint.
+__device__ __forceinline__ F3 levels_glare(F3 xyz, CEA& e, const float* __restrict__ glare, uint32_t i) {
+    if (e.levels_active) {
+        const float Y = xyz.y;
+        const float Yp = fminf(fmaxf(e.levels_m * Y + e.levels_q, 0.0f), 1.0f);
+        const float sc = Yp / (Y + kTiny);
+        xyz = {sc * xyz.x, sc * Y, sc * xyz.z};
+    }
+    if (glare) {
+        const float F = glare[i];
+        xyz.x = xyz.x + (F / 100.0f) * e.glare_tint[0];
+        xyz.y = xyz.y + (F / 100.0f) * e.glare_tint[1];
+        xyz.z = xyz.z + (F / 100.0f) * e.glare_tint[2];
+    }
+    return xyz;
+}
+
+// EDR tone map on luminance (print.metal:319-347), before the output matrix and transfer.  The
+// luminance is summed as written (no contraction), as the oracle does: the map divides by it.
+__device__ __forceinline__ F3 edr_map(F3 v, const PrintConsts& q) {
+#pragma clang fp contract(off)
+    const float y = fmaxf((q.edr_r_y[0] * v.x + q.edr_r_y[1] * v.y) + q.edr_r_y[2] * v.z, 0.0f);
+    float yp = y;
+    if (y > 0.0f && (y < q.edr_toe_y || y > q.edr_shoulder_y)) {
+        const uint32_t N = max(q.edr_n, 2u);
+        const float u = fminf(fmaxf((flog2(y) - q.edr_domain_lo) / q.edr_domain_span, 0.0f), 1.0f) * float(N - 1);
+        const uint32_t i0 = min(uint32_t(floorf(u)), N - 2);
+        const float t = u - float(i0);
+        yp = fexp2((1.0f - t) * q.edr_table[i0] + t * q.edr_table[i0 + 1]);
+    }
+    const float den = y + kTiny;
+    return {(v.x * yp) / den, (v.y * yp) / den, (v.z * yp) / den};
+}
+
 // Pixel i (< n) of one frame: negative densities -> enlarger -> paper -> scan -> output RGB.
 __device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __restrict__ neg, float* __restrict__ rgb,
                                 CPC& q, CEA& e, const float* __restrict__ paper,
@@ -89,12 +124,7 @@ __device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __
         }
         xyz = {fmaxf(r0, 0.0f) + kTiny, fmaxf(r1, 0.0f) + kTiny, fmaxf(r2, 0.0f) + kTiny};
     }
-    if (glare) {  // fx_glare.metal:13-22
-        const float F = glare[i];
-        xyz.x = xyz.x + (F / 100.0f) * e.glare_tint[0];
-        xyz.y = xyz.y + (F / 100.0f) * e.glare_tint[1];
-        xyz.z = xyz.z + (F / 100.0f) * e.glare_tint[2];
-    }
+    xyz = levels_glare(xyz, e, glare, i);
     const F3 o = scan_to_rgb(xyz, q, cmax);
     rgb[i] = o.x;
     rgb[size_t(n) + i] = o.y;
@@ -189,12 +219,7 @@ __global__ __launch_bounds__(256) void k_print16(const float* __restrict__ neg, 
                                fmaxf(fminf(d.z, sh), sl), lane);
     if (i >= n) return;
     F3 xyz{fmaxf(x.x, 0.0f) + kTiny, fmaxf(x.y, 0.0f) + kTiny, fmaxf(x.z, 0.0f) + kTiny};
-    if (glare) {
-        const float F = glare[i];
-        xyz.x = xyz.x + (F / 100.0f) * e.glare_tint[0];
-        xyz.y = xyz.y + (F / 100.0f) * e.glare_tint[1];
-        xyz.z = xyz.z + (F / 100.0f) * e.glare_tint[2];
-    }
+    xyz = levels_glare(xyz, cref(e), glare, i);
     const F3 o = scan_to_rgb(xyz, q, cmax);
     rgb[i] = o.x;
     rgb[size_t(n) + i] = o.y;
@@ -215,6 +240,7 @@ __global__ __launch_bounds__(256) void k_unsharp_encode(const float* __restrict_
         v.y = q.unsharp_a * v.y + q.unsharp_b * blurred[size_t(n) + i];
         v.z = q.unsharp_a * v.z + q.unsharp_b * blurred[2 * size_t(n) + i];
     }
+    if (q.edr_active) v = edr_map(v, q);
     if (q.cctf_active) {
         const F3 m = mat_apply(q.m_nid, v);
         v = {cctf_encode(m.x, q.cctf_mode), cctf_encode(m.y, q.cctf_mode), cctf_encode(m.z, q.cctf_mode)};
@@ -266,6 +292,7 @@ __global__ __launch_bounds__(kUNT) void k_unsharp_encode2d(const float* __restri
         const uint32_t x = uint32_t(x0 + col), y = uint32_t(y0 + row);
         if (x >= w || y >= h) continue;
         F3 v{res[0][u], res[1][u], res[2][u]};
+        if (q.edr_active) v = edr_map(v, q);
         if (q.cctf_active) {
             const F3 m = mat_apply(q.m_nid, v);
             v = {cctf_encode(m.x, q.cctf_mode), cctf_encode(m.y, q.cctf_mode), cctf_encode(m.z, q.cctf_mode)};

@@ -88,7 +88,56 @@ __global__ __launch_bounds__(256) void k_ds_resample(const float* __restrict__ s
     }
 }
 
+// Print-preview table (kernels/print.metal:368-407) on the planar negative: per channel
+// u = clamp((d - lo) s, 0, 1) (S - 1), i0 = floor(u), i1 = min(i0 + 1, S - 1), f = u - i0;
+// trilinear along k, then j, then i; 16-bit encode (NaN -> 0), alpha 65535.
+__device__ inline uint16_t enc16(float v) {
+    if (isnan(v)) return 0;
+    const float c = fminf(fmaxf(v, 0.0f), 1.0f);
+    return uint16_t(floorf(c * 65535.0f + 0.5f));
+}
+
+__global__ __launch_bounds__(256) void k_print_lut(const float* __restrict__ neg, const float* __restrict__ lut,
+                                                   uint16_t* __restrict__ out, uint32_t stride, LutArgs a) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= a.w) return;
+    const size_t n = size_t(a.w) * a.h, i = size_t(y) * a.w + x;
+    const uint32_t S = a.size;
+    uint32_t i0[3], i1[3];
+    float f[3];
+    for (int c = 0; c < 3; ++c) {
+        const float u = fminf(fmaxf((neg[size_t(c) * n + i] - a.lo[c]) * a.s[c], 0.0f), 1.0f) * float(S - 1);
+        const float fl = floorf(u);
+        i0[c] = uint32_t(fl);
+        f[c] = u - fl;
+        i1[c] = min(i0[c] + 1, S - 1);
+    }
+    uint16_t px[4];
+    for (int ch = 0; ch < 3; ++ch) {
+        float cc[2][2];
+        for (int di = 0; di < 2; ++di)
+            for (int dj = 0; dj < 2; ++dj) {
+                const size_t base = (size_t(di ? i1[0] : i0[0]) * S + (dj ? i1[1] : i0[1])) * S;
+                cc[di][dj] = lut[(base + i0[2]) * 3 + ch] * (1.0f - f[2]) + lut[(base + i1[2]) * 3 + ch] * f[2];
+            }
+        const float v = (cc[0][0] * (1.0f - f[1]) + cc[0][1] * f[1]) * (1.0f - f[0]) +
+                        (cc[1][0] * (1.0f - f[1]) + cc[1][1] * f[1]) * f[0];
+        px[ch] = enc16(v);
+    }
+    px[3] = 65535;
+    ushort4 o;
+    o.x = px[0];
+    o.y = px[1];
+    o.z = px[2];
+    o.w = px[3];
+    reinterpret_cast<ushort4*>(out)[size_t(y) * stride + x] = o;
+}
+
 }  // namespace
+
+void print_lut(const float* neg, const float* lut, uint16_t* out, uint32_t stride, const LutArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_print_lut, dim3((a.w + 255) / 256, a.h), dim3(256), 0, s, neg, lut, out, stride, a);
+}
 
 void ds_blur_v(const float* src, float* dst, const float* taps, uint32_t w, uint32_t h, int r, hipStream_t s) {
     hipLaunchKernelGGL(k_ds_blur_v, dim3((w + 63) / 64, (h + 3) / 4), dim3(64, 4), 0, s, src, dst, taps, w, h, r);

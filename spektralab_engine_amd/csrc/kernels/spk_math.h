@@ -165,6 +165,22 @@ SPKM_FN float logf_(float x) {
 
 // float32 results, rounded once from binary64.
 SPKM_FN float expf_(float x) { return float(exp_d(double(x))); }
+// x^y for the input decode: 0 for x == 0 (y > 0), NaN for x < 0 or NaN, exp(y ln x) otherwise.
+SPKM_FN float powf_(float x, float y) {
+    SPKM_NO_CONTRACT
+    if (x == 0.0f) return 0.0f;
+    if (!(x > 0.0f)) return __builtin_nanf("");
+    return float(exp_d(double(y) * log_d(double(x))));
+}
+// log2 and exp2 for the highlight headroom.
+SPKM_FN float log2f_(float x) {
+    SPKM_NO_CONTRACT
+    return float(log_d(double(x)) * 1.44269504088896340736);  // 1 / ln 2
+}
+SPKM_FN float exp2f_(float x) {
+    SPKM_NO_CONTRACT
+    return float(exp_d(double(x) * 0.69314718055994530942));
+}
 
 // lgamma(x) for x = k + 1 as the grain sampler evaluates it (reference kernels/fx_random.h:19-41):
 // Stirling's series in 1/x0^2 with an upward shift to x0 >= 7, binary32 in the written order;

@@ -30,6 +30,7 @@ import numpy as np
 WL = 380.0 + 5.0 * np.arange(81)
 FILM_STOCK = "spk_synth_negative"
 PAPER_STOCK = "spk_synth_paper"
+PAPER_STOCK_EDR = "spk_synth_paper_edr"  # the same paper with an EDR tone map
 
 DTYPE_CODE = {np.dtype("float64"): 0, np.dtype("float32"): 1, np.dtype("float16"): 2,
               np.dtype("int32"): 3, np.dtype("uint8"): 4}
@@ -231,11 +232,37 @@ def paper_profile() -> dict:
     }
 
 
+def edr_tone_map() -> dict:
+    """A tone map over log2 Y in [-10, 4]: identity up to the shoulder at Y = 0.5, then half the slope;
+    the toe (Y < 2^-8) lifted by a quarter stop."""
+    x = np.linspace(-10.0, 4.0, 57)
+    t = np.where(x > -1.0, -1.0 + 0.5 * (x + 1.0), x)
+    t = np.where(x < -8.0, x + 0.25 * (-8.0 - x) / 2.0, t)
+    return {"version": 1, "domain_log2_y": [-10.0, 4.0], "toe_y": 2.0 ** -8, "shoulder_y": 0.5,
+            "target_log2_y": t.tolist()}
+
+
+SYNTH_VERSION = "3"  # bumped when the folder's contents change
+LUT_SIZE = 17
+
+
+def print_lut(S: int = LUT_SIZE) -> tuple[np.ndarray, np.ndarray]:
+    """A print-preview table for the synthetic paper: (S, S, S, 3) float32 over negative densities
+    -0.2 .. 3.0 per channel (axes (3, S)); each output channel a smooth transmittance of the three
+    densities (not the engine's print: a stand-in with the table's format)."""
+    ax = np.linspace(-0.2, 3.0, S)
+    d0, d1, d2 = np.meshgrid(ax, ax, ax, indexing="ij")
+    mix = np.array([[0.9, 0.08, 0.02], [0.07, 0.85, 0.08], [0.03, 0.1, 0.87]])
+    t = np.stack([10.0 ** (-(mix[c, 0] * d0 + mix[c, 1] * d1 + mix[c, 2] * d2) * 0.6) for c in range(3)], axis=-1)
+    return t.astype(np.float32), np.stack([ax, ax, ax]).astype(np.float32)
+
+
 def make_resources(directory, L: int = 192) -> Path:
-    """Writes the folder (idempotent: an existing complete folder of the same L is kept)."""
+    """Writes the folder (idempotent: an existing complete folder of the same L and version is kept)."""
     d = Path(directory)
     stamp = d / ".synth_L"
-    if stamp.exists() and stamp.read_text().strip() == str(L) and (d / "spektralab_constants.bin").exists():
+    tag = f"{L}-{SYNTH_VERSION}"
+    if stamp.exists() and stamp.read_text().strip() == tag and (d / "spektralab_constants.bin").exists():
         return d
     (d / "profiles").mkdir(parents=True, exist_ok=True)
     cmf = cmfs_1931()
@@ -259,10 +286,21 @@ def make_resources(directory, L: int = 192) -> Path:
         entries[p + "whitepoint"] = np.array(white, dtype=np.float64)
         entries[p + "matrix_RGB_to_XYZ"] = M
         entries[p + "matrix_XYZ_to_RGB"] = np.linalg.inv(M)
+    lut, axes = print_lut()
+    entries[f"print_lut/{PAPER_STOCK}"] = lut
+    entries[f"print_lut_axes/{PAPER_STOCK}"] = axes
     write_blob(d / "spektralab_constants.bin", entries)
+    (d / "print_luts.json").write_text(json.dumps({PAPER_STOCK: {
+        "paired_film": FILM_STOCK, "lut_size": LUT_SIZE, "declared_pairing": True,
+        "output_color_space": "ProPhoto RGB", "output_cctf_encoding": True}}))
     (d / "profiles" / f"{FILM_STOCK}.json").write_text(json.dumps(film_profile(cmf, table)))
     (d / "profiles" / f"{PAPER_STOCK}.json").write_text(json.dumps(paper_profile()))
-    stamp.write_text(str(L))
+    edr = paper_profile()
+    edr["info"]["stock"] = PAPER_STOCK_EDR
+    edr["info"]["name"] = "Synthetic colour paper (EDR tone map)"
+    edr["data"]["edr_tone_map"] = edr_tone_map()
+    (d / "profiles" / f"{PAPER_STOCK_EDR}.json").write_text(json.dumps(edr))
+    stamp.write_text(tag)
     return d
 
 

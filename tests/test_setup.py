@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import spektralab_engine_amd as spk
+from spektralab_engine_amd import synth
 from conftest import CONFIG1, CONFIG2, CONFIG2_OVERRIDES, delta
 
 
@@ -140,12 +141,37 @@ def test_halation_mixture_override(resources, oracle):
 
 
 def test_refuses_stages_it_does_not_build(resources):
-    with pytest.raises(spk.SpkError, match="geometry"):
-        spk.host_plan(resources, delta(geometry_quarter_turns=1), 64, 64)
-    with pytest.raises(spk.SpkError, match="input_cctf_decoding"):
-        spk.host_plan(resources, delta(input_cctf_decoding=True), 64, 64)
+    with pytest.raises(spk.SpkError, match="scan_film"):
+        spk.host_plan(resources, delta(scan_film=True), 64, 64)
+    with pytest.raises(spk.SpkError, match="grain_sublayers_active"):
+        spk.host_plan(resources, delta(grain_active=True, grain_sublayers_active=False), 64, 64)
+    # a paper without an EDR tone map
     with pytest.raises(spk.SpkError, match="extended_dynamic_range"):
         spk.host_plan(resources, delta(extended_dynamic_range=True), 64, 64)
+
+
+def test_geometry_plan_sizes(oracle, resources):
+    """Output sizes (geometry.cpp:45-62): round(crop * source), swapped by an odd quarter turn."""
+    plan = spk.host_plan(resources, delta(geometry_crop_w=0.5, geometry_crop_h=0.25, geometry_quarter_turns=1),
+                         640, 480)
+    assert oracle.plan_size(plan) == (120, 320)
+    assert oracle.source_size(plan) == (640, 480)
+    plan = spk.host_plan(resources, delta(geometry_rotation_deg=10.0), 640, 480)
+    assert oracle.plan_size(plan) == (640, 480) and oracle.field(plan, "geometry_active")[0] == 1
+    plan = spk.host_plan(resources, delta(), 640, 480)
+    assert oracle.field(plan, "geometry_active")[0] == 0
+
+
+def test_levels_and_edr_plan_fields(oracle, resources):
+    """Levels: the black and white targets B = 0.01 / 12.92 and W = ((0.98 + 0.055) / 1.055)^2.4 map the
+    film's Dmin and white density (print/setup.cpp:373-381): m y_k + q = B and m y_w + q = W, so
+    with both on, m = (W - B) / (y_w - y_k) > 0.  EDR: the paper's tone-map nodes, narrowed."""
+    plan = spk.host_plan(resources, delta(scanner_black_correction=True, scanner_white_correction=True), 64, 64)
+    assert oracle.field(plan, "levels_active")[0] == 1 and oracle.field(plan, "levels_m")[0] > 0
+    plan = spk.host_plan(resources, delta(extended_dynamic_range=True, print_stock=synth.PAPER_STOCK_EDR), 64, 64)
+    assert oracle.field(plan, "edr_active")[0] == 1 and oracle.field(plan, "edr_n")[0] == 57
+    want = np.float32(synth.edr_tone_map()["target_log2_y"])
+    assert np.array_equal(oracle.field(plan, "edr_table", 57), want)
 
 
 def test_coupler_and_glare_plan_fields(oracle, resources):
