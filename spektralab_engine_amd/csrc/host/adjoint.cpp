@@ -174,6 +174,7 @@ k::DevelopArgs Diff::develop_args(const std::vector<double>& th) const {
     std::memcpy(da.dmin_l, p.dmin_l, sizeof da.dmin_l);
     std::memcpy(da.sat_d, p.sat_d, sizeof da.sat_d);
     std::memcpy(da.count, p.count, sizeof da.count);
+    k::set_grain_scale(da);
     std::memcpy(da.uniformity, p.uniformity, sizeof da.uniformity);
     std::memcpy(da.dmin_c, p.dmin_c, sizeof da.dmin_c);
     return da;

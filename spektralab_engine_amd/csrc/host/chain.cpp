@@ -620,6 +620,7 @@ void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* ne
     std::memcpy(da.dmin_l, p.dmin_l, sizeof da.dmin_l);
     std::memcpy(da.sat_d, p.sat_d, sizeof da.sat_d);
     std::memcpy(da.count, p.count, sizeof da.count);
+    k::set_grain_scale(da);
     std::memcpy(da.uniformity, p.uniformity, sizeof da.uniformity);
     std::memcpy(da.dmin_c, p.dmin_c, sizeof da.dmin_c);
     for (int c = 0; c < 3; ++c) {

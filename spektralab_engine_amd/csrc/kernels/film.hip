@@ -181,13 +181,17 @@ __device__ inline uint32_t lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
 }
 
-// Drains `count` listed draws of one method; base: the wavefront's first pixel.
+// Drains `count` listed draws of one method; base: the wavefront's first pixel.  A PTRS draw
+// makes one try per pass of the loop: the (x, y) pair of a fresh Philox block, or the (z, w) pair
+// the lane kept from its previous pass.  A try accepts 75-87 % of the time (lam 10..300), so
+// running both tries of a block in every pass would leave most lanes idle through the second one.
 template <bool kLarge>
 __device__ __forceinline__ void drain(GrainScratch& gs, uint32_t count, uint32_t lane, uint32_t base, uint32_t seed,
                                       float hl2p, const float* __restrict__ lgam) {
     uint32_t head = 0;
     bool busy = false;
-    uint32_t j = 0, src = 0, blk = 0, stream = 0;
+    bool second = false;  // PTRS: the current block's (z, w) pair is next
+    uint32_t j = 0, src = 0, blk = 0, stream = 0, kz = 0, kw = 0;
     float lam = 0.0f, E = 0.0f, X = 0.0f, prod = 1.0f;
     Ptrs P{};
     for (;;) {
@@ -204,6 +208,7 @@ __device__ __forceinline__ void drain(GrainScratch& gs, uint32_t count, uint32_t
                 blk = 0;
                 busy = true;
                 if (kLarge) {
+                    second = false;
                     ptrs_setup(P, lam);
                 } else {
                     E = spkm::expf_(-lam);
@@ -215,17 +220,33 @@ __device__ __forceinline__ void drain(GrainScratch& gs, uint32_t count, uint32_t
         head = min(head + uint32_t(__popcll(wm)), count);
         if (__ballot(busy) == 0) break;
         if (busy) {
-            const uint4 w = philox_block(base + src, stream, blk, seed);
             bool done = false;
             float r = 0.0f;
             if (kLarge) {
-                if (ptrs_try(P, u01(w.x), u01(w.y), lgam, hl2p, &r)) done = true;
-                else if (ptrs_try(P, u01(w.z), u01(w.w), lgam, hl2p, &r)) done = true;
-                else if (++blk == 32u) {
-                    r = floorf(lam);
+                uint32_t wu, wv;
+                if (!second) {
+                    const uint4 w = philox_block(base + src, stream, blk, seed);
+                    wu = w.x;
+                    wv = w.y;
+                    kz = w.z;
+                    kw = w.w;
+                } else {
+                    wu = kz;
+                    wv = kw;
+                }
+                if (ptrs_try(P, u01(wu), u01(wv), lgam, hl2p, &r)) {
                     done = true;
+                } else if (!second) {
+                    second = true;
+                } else {
+                    second = false;
+                    if (++blk == 32u) {
+                        r = floorf(lam);
+                        done = true;
+                    }
                 }
             } else {
+                const uint4 w = philox_block(base + src, stream, blk, seed);
                 const uint32_t wv[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
@@ -509,7 +530,8 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
                 d[l] = y0 + t * (y1 - y0);
             }
         }
-        // sublayer_draw's arithmetic, the Poisson draw itself deferred (fx_random.h:91-99)
+        // sublayer_draw's arithmetic, the Poisson draw itself deferred (fx_random.h:91-99); D / n
+        // is the host's binary32 quotient, the same float the division here would give
         for (uint32_t l = 0; l < 3; ++l) {
             const uint32_t lc = l * 3 + c;
             const float dl = d[l] + q.dmin_l[lc];
@@ -518,7 +540,7 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
             qq = qq < 1e-6f ? 1e-6f : (qq > 0.999999f ? 0.999999f : qq);
             const float sat = 1.0f - ((qq * q.uniformity[c]) * 0.999999f);
             gs.v[3 * c + l][lane] = (nl / sat) * qq;
-            mult[3 * c + l] = (Dl / nl) * sat;
+            mult[3 * c + l] = q.d_per_n[lc] * sat;
         }
     }
     if (q.coupler_stage == 1u) {

@@ -72,6 +72,7 @@ struct DevelopArgs {
     uint32_t write_density;   // also write the pre-grain density (grain_amount != 1)
     uint32_t seed, positive, gk;
     float dmin_l[9], sat_d[9], count[9], uniformity[3], dmin_c[3];
+    float d_per_n[9];         // sat_d / count in binary32 (set by set_grain_scale)
     // DIR couplers: stage 1 writes L to out and chi := M^T s(D) to density; stage 2 reads L from H
     // and the blurred chi from acc and develops D := F(L - chi; y0) (then grain as usual).
     uint32_t coupler_stage;   // 0: no couplers
@@ -81,6 +82,12 @@ struct DevelopArgs {
     float* counts;            // grain: the Poisson counts drawn, 9 planes: draw 3 c + l of pixel i at
                               // [(3 c + l) n + i] (or null)
 };
+
+// The per-draw density of one grain of sub-layer l, channel c: D_lc / n_lc rounded once in
+// binary32, as the draw's scale (D / n) sat needs it.
+inline void set_grain_scale(DevelopArgs& a) {
+    for (int i = 0; i < 9; ++i) a.d_per_n[i] = a.sat_d[i] / a.count[i];
+}
 
 // Per-run constants of the enlarger (they follow the live print fields), passed by value.
 // eps and beta (and the scan's) are pre-scaled by -log2(10): transmittance = exp2(eps . d + beta).
