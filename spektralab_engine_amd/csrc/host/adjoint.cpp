@@ -238,6 +238,7 @@ void Diff::forward(int slot, const double* theta, const float* input, float* enc
     }
     float* neg = plane(sl.neg, 3 * np);
     k::DevelopArgs da = develop_args(th);
+    chain::set_hints(da, t);
     if (grain_) {
         float* counts = plane(sl.counts, 9 * np);
         da.counts = counts;

@@ -391,6 +391,33 @@ Profiler::~Profiler() {
     for (hipEvent_t e : pool) (void)hipEventDestroy(e);
 }
 
+bool axis_hints(const float* a, size_t st, size_t n, uint16_t* hint, float& lo, float& scale) {
+    lo = scale = 0.0f;
+    if (n < 2 || n > 65535) return false;
+    for (size_t i = 1; i < n; ++i)
+        if (!(a[i * st] >= a[(i - 1) * st])) return false;
+    const double first = a[0], last = a[(n - 1) * st];
+    if (!(last > first)) return false;
+    const double width = (last - first) / k::kHintBins;
+    size_t up = 0;  // upper_bound of the bin's lower edge
+    for (uint32_t b = 0; b < k::kHintBins; ++b) {
+        const double edge = first + b * width;
+        while (up < n && double(a[up * st]) <= edge) ++up;
+        hint[b] = uint16_t(std::min(up, n - 1));
+    }
+    lo = a[0];
+    scale = num::f32(k::kHintBins / (last - first));
+    return scale > 0.0f && std::isfinite(scale);
+}
+
+void set_hints(k::DevelopArgs& a, const Tables& t) {
+    a.hints = t.hints ? gpu::as<const uint16_t>(t.hints) : nullptr;
+    for (int i = 0; i < 4; ++i) {
+        a.hint_lo[i] = a.hints ? t.hint_lo[i] : 0.0f;
+        a.hint_scale[i] = a.hints ? t.hint_scale[i] : 0.0f;
+    }
+}
+
 Tables upload_tables(const spk_plan& p, gpu::Pool& pool) {
     Tables t;
     t.L = p.L;
@@ -401,6 +428,16 @@ Tables upload_tables(const spk_plan& p, gpu::Pool& pool) {
     t.gaxis = upload(p.grain_axis, 3 * gk * sizeof(float), pool);
     t.ginv = upload(p.grain_inv, 3 * gk * sizeof(float), pool);
     t.glayers = upload(p.grain_layers, 9 * gk * sizeof(float), pool);
+    {
+        std::vector<uint16_t> hints(4 * size_t(k::kHintBins), 0);
+        bool any = false;
+        if (axis_hints(p.curves, 1, p.K, hints.data(), t.hint_lo[0], t.hint_scale[0])) any = true;
+        for (int c = 0; c < 3 && p.grain_k > 0; ++c)
+            if (axis_hints(p.grain_axis + c, 3, p.grain_k, hints.data() + (1 + c) * k::kHintBins, t.hint_lo[1 + c],
+                           t.hint_scale[1 + c]))
+                any = true;
+        if (any) t.hints = upload(hints.data(), hints.size() * sizeof(uint16_t), pool);
+    }
     const std::vector<float> lg = lgamma_table();
     t.lgam = upload(lg.data(), lg.size() * sizeof(float), pool);
     t.paper = upload(p.paper, 5 * size_t(std::max<uint32_t>(p.paper_k, 1)) * sizeof(float), pool);
@@ -621,6 +658,7 @@ void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* ne
     std::memcpy(da.sat_d, p.sat_d, sizeof da.sat_d);
     std::memcpy(da.count, p.count, sizeof da.count);
     k::set_grain_scale(da);
+    set_hints(da, t);
     std::memcpy(da.uniformity, p.uniformity, sizeof da.uniformity);
     std::memcpy(da.dmin_c, p.dmin_c, sizeof da.dmin_c);
     for (int c = 0; c < 3; ++c) {

@@ -42,9 +42,16 @@ std::vector<float> lgamma_table();
 
 struct Tables {
     gpu::DevBuf table, curves, gaxis, ginv, glayers, lgam, paper, cmax, print_consts, scan_mfma;
+    gpu::DevBuf hints;  // the develop kernel's search hints (k::kHintBins per axis; null: none)
+    float hint_lo[4] = {0, 0, 0, 0}, hint_scale[4] = {0, 0, 0, 0};
     uint32_t L = 0, K = 0;
 };
 Tables upload_tables(const spk_plan& p, gpu::Pool& pool);
+// Points the develop arguments at the tables' search hints.
+void set_hints(k::DevelopArgs& a, const Tables& t);
+// Search hints of the axis a[0], a[st], ..., a[(n - 1) st] into hint (k::kHintBins entries);
+// returns false, leaving lo and scale 0, for an axis that is not non-decreasing or too short.
+bool axis_hints(const float* a, size_t st, size_t n, uint16_t* hint, float& lo, float& scale);
 
 // rgb: interleaved w x h x 3 on the device; negative: 3 planar planes (w*h*3 floats).
 void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,

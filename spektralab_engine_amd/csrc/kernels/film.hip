@@ -62,17 +62,35 @@ __device__ inline void mitchell_axis(float t, uint32_t L, int* idx, float* w) {
     w[3] = mitchell(frac - 2.0f);
 }
 
-// F(v; x, inv, y): clamped piecewise-linear with a right bisection (kernels/film.metal:88-101).
+// upper_bound of v in the axis a[0], a[st], ..., a[(n - 1) st]: the first index whose value
+// exceeds v, for a v strictly inside the axis range (or NaN).  With hints (scale > 0, the axis
+// non-decreasing) the search starts at the hint of v's bin and walks to the exact place, the
+// index the right bisection below finds; without, it bisects (kernels/film.metal:88-101).
+__device__ inline uint32_t axis_upper(const float* __restrict__ a, uint32_t st, uint32_t n, float v,
+                                      const uint16_t* __restrict__ hint, float lo, float scale) {
+    if (scale > 0.0f && hint) {
+        const float f = fminf(fmaxf((v - lo) * scale, 0.0f), float(kHintBins - 1));  // NaN -> bin 0
+        uint32_t i = min(uint32_t(hint[uint32_t(f)]), n - 1);
+        while (i < n - 1 && a[i * st] <= v) ++i;  // a[(n - 1) st] > v
+        while (i > 0 && a[(i - 1) * st] > v) --i;
+        return i;
+    }
+    uint32_t l = 0, h = n;
+    while (l < h) {
+        const uint32_t mid = l + (h - l) / 2;
+        if (a[mid * st] <= v) l = mid + 1;
+        else h = mid;
+    }
+    return l;
+}
+
+// F(v; x, inv, y): clamped piecewise-linear between the samples around v (kernels/film.metal:88-101).
 __device__ inline float curve_lookup(const float* __restrict__ x, const float* __restrict__ inv,
-                                     const float* __restrict__ y, uint32_t K, float v) {
+                                     const float* __restrict__ y, uint32_t K, float v, const uint16_t* hint,
+                                     float hlo, float hscale) {
     if (v <= x[0]) return y[0];
     if (v >= x[K - 1]) return y[K - 1];
-    uint32_t lo = 0, hi = K;
-    while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (x[mid] <= v) lo = mid + 1;
-        else hi = mid;
-    }
+    const uint32_t lo = axis_upper(x, 1, K, v, hint, hlo, hscale);
     const uint32_t l = lo == 0 ? 0 : lo - 1;
     const float t = (v - x[l]) * inv[l];
     return y[l] + t * (y[l + 1] - y[l]);
@@ -491,12 +509,13 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
         float D;
         if (q.coupler_stage == 2u) {  // film.metal:282-292: F(L - chi; y0)
             const float l0 = H[o] - acc[o];
-            D = curve_lookup(x, inv, curves + 5 * K + c * K, K, q.fg[c] * l0 + q.fo[c]);
+            D = curve_lookup(x, inv, curves + 5 * K + c * K, K, q.fg[c] * l0 + q.fo[c], q.hints, q.hint_lo[0],
+                             q.hint_scale[0]);
         } else {
             float v = H[o];
             if (q.mix_active) v = q.mix_a[c] * v + q.mix_b[c] * acc[o];
             const float L = spkm::log10f_(fmaxf(v, 0.0f) + 1e-10f);
-            D = curve_lookup(x, inv, y + c * K, K, q.fg[c] * L + q.fo[c]);
+            D = curve_lookup(x, inv, y + c * K, K, q.fg[c] * L + q.fo[c], q.hints, q.hint_lo[0], q.hint_scale[0]);
             if (q.coupler_stage == 1u) {  // film.metal:264-278
                 out[o] = L;
                 s1[c] = q.coupler_positive ? q.coupler_dmax[c] - D : D;
@@ -516,12 +535,8 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
         } else if (vv >= gaxis[3 * (GK - 1) + c]) {
             for (uint32_t l = 0; l < 3; ++l) d[l] = glayers[((GK - 1) * 3 + l) * 3 + c];
         } else {
-            uint32_t lo = 0, hi = GK;
-            while (lo < hi) {
-                const uint32_t mid = lo + (hi - lo) / 2;
-                if (gaxis[3 * mid + c] <= vv) lo = mid + 1;
-                else hi = mid;
-            }
+            const uint16_t* gh = q.hints ? q.hints + (1 + c) * kHintBins : nullptr;
+            const uint32_t lo = axis_upper(gaxis + c, 3, GK, vv, gh, q.hint_lo[1 + c], q.hint_scale[1 + c]);
             const uint32_t l0 = lo > 0 ? lo - 1 : 0;  // lo == 0 only for a NaN or a non-monotone axis
             const float t = (vv - gaxis[3 * l0 + c]) * ginv[3 * l0 + c];
             for (uint32_t l = 0; l < 3; ++l) {

@@ -63,6 +63,11 @@ struct GeometryArgs {
     float bt709_bp, bt709_exp;
 };
 
+// Search hints of a non-decreasing table axis: kHintBins equal bins over [first, last) of the axis;
+// entry b is the number of axis values <= the bin's lower edge, the point from which a short
+// walk finds upper_bound(axis, v) for a v in that bin.
+inline constexpr uint32_t kHintBins = 1024;
+
 // The fused develop kernel: halation mix, log10, density curves, grain draw.
 struct DevelopArgs {
     uint32_t w, h, K;
@@ -79,6 +84,10 @@ struct DevelopArgs {
     uint32_t coupler_positive;
     float coupler_m[9], coupler_dmax[3];
     float fg[3], fo[3];       // curve argument map fg L + fo (identity: 1, 0)
+    // Axis 0: the curves' x; axis 1 + c: the grain axis of channel c.  Bin of v:
+    // (v - hint_lo[a]) hint_scale[a]; hint_scale[a] = 0 (or hints null): bisect instead.
+    const uint16_t* hints;    // 4 x kHintBins
+    float hint_lo[4], hint_scale[4];
     float* counts;            // grain: the Poisson counts drawn, 9 planes: draw 3 c + l of pixel i at
                               // [(3 c + l) n + i] (or null)
 };
