@@ -126,6 +126,12 @@ SPK_API spk_status spkx_gaussian(spk_engine* engine, const float* in, float* out
  * *out_w, *out_h always and the image when out is not NULL and cap_floats allows. */
 SPK_API spk_status spkx_downscale(spk_engine* engine, const float* in, uint32_t width, uint32_t height,
                                   uint32_t long_edge, float* out, size_t cap_floats, uint32_t* out_w, uint32_t* out_h);
+/* Measured vector-ALU issue rates of the engine's GPU, the roofs of the compute-bound kernels (no
+ * counterpart in the reference): {"compute_units", "clock_mhz", "waves", "instructions_per_wave",
+ * "rates": {kind: {"ms", "wave_instructions_per_s", "per_cu_per_clock"}}} for the kinds v_fma_f32,
+ * v_exp_f32, v_fma_f64 and v_mad_u64_u32 (each timed alone: 8 independent chains per thread,
+ * 32 waves per compute unit, median of 5 launches). */
+SPK_API spk_status spkx_valu_roofs(spk_engine* engine, char** out_json);
 
 #ifdef __cplusplus
 }

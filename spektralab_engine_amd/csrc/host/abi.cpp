@@ -4,10 +4,12 @@
 // spk_capabilities, spk_get_params, spk_cancel, spk_string_free, spk_last_error), check required
 // pointers (SPK_ERR_INVALID_ARG), call the engine or session inside a try, map exceptions to a
 // status plus message.  Nothing throws across the boundary.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../kernels/kernels.h"
 #include "adjoint.h"
@@ -511,6 +513,64 @@ SPK_API spk_status spkx_poisson(spk_engine* engine, const float* lam, uint32_t n
                              static_cast<float*>(g->ptr), nullptr);
         spk::gpu::check(hipGetLastError(), "poisson kernel");
         spk::gpu::check(hipMemcpy(out, o->ptr, size_t(n) * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        return SPK_OK;
+    });
+}
+
+SPK_API spk_status spkx_valu_roofs(spk_engine* engine, char** out_json) {
+    le::clear();
+    if (out_json) *out_json = nullptr;
+    if (!engine || !out_json) return invalid("a required argument is NULL");
+    return guarded([&] {
+        Engine& e = *E(engine);
+        spk::gpu::DeviceScope ds(e.device());
+        hipDeviceProp_t prop{};
+        spk::gpu::check(hipGetDeviceProperties(&prop, e.device()), "hipGetDeviceProperties");
+        const uint32_t cus = uint32_t(prop.multiProcessorCount);
+        const uint32_t blocks = cus * 8u;  // 32 waves per compute unit, 8 per SIMD
+        const uint32_t iters = 2048u;
+        const double waves = double(blocks) * 4.0;
+        const double per_wave = double(iters) * spk::k::valu_probe_chains();
+        spk::gpu::DevBuf o = e.pool().acquire(size_t(blocks) * 256u * sizeof(float));
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        spk::gpu::check(hipEventCreate(&t0), "hipEventCreate");
+        spk::gpu::check(hipEventCreate(&t1), "hipEventCreate");
+        const double clock_hz = double(prop.clockRate) * 1e3;
+        spk::json::Value r;
+        r["compute_units"] = double(cus);
+        r["clock_mhz"] = double(prop.clockRate) / 1e3;
+        r["waves"] = waves;
+        r["instructions_per_wave"] = per_wave;
+        spk::json::Value rates;
+        const char* names[4] = {"v_fma_f32", "v_exp_f32", "v_fma_f64", "v_mad_u64_u32"};
+        for (int kind = 0; kind < 4; ++kind) {
+            float* out = static_cast<float*>(o->ptr);
+            spk::k::valu_probe(kind, iters, blocks, out, nullptr);  // warm-up
+            spk::gpu::check(hipGetLastError(), "probe kernel");
+            spk::gpu::check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            std::vector<float> ms;
+            for (int rep = 0; rep < 5; ++rep) {
+                spk::gpu::check(hipEventRecord(t0, nullptr), "hipEventRecord");
+                spk::k::valu_probe(kind, iters, blocks, out, nullptr);
+                spk::gpu::check(hipEventRecord(t1, nullptr), "hipEventRecord");
+                spk::gpu::check(hipEventSynchronize(t1), "hipEventSynchronize");
+                float m = 0.0f;
+                spk::gpu::check(hipEventElapsedTime(&m, t0, t1), "hipEventElapsedTime");
+                ms.push_back(m);
+            }
+            std::sort(ms.begin(), ms.end());
+            const double sec = double(ms[ms.size() / 2]) * 1e-3;
+            const double rate = waves * per_wave / sec;
+            spk::json::Value k;
+            k["ms"] = sec * 1e3;
+            k["wave_instructions_per_s"] = rate;
+            k["per_cu_per_clock"] = rate / (double(cus) * clock_hz);
+            rates[names[kind]] = std::move(k);
+        }
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        r["rates"] = std::move(rates);
+        put_json(out_json, spk::json::write(r));
         return SPK_OK;
     });
 }

@@ -112,7 +112,8 @@ __device__ inline bool ptrs_try(const Ptrs& P, float u, float v, const float* __
     *k_out = k;
     if (us >= 0.07f && V <= P.vr) return true;
     if (k < 0.0f || (us < 0.013f && V > us)) return false;
-    const float Lh = ((spkm::logf_(V) + P.loginv) - spkm::logf_(P.a / (us * us) + P.b));
+    // V in (0, 1] and a / us^2 + b >= b > 0 (us >= 2^-25): logf_pos is logf_ there
+    const float Lh = ((spkm::logf_pos(V) + P.loginv) - spkm::logf_pos(P.a / (us * us) + P.b));
     const float lg = k < float(spkm::kLgammaTable) ? lgam[int(k)] : spkm::lgamma_k(k + 1.0f, hl2p);
     const float Rt = ((-P.lam + k * P.loglam) - lg);
     return Lh <= Rt;
@@ -125,12 +126,13 @@ __device__ inline bool ptrs_try(const Ptrs& P, float u, float v, const float* __
 __device__ inline void ptrs_setup(Ptrs& P, float lam) {
     P.lam = lam;
     const float slam = sqrtf(lam);
-    P.loglam = spkm::logf_(lam);
     P.b = 0.931f + (2.53f * slam);
     P.a = -0.059f + (0.02483f * P.b);
     const float invalpha = 1.1239f + (1.1328f / (P.b - 3.4f));
     P.vr = 0.9277f - (3.6224f / (P.b - 2.0f));
-    P.loginv = spkm::logf_(invalpha);
+    // lam >= 10 and invalpha > 1.1239 are finite and positive (lam is at most n / 2e-6 here)
+    P.loglam = spkm::logf_pos(lam);
+    P.loginv = spkm::logf_pos(invalpha);
 }
 
 __device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t seed, float hl2p,

@@ -224,6 +224,12 @@ void develop_batch(const DevelopJob* jobs, const uint32_t* tile_prefix, uint32_t
                    uint32_t grid, uint32_t block, hipStream_t s);
 void print_batch(const PrintJob* jobs, const uint32_t* tile_prefix, uint32_t nframes, uint32_t grid, uint32_t block,
                  hipStream_t s);
+// VALU issue-rate probes (probe.hip): blocks x 256 threads, each running valu_probe_chains()
+// independent chains of iters instructions of one kind; out: one float per thread.
+inline constexpr int kProbeFmaF32 = 0, kProbeExpF32 = 1, kProbeFmaF64 = 2, kProbeMadU64 = 3;
+uint32_t valu_probe_chains();
+void valu_probe(int kind, uint32_t iters, uint32_t blocks, float* out, hipStream_t s);
+
 void philox_test(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n, hipStream_t s);
 void poisson_test(const float* lam, float* out, uint32_t n, uint32_t stream, uint32_t seed, const float* lgam,
                   hipStream_t s);

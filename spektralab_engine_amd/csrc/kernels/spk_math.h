@@ -145,10 +145,9 @@ SPKM_FN double fma_d(double a, double b, double c) { return __builtin_fma(a, b, 
 // |r| <= 2^-8); ln x = (e ln2 + logc) + log1p(r), log1p by its Taylor series through r^5
 // (truncation < 2^-48 / 6).  Relative error before the final rounding < 2^-50: the float result
 // is the correctly rounded one except within that distance of a rounding midpoint.
-SPKM_FN float logf_(float x) {
+// logf_pos: the same function for a finite x > 0 (no special-case tests).
+SPKM_FN float logf_pos(float x) {
     SPKM_NO_CONTRACT
-    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_inff() : __builtin_nanf("");
-    if (x == __builtin_inff()) return x;
     const uint64_t u = bits_of(double(x));  // float subnormals are normal in binary64
     const int e = int((u >> 52) & 0x7FF) - 1023;
     const uint32_t i = uint32_t(u >> 45) & 0x7Fu;
@@ -161,6 +160,11 @@ SPKM_FN float logf_(float x) {
     const double lp = fma_d(r * r, p, r);
     const double t = fma_d(double(e), 0x1.62e42fefa39efp-1, logc);
     return float(t + lp);
+}
+SPKM_FN float logf_(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_inff() : __builtin_nanf("");
+    if (x == __builtin_inff()) return x;
+    return logf_pos(x);
 }
 
 // float32 results, rounded once from binary64.
