@@ -149,6 +149,21 @@ def main():
         entry = json.loads(tf.read_text()).get("kernels", {}).get(STAGE_KERNEL[dom])
         if entry:
             traffic = round(entry["hbm_bytes_per_launch"])
+    # The compute-bound kernels' roof: the vector-ALU issue rate, measured live (spkx_valu_roofs),
+    # against the dominant kernel's VALU instructions per wave from the hardware counters
+    # (rocprofv3 SQ_INSTS_VALU / SQ_WAVES of the same workload, in the traffic profile).
+    roofs = eng.valu_roofs()
+    valu = None
+    if tf.exists() and (W, H) == (6000, 4000):
+        entry = json.loads(tf.read_text()).get("kernels", {}).get(STAGE_KERNEL[dom])
+        if entry and entry.get("waves_per_launch"):
+            peak = roofs["rates"]["v_fma_f32"]["wave_instructions_per_s"]
+            got = entry["valu_insts_per_wave"] * entry["waves_per_launch"] / (dom_ms / 1000.0)
+            valu = {"bound": "valu", "kernel": STAGE_KERNEL[dom], "achieved": round(got / 1e9, 1),
+                    "peak": round(peak / 1e9, 1), "unit": "G wave-instructions/s", "frac": round(got / peak, 4),
+                    "valu_insts_per_wave": round(entry["valu_insts_per_wave"], 1),
+                    "waves_per_launch": entry["waves_per_launch"],
+                    "peak_source": "v_fma_f32 issue rate measured live (spkx_valu_roofs)"}
     log(f"[bench] {ms:.3f} ms/frame, {mpxs:.1f} Mpx/s; stages (ms): " +
         ", ".join(f"{k} {v:.3f}" for k, v in per_stage.items()))
 
@@ -221,6 +236,10 @@ def main():
         "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "launch_ms": round(dom_ms, 4), "bytes_per_px": STAGE_BYTES[dom]},
+        "valu_roofline": valu,
+        "valu_roofs": {k: {"per_cu_per_clock": round(v["per_cu_per_clock"], 3),
+                           "g_wave_instructions_per_s": round(v["wave_instructions_per_s"] / 1e9, 1)}
+                       for k, v in roofs["rates"].items()},
         "spk_render_host_result": {"ms_per_frame": round(abi_ms, 3), "mpx_s": round(px / (abi_ms / 1000.0) / 1e6, 1),
                                    "note": "PCIe-inclusive: RGBA16 copied into pinned host memory; not the value"},
         "cpu_baseline": cpu,
