@@ -345,7 +345,8 @@ void Diff::backward(int slot, const float* input, const float* g_enc, float* g_i
     // develop
     float* gH = gB;
     float* gAcc = n_bounce_ ? gR : nullptr;  // g_R is no longer needed
-    const k::DevelopArgs da = develop_args(th);
+    k::DevelopArgs da = develop_args(th);
+    chain::set_hints(da, t);
     k::adj_develop_bwd(H, acc, gpu::as<float>(sl.counts), gA, gH, gAcc, gpu::as<float>(t.curves),
                        gpu::as<float>(t.gaxis), gpu::as<float>(t.ginv), gpu::as<float>(t.glayers), da, n,
                        part + rg.dev, s);

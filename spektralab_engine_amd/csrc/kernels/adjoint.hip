@@ -15,6 +15,7 @@
 
 #include <algorithm>
 
+#include "axis.h"
 #include "kernels.h"
 #include "print_math.h"
 
@@ -208,7 +209,8 @@ __global__ __launch_bounds__(256) void k_adj_print_bwd(const float* __restrict__
 constexpr uint32_t kWtPx = 256;
 __global__ __launch_bounds__(256) void k_adj_wt(const float* __restrict__ neg, const float* __restrict__ g_R,
                                                 EnlargerArgs ea, uint32_t n, double* __restrict__ part) {
-    __shared__ float sd[3][kWtPx], sg[3][kWtPx];
+    // pixel t's densities and g_R: (d0, d1, d2, g0), (g1, g2, -, -), two broadcast reads per pixel
+    __shared__ float4 sp[kWtPx][2];
     __shared__ double red[3][SPK_NL3];
     const uint32_t tid = threadIdx.x;
     CEA& e = cref(ea);
@@ -222,20 +224,27 @@ __global__ __launch_bounds__(256) void k_adj_wt(const float* __restrict__ neg, c
         __syncthreads();
         for (uint32_t t = tid; t < kWtPx; t += blockDim.x) {
             const uint32_t p = p0 + t;
-            for (int c = 0; c < 3; ++c) {
-                sd[c][t] = p < n ? neg[size_t(c) * n + p] : 0.0f;
-                sg[c][t] = p < n ? g_R[size_t(c) * n + p] : 0.0f;
+            float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            if (p < n) {
+                for (int c = 0; c < 3; ++c) {
+                    v[c] = neg[size_t(c) * n + p];
+                    v[3 + c] = g_R[size_t(c) * n + p];
+                }
             }
+            sp[t][0] = make_float4(v[0], v[1], v[2], v[3]);
+            sp[t][1] = make_float4(v[4], v[5], 0.0f, 0.0f);
         }
         __syncthreads();
         if (act) {
             float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
             const uint32_t cnt = min(kWtPx, n - p0);
+#pragma unroll 4
             for (uint32_t t = ph; t < cnt; t += 3) {
-                const float T = fexp2(((sd[0][t] * e0 + sd[1][t] * e1) + sd[2][t] * e2) + bb);
-                f0 += sg[0][t] * T;
-                f1 += sg[1][t] * T;
-                f2 += sg[2][t] * T;
+                const float4 a = sp[t][0], b = sp[t][1];
+                const float T = fexp2(((a.x * e0 + a.y * e1) + a.z * e2) + bb);
+                f0 += a.w * T;
+                f1 += b.x * T;
+                f2 += b.y * T;
             }
             a0 += f0;
             a1 += f1;
@@ -254,16 +263,11 @@ __global__ __launch_bounds__(256) void k_adj_wt(const float* __restrict__ neg, c
 
 // ---- develop (frozen counts): exposure + halation accumulator -> grain sum, transposed ---------
 __device__ inline float curve_at(const float* __restrict__ x, const float* __restrict__ inv, const float* __restrict__ y,
-                                 uint32_t K, float v, float* slope) {
+                                 uint32_t K, float v, float* slope, const uint16_t* hint, float hlo, float hscale) {
     *slope = 0.0f;
     if (v <= x[0]) return y[0];
     if (v >= x[K - 1]) return y[K - 1];
-    uint32_t lo = 0, hi = K;
-    while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (x[mid] <= v) lo = mid + 1;
-        else hi = mid;
-    }
+    const uint32_t lo = axis_upper(x, 1, K, v, hint, hlo, hscale);
     const uint32_t l = lo == 0 ? 0 : lo - 1;
     *slope = (y[l + 1] - y[l]) * inv[l];
     return y[l] + ((v - x[l]) * inv[l]) * (y[l + 1] - y[l]);
@@ -290,7 +294,8 @@ __global__ __launch_bounds__(256) void k_adj_develop_bwd(const float* __restrict
             const float L = flog10(rp);
             const float arg = q.fg[c] * L + q.fo[c];
             float sD;
-            const float D = curve_at(curves, curves + K, curves + 2 * K + c * K, K, arg, &sD);
+            const float D = curve_at(curves, curves + K, curves + 2 * K + c * K, K, arg, &sD, q.hints, q.hint_lo[0],
+                                     q.hint_scale[0]);
             float dGdD = 1.0f;  // grain off: the negative is D
             if (q.grain_active) {
                 const float vv = q.positive ? -D : D;
@@ -302,12 +307,8 @@ __global__ __launch_bounds__(256) void k_adj_develop_bwd(const float* __restrict
                 } else if (vv >= gaxis[3 * (GK - 1) + c]) {
                     for (uint32_t l = 0; l < 3; ++l) dv[l] = glayers[((GK - 1) * 3 + l) * 3 + c];
                 } else {
-                    uint32_t lo = 0, hi = GK;
-                    while (lo < hi) {
-                        const uint32_t mid = lo + (hi - lo) / 2;
-                        if (gaxis[3 * mid + c] <= vv) lo = mid + 1;
-                        else hi = mid;
-                    }
+                    const uint16_t* gh = q.hints ? q.hints + (1 + c) * kHintBins : nullptr;
+                    const uint32_t lo = axis_upper(gaxis + c, 3, GK, vv, gh, q.hint_lo[1 + c], q.hint_scale[1 + c]);
                     const uint32_t l0 = lo > 0 ? lo - 1 : 0;
                     const float t = (vv - gaxis[3 * l0 + c]) * ginv[3 * l0 + c];
                     for (uint32_t l = 0; l < 3; ++l) {
