@@ -10,7 +10,9 @@ in HBM and the output written to HBM (spkx_render_device); K steps are timed bet
 synchronisations.
 
 The JSON line reports Mpx/s, the live HIP-event duration and algorithmic-byte roofline of the
-dominant kernel, and a CPU baseline: the oracle (oracle/chain.cpp, binary32) timed on a bounded
+dominant kernel, the same kernel against the vector-ALU issue rate that binds it ("valu_roofline":
+its VALU instructions per wavefront from the hardware counters in profiles/, the FMA issue rate
+measured live by spkx_valu_roofs), and a CPU baseline: the oracle (oracle/chain.cpp, binary32) timed on a bounded
 sample (a 1536 x 1024 crop of the same workload) on this host's cores.  It also carries the
 batched workload (configuration 3, "batched": 64 frames of mixed sizes and per-frame parameters,
 one hipGraph launch, against 64 sequential spk_render and spkx_render_device calls).
