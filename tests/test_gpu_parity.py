@@ -186,3 +186,15 @@ def test_errors_map_to_statuses(engine):
         assert e.value.status == -2 and "extended_dynamic_range" in e.value.message
         out, _ = s.render("full")  # the session still renders with its previous parameters
         assert out.shape == (48, 64, 4)
+
+
+def test_valu_roofs_report_every_kind(engine):
+    """spkx_valu_roofs times each probe kind on the device: every rate is positive and finite, and
+    the per-CU-per-clock figures stay within what four SIMDs can issue (at most 1 wave64
+    instruction per CU per clock, with headroom for clocks above the reported one)."""
+    r = engine.valu_roofs()
+    assert r["compute_units"] > 0 and r["clock_mhz"] > 0 and r["instructions_per_wave"] > 0
+    for kind in ("v_fma_f32", "v_exp_f32", "v_fma_f64", "v_mad_u64_u32"):
+        k = r["rates"][kind]
+        assert k["ms"] > 0.0 and np.isfinite(k["wave_instructions_per_s"])
+        assert 0.0 < k["per_cu_per_clock"] < 1.5
