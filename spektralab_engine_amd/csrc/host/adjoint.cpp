@@ -376,6 +376,17 @@ void Diff::backward(int slot, const float* input, const float* g_enc, float* g_i
             };
             blur(gAcc, tp, 0.0, true);
             k::adj_bounce(tp, H, gH, wg, n, part + rg.bw[kk], s);
+            // a FIR-tile component (one sigma for the three channels): both blurs and the dot
+            // product in one pass over tiles; otherwise the two blurs, then the dot product
+            const int rr = setup::fir_radius(sg[0]);
+            const bool tile_fd = sg[0] == sg[1] && sg[1] == sg[2] && sg[0] > 0.0 && sg[0] < p.fir_crossover &&
+                                 rr >= 1 && rr <= k::kFir2dMaxR && k::fir2d_tiles(w, h) <= k::adj_blocks(n);
+            if (tile_fd) {
+                const k::FirArgs ap = chain::fir_args_at(sg[0] * (1.0 + kSigmaStep), sg[0], w, h);
+                const k::FirArgs am = chain::fir_args_at(sg[0] * (1.0 - kSigmaStep), sg[0], w, h);
+                k::adj_fir2d_dot(H, gAcc, ap, am, sc, 3, 0, k::adj_blocks(n), part + rg.bs[kk], s);
+                continue;
+            }
             blur(H, fp, kSigmaStep, false);
             blur(H, fm, -kSigmaStep, false);
             k::adj_dot_diff(gAcc, fp, fm, sc, n, part + rg.bs[kk], s);

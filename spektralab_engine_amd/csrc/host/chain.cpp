@@ -238,6 +238,29 @@ void blur1(const float* in, float* out, double sigma, double crossover, uint32_t
 
 }  // namespace
 
+k::FirArgs fir_args_at(double sigma, double branch_sigma, uint32_t w, uint32_t h) {
+    const int r = setup::fir_radius(branch_sigma);
+    if (r > k::kFirMaxR) fail_internal("FIR radius above the kernel maximum");
+    k::FirArgs a{};
+    a.w = w;
+    a.h = h;
+    a.r = uint32_t(r);
+    if (sigma == branch_sigma) {
+        const std::vector<float> taps = setup::fir_taps(sigma);
+        for (size_t i = 0; i < taps.size(); ++i) a.taps[i] = taps[i];
+    } else {  // the taps at sigma over the branch's radius
+        std::vector<double> d(size_t(2 * r + 1));
+        double sum = 0.0;
+        for (int t = -r; t <= r; ++t) {
+            const double u = double(t) / sigma;
+            d[size_t(t + r)] = std::exp(-0.5 * u * u);
+            sum += d[size_t(t + r)];
+        }
+        for (size_t i = 0; i < d.size(); ++i) a.taps[i] = num::f32(d[i] / sum);
+    }
+    return a;
+}
+
 void blur_planes(const float* in, float* out, uint32_t nplanes, double sigma, double branch_sigma, double crossover,
                  bool adjoint, uint32_t w, uint32_t h, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep) {
     const size_t np = size_t(w) * h;
@@ -250,24 +273,7 @@ void blur_planes(const float* in, float* out, uint32_t nplanes, double sigma, do
         // The FIR pass with half-sample reflected edges and symmetric taps is a symmetric operator
         // (entry (i, j) sums w_t over t = j - i and t = -1 - i - j, mod 2n, and w_t = w_-t), so the
         // adjoint runs the same passes.
-        if (r > k::kFirMaxR) fail_internal("FIR radius above the kernel maximum");
-        k::FirArgs a{};
-        a.w = w;
-        a.h = h;
-        a.r = uint32_t(r);
-        if (sigma == branch_sigma) {
-            const std::vector<float> taps = setup::fir_taps(sigma);
-            for (size_t i = 0; i < taps.size(); ++i) a.taps[i] = taps[i];
-        } else {  // the taps at sigma over the branch's radius
-            std::vector<double> d(size_t(2 * r + 1));
-            double sum = 0.0;
-            for (int t = -r; t <= r; ++t) {
-                const double u = double(t) / sigma;
-                d[size_t(t + r)] = std::exp(-0.5 * u * u);
-                sum += d[size_t(t + r)];
-            }
-            for (size_t i = 0; i < d.size(); ++i) a.taps[i] = num::f32(d[i] / sum);
-        }
+        const k::FirArgs a = fir_args_at(sigma, branch_sigma, w, h);
         if (r <= k::kFir2dMaxR && in != out) {
             k::fir2d(in, out, nullptr, a, nplanes, s);
             return;

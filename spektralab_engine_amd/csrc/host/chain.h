@@ -101,6 +101,10 @@ k::EnlargerArgs enlarger_args(const spk_plan& p);
 void blur3(const float* in, float* out, const double sigma[3], uint32_t mode, const float* acc, const double weight[3],
            uint32_t w, uint32_t h, double crossover, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep);
 
+// FIR arguments of a w x h blur with the taps at sigma over the radius of branch_sigma (sigma
+// below the crossover; the radius at most k::kFirMaxR).
+k::FirArgs fir_args_at(double sigma, double branch_sigma, uint32_t w, uint32_t h);
+
 // The Gaussian of nplanes contiguous w x h planes with the branch (identity, FIR of its radius, or
 // recursive) that branch_sigma takes, evaluated at sigma (the FIR taps at sigma over branch_sigma's
 // radius; the recursion's coefficients at sigma); adjoint: the transposed filter.  In place only

@@ -16,8 +16,10 @@
 #include <algorithm>
 
 #include "axis.h"
+#include "fir_tile.h"
 #include "kernels.h"
 #include "print_math.h"
+#include "xcd.h"
 
 namespace spk::k {
 namespace {
@@ -479,6 +481,71 @@ __global__ __launch_bounds__(256) void k_adj_dot_diff(const float* __restrict__ 
     block_sums<3>(q, part);
 }
 
+// The sigma gradient of one FIR-tile mixture component by central differences, fused: both blurs
+// (taps at sigma (1 + step) and sigma (1 - step) over sigma's radius) from one staged input tile,
+// each evaluated by the same float operations as k_fir2d, then sum over the tile of
+// s_c g (G+ H - G- H).  Tile b of plane p (channel c0 + p) writes part[b * 3 + c]; rows b + tiles,
+// b + 2 tiles, ... below nrows of that column are zeroed (nrows >= tiles).
+constexpr int kFdTX = 64, kFdTY = 32, kFdNT = 256;
+
+template <int R>
+__global__ __launch_bounds__(kFdNT) void k_adj_fir2d_dot(const float* __restrict__ in, const float* __restrict__ g,
+                                                         FirArgs qp, FirArgs qm, float s0, float s1, float s2,
+                                                         uint32_t nplanes, uint32_t c0, uint32_t nrows,
+                                                         double* __restrict__ part) {
+    using T = FirTile<R, kFdTX, kFdTY>;
+    __shared__ T t;
+    __shared__ float vm[kFdTY * T::kS];  // the minus blur's vertical pass
+    __shared__ double red[kFdNT / 64];
+    const int w = int(qp.w), h = int(qp.h);
+    const uint32_t gx = (qp.w + kFdTX - 1) / kFdTX, gy = (qp.h + kFdTY - 1) / kFdTY;
+    const uint32_t tiles = gx * gy, total = tiles * nplanes;
+    const uint32_t item = xcd_item(blockIdx.x, total);
+    if (item >= total) return;
+    const uint32_t tile = item % tiles, plane = item / tiles;
+    const uint32_t bx = tile % gx, by = tile / gx;
+    const uint32_t c = c0 + plane;
+    const size_t poff = size_t(plane) * qp.w * qp.h;
+    const int x0 = int(bx) * kFdTX, y0 = int(by) * kFdTY;
+    fir_tile_load<R, kFdTX, kFdTY, kFdNT>(t, in + poff, w, h, x0, y0);
+    __syncthreads();
+    fir_tile_v<R, kFdTX, kFdTY, kFdNT>(t, qp.taps);
+    for (int e = int(threadIdx.x); e < kFdTY * T::kW; e += kFdNT) {
+        const int row = e / T::kW, col = e - row * T::kW;
+        float a = 0.0f;
+#pragma unroll
+        for (int i = 0; i <= 2 * R; ++i) a = fmaf(qm.taps[i], t.in[(row + i) * T::kS + col], a);
+        vm[row * T::kS + col] = a;
+    }
+    __syncthreads();
+    const float sc = c == 0 ? s0 : (c == 1 ? s1 : s2);
+    float acc = 0.0f;
+    constexpr int PER = kFdTX * kFdTY / kFdNT;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = int(threadIdx.x) + u * kFdNT;
+        const int row = e / kFdTX, col = e - row * kFdTX;
+        const int x = x0 + col, y = y0 + row;
+        if (x < w && y < h) {
+            const float fp = fir_tile_h<R, kFdTX, kFdTY>(t, qp.taps, row, col);
+            float fm = 0.0f;
+#pragma unroll
+            for (int i = 0; i <= 2 * R; ++i) fm = fmaf(qm.taps[i], vm[row * T::kS + col + i], fm);
+            acc += sc * (g[poff + size_t(y) * qp.w + x] * (fp - fm));
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = double(acc);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int k = 0; k < kFdNT / 64; ++k) sum += red[k];
+        part[size_t(tile) * 3 + c] = sum;
+        for (uint32_t r = tile + tiles; r < nrows; r += tiles) part[size_t(r) * 3 + c] = 0.0;
+    }
+}
+
 // gq[q] := sum over b < terms[q].nblocks of part[terms[q].offset + b * terms[q].stride] (0 when
 // nblocks is 0); one workgroup per quantity, a fixed summation order.
 __global__ __launch_bounds__(256) void k_adj_reduce(const AdjTerm* __restrict__ terms, const double* __restrict__ part,
@@ -551,6 +618,30 @@ void adj_dot_diff(const float* g, const float* fp, const float* fm, const float 
                   hipStream_t s) {
     hipLaunchKernelGGL(k_adj_dot_diff, dim3(blocks_of(n)), dim3(kAdjBlock), 0, s, g, fp, fm, scale[0], scale[1],
                        scale[2], n, part);
+}
+
+uint32_t fir2d_tiles(uint32_t w, uint32_t h) { return ((w + kFdTX - 1) / kFdTX) * ((h + kFdTY - 1) / kFdTY); }
+
+void adj_fir2d_dot(const float* in, const float* g, const FirArgs& plus, const FirArgs& minus, const float scale[3],
+                   uint32_t nplanes, uint32_t c0, uint32_t nrows, double* part, hipStream_t s) {
+    const dim3 grid(xcd_grid(fir2d_tiles(plus.w, plus.h) * nplanes)), block(kFdNT);
+    switch (plus.r) {
+#define SPK_FD_CASE(RR)                                                                                          \
+    case RR:                                                                                                     \
+        hipLaunchKernelGGL(k_adj_fir2d_dot<RR>, grid, block, 0, s, in, g, plus, minus, scale[0], scale[1],        \
+                           scale[2], nplanes, c0, nrows, part);                                                  \
+        break;
+        SPK_FD_CASE(1)
+        SPK_FD_CASE(2)
+        SPK_FD_CASE(3)
+        SPK_FD_CASE(4)
+        SPK_FD_CASE(5)
+        SPK_FD_CASE(6)
+        SPK_FD_CASE(7)
+        SPK_FD_CASE(8)
+#undef SPK_FD_CASE
+        default: break;
+    }
 }
 
 void adj_reduce(const AdjTerm* terms, uint32_t nq, const double* part, double* gq, hipStream_t s) {

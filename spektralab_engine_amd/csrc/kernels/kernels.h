@@ -270,6 +270,12 @@ struct AdjTerm {
 };
 uint32_t adj_blocks(uint32_t n);
 uint32_t adj_wt_blocks(uint32_t n);
+// The sigma gradient of a FIR-tile component (radius 1..kFir2dMaxR) by central differences, fused:
+// part[b * 3 + c] := sum over tile b of plane p (channel c = c0 + p) of
+// scale_c g (G_plus in - G_minus in), for b < fir2d_tiles(w, h); rows up to nrows zeroed.
+uint32_t fir2d_tiles(uint32_t w, uint32_t h);
+void adj_fir2d_dot(const float* in, const float* g, const FirArgs& plus, const FirArgs& minus, const float scale[3],
+                   uint32_t nplanes, uint32_t c0, uint32_t nrows, double* part, hipStream_t s);
 
 // enc (interleaved) := transfer(m_nid (ua lin + ub blur)) (blur null: lin), pre-quantisation.
 void adj_encode_fwd(const float* lin, const float* blur, float* enc, const PrintConsts* pc, float ua, float ub,
