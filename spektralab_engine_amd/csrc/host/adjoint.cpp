@@ -387,6 +387,18 @@ void Diff::backward(int slot, const float* input, const float* g_enc, float* g_i
                 k::adj_fir2d_dot(H, gAcc, ap, am, sc, 3, 0, k::adj_blocks(n), part + rg.bs[kk], s);
                 continue;
             }
+            // a separable FIR component: the vertical passes of both blurs over one read of H, then
+            // both horizontal passes reduced against g_acc without storing them
+            const bool sep_fd = sg[0] == sg[1] && sg[1] == sg[2] && sg[0] > 0.0 && sg[0] < p.fir_crossover &&
+                                rr > k::kFir2dMaxR && rr <= k::kFirMaxR &&
+                                k::fir_h_dot_blocks(w, h, 3) <= k::adj_blocks(n);
+            if (sep_fd) {
+                const k::FirArgs ap = chain::fir_args_at(sg[0] * (1.0 + kSigmaStep), sg[0], w, h);
+                const k::FirArgs am = chain::fir_args_at(sg[0] * (1.0 - kSigmaStep), sg[0], w, h);
+                k::fir_v2(H, fp, fm, ap, am, 3, s);
+                k::fir_h_dot(fp, fm, gAcc, ap, am, sc, 3, k::adj_blocks(n), part + rg.bs[kk], s);
+                continue;
+            }
             blur(H, fp, kSigmaStep, false);
             blur(H, fm, -kSigmaStep, false);
             k::adj_dot_diff(gAcc, fp, fm, sc, n, part + rg.bs[kk], s);

@@ -171,6 +171,158 @@ __global__ __launch_bounds__(256) void k_fir_h(const float* __restrict__ in, flo
     }
 }
 
+// The sigma gradient's central difference for a separable FIR component, fused: k_fir_v2 runs
+// the vertical pass of k_fir_v with two tap sets (sigma (1 +- step)) over one read of the input;
+// k_fir_h_dot runs k_fir_h's horizontal pass on both results and, instead of storing them, sums
+// s_c g (plus - minus) per workgroup and channel.  The float operations per output are those of
+// k_fir_v and k_fir_h.
+__global__ __launch_bounds__(256) void k_fir_v2(const float* __restrict__ in, float* __restrict__ outp,
+                                                float* __restrict__ outm, FirArgs qp, FirArgs qm, uint32_t nplanes) {
+    const int w = int(qp.w), h = int(qp.h), R = int(qp.r);
+    const uint32_t gx = (qp.w + 63) / 64, gy = (qp.h + 4 * kT - 1) / (4 * kT);
+    const uint32_t total = gx * gy * nplanes;
+    const uint32_t item = xcd_item(blockIdx.x, total);
+    if (item >= total) return;
+    const uint32_t by = item % gy, rest = item / gy, bx = rest % gx, plane = rest / gx;
+    const size_t poff = size_t(plane) * qp.w * qp.h;
+    in += poff;
+    const int x = int(bx * 64 + threadIdx.x);
+    const int y0 = int((by * 4 + threadIdx.y) * kT);
+    if (y0 >= h) return;
+    const int xc = x < w ? x : w - 1;
+    const float* col = in + xc;
+    auto load = [&](int t) {
+        int yy = y0 - R + t;
+        if (yy < 0 || yy >= h) yy = reflect_index(yy, h);
+        return col[size_t(yy) * w];
+    };
+    const int n = 2 * R + 1;
+    float ap[kT], am[kT], win[2 * kT];
+#pragma unroll
+    for (int j = 0; j < kT; ++j) {
+        ap[j] = 0.0f;
+        am[j] = 0.0f;
+        win[j] = load(j);
+    }
+    for (int i0 = 0; i0 < n; i0 += kT) {
+#pragma unroll
+        for (int k = 0; k < kT; ++k) win[kT + k] = load(i0 + kT + k);
+#pragma unroll
+        for (int ii = 0; ii < kT; ++ii) {
+            if (i0 + ii < n) {
+                const float tp = qp.taps[i0 + ii], tm = qm.taps[i0 + ii];
+#pragma unroll
+                for (int j = 0; j < kT; ++j) {
+                    ap[j] = fmaf(tp, win[ii + j], ap[j]);
+                    am[j] = fmaf(tm, win[ii + j], am[j]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kT; ++k) win[k] = win[kT + k];
+    }
+    if (x >= w) return;
+#pragma unroll
+    for (int j = 0; j < kT; ++j) {
+        if (y0 + j < h) {
+            outp[poff + size_t(y0 + j) * w + x] = ap[j];
+            outm[poff + size_t(y0 + j) * w + x] = am[j];
+        }
+    }
+}
+
+// The horizontal pass of one staged row segment (k_fir_h's window), into acc.
+__device__ inline void fir_h_row(const float* tile, const FirArgs& q, int base, float (&acc)[kT]) {
+    const int n = 2 * int(q.r) + 1;
+    float win[2 * kT];
+#pragma unroll
+    for (int j = 0; j < kT; ++j) {
+        acc[j] = 0.0f;
+        win[j] = tile[hpad(base + j)];
+    }
+    for (int i0 = 0; i0 < n; i0 += kT) {
+#pragma unroll
+        for (int k = 0; k < kT; ++k) win[kT + k] = tile[hpad(base + i0 + kT + k)];
+#pragma unroll
+        for (int ii = 0; ii < kT; ++ii) {
+            if (i0 + ii < n) {
+                const float tap = q.taps[i0 + ii];
+#pragma unroll
+                for (int j = 0; j < kT; ++j) acc[j] = fmaf(tap, win[ii + j], acc[j]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kT; ++k) win[k] = win[kT + k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fir_h_dot(const float* __restrict__ inp, const float* __restrict__ inm,
+                                                   const float* __restrict__ g, FirArgs qp, FirArgs qm, float s0,
+                                                   float s1, float s2, uint32_t nplanes, uint32_t nrows,
+                                                   double* __restrict__ part) {
+    extern __shared__ float lds_hd[];
+    __shared__ float red[3][4];
+    const int w = int(qp.w), R = int(qp.r);
+    const int hp = int(qp.h);                 // rows of one plane
+    const int h = int(qp.h * nplanes);       // rows of all planes, contiguous
+    const int y = int(blockIdx.y * 4 + threadIdx.y);
+    const int x0 = int(blockIdx.x) * kHSeg;
+    const int span = kHSeg + 2 * R + kT;
+    const int stride = hpad(span) + 1;
+    float* tp = lds_hd + threadIdx.y * 2 * stride;
+    float* tm = tp + stride;
+    const int lane = int(threadIdx.x);
+    const bool rowok = y < h;
+    if (rowok) {
+        const float* rp = inp + size_t(y) * w;
+        const float* rm = inm + size_t(y) * w;
+        for (int t = lane; t < span; t += 64) {
+            int xx = x0 - R + t;
+            if (xx < 0 || xx >= w) xx = reflect_index(xx, w);
+            tp[hpad(t)] = rp[xx];
+            tm[hpad(t)] = rm[xx];
+        }
+    }
+    __syncthreads();
+    const int base = lane * kT;
+    float ap[kT], am[kT];
+    fir_h_row(tp, qp, base, ap);
+    fir_h_row(tm, qm, base, am);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kT; ++j) {
+        tp[hpad(base + j)] = ap[j];
+        tm[hpad(base + j)] = am[j];
+    }
+    __syncthreads();
+    float q[3] = {0.0f, 0.0f, 0.0f};
+    if (rowok) {
+        const int c = y / hp;
+        const float sc = c == 0 ? s0 : (c == 1 ? s1 : s2);
+        float sum = 0.0f;
+#pragma unroll
+        for (int u = 0; u < kT; ++u) {
+            const int t = lane + 64 * u, x = x0 + t;
+            if (x < w) sum += sc * (g[size_t(y) * w + x] * (tp[hpad(t)] - tm[hpad(t)]));
+        }
+        q[c] = sum;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float v = q[c];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[c][threadIdx.y] = v;
+    }
+    __syncthreads();
+    const uint32_t b = blockIdx.y * gridDim.x + blockIdx.x, nb = gridDim.x * gridDim.y;
+    const uint32_t tid = threadIdx.y * 64 + threadIdx.x;
+    if (tid < 3) {
+        part[size_t(b) * 3 + tid] = ((double(red[tid][0]) + double(red[tid][1])) + double(red[tid][2])) + double(red[tid][3]);
+        for (uint32_t r = b + nb; r < nrows; r += nb) part[size_t(r) * 3 + tid] = 0.0;
+    }
+}
+
 // Small radii: both passes in one kernel on 64 x 32 output tiles (fir_tile.h); nplanes contiguous
 // planes; tiles walked in XCD order so that neighbours share their halo rows in one L2.
 constexpr int kF2TX = 64, kF2TY = 32, kF2NT = 256;
@@ -523,6 +675,25 @@ __global__ __launch_bounds__(256) void k_lincomb(const float* __restrict__ x, co
 void fir_v(const float* in, float* out, const FirArgs& a, uint32_t nplanes, hipStream_t s) {
     const uint32_t gx = (a.w + 63) / 64, gy = (a.h + 4 * kT - 1) / (4 * kT);
     hipLaunchKernelGGL(k_fir_v, dim3(xcd_grid(gx * gy * nplanes)), dim3(64, 4), 0, s, in, out, a, nplanes);
+}
+
+uint32_t fir_h_dot_blocks(uint32_t w, uint32_t h, uint32_t nplanes) {
+    return ((w + kHSeg - 1) / kHSeg) * ((h * nplanes + 3) / 4);
+}
+
+void fir_v2(const float* in, float* outp, float* outm, const FirArgs& plus, const FirArgs& minus, uint32_t nplanes,
+            hipStream_t s) {
+    const uint32_t gx = (plus.w + 63) / 64, gy = (plus.h + 4 * kT - 1) / (4 * kT);
+    hipLaunchKernelGGL(k_fir_v2, dim3(xcd_grid(gx * gy * nplanes)), dim3(64, 4), 0, s, in, outp, outm, plus, minus,
+                       nplanes);
+}
+
+void fir_h_dot(const float* inp, const float* inm, const float* g, const FirArgs& plus, const FirArgs& minus,
+               const float scale[3], uint32_t nplanes, uint32_t nrows, double* part, hipStream_t s) {
+    const int span = kHSeg + 2 * int(plus.r) + kT;
+    const size_t lds = 4 * 2 * size_t(hpad(span) + 1) * sizeof(float);
+    hipLaunchKernelGGL(k_fir_h_dot, dim3((plus.w + kHSeg - 1) / kHSeg, (plus.h * nplanes + 3) / 4), dim3(64, 4), lds,
+                       s, inp, inm, g, plus, minus, scale[0], scale[1], scale[2], nplanes, nrows, part);
 }
 
 void fir_h(const float* in, float* out, const float* acc, const FirArgs& a, uint32_t nplanes, hipStream_t s) {

@@ -274,6 +274,14 @@ uint32_t adj_wt_blocks(uint32_t n);
 // part[b * 3 + c] := sum over tile b of plane p (channel c = c0 + p) of
 // scale_c g (G_plus in - G_minus in), for b < fir2d_tiles(w, h); rows up to nrows zeroed.
 uint32_t fir2d_tiles(uint32_t w, uint32_t h);
+// The same for a separable FIR component (blur.hip): fir_v2 writes the vertical passes of both tap
+// sets; fir_h_dot runs both horizontal passes and writes part[b * 3 + c] for its
+// fir_h_dot_blocks(w, h, nplanes) workgroups (channel c = plane), zeroing rows up to nrows.
+uint32_t fir_h_dot_blocks(uint32_t w, uint32_t h, uint32_t nplanes);
+void fir_v2(const float* in, float* outp, float* outm, const FirArgs& plus, const FirArgs& minus, uint32_t nplanes,
+            hipStream_t s);
+void fir_h_dot(const float* inp, const float* inm, const float* g, const FirArgs& plus, const FirArgs& minus,
+               const float scale[3], uint32_t nplanes, uint32_t nrows, double* part, hipStream_t s);
 void adj_fir2d_dot(const float* in, const float* g, const FirArgs& plus, const FirArgs& minus, const float scale[3],
                    uint32_t nplanes, uint32_t c0, uint32_t nrows, double* part, hipStream_t s);
 
