@@ -35,6 +35,19 @@ SPK_API spk_status spkx_set_overrides(spk_session* session, const char* json);
  * The call returns after enqueueing; *out_w, *out_h receive the size.  No host copy is made. */
 SPK_API spk_status spkx_render_device(spk_session* session, const char* tier, int32_t reprint, void* dev_out,
                                       uint32_t row_stride_px, void* stream, uint32_t* out_w, uint32_t* out_h);
+/* spkx_render_device for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, hosts that forbid hipMalloc); no counterpart in the reference.  Every device
+ * buffer of the render (planes, blur temporaries, the plan's tables when the session has none on
+ * the device yet, a downscaled tier image, the auto-exposure meter image) is carved from one
+ * static scratch region of 512 MiB per GPU, which the library holds from its load to the end of
+ * the process; the call makes no hipMalloc/hipFree.  A render whose buffers do not fit (about
+ * 4 MP for the full chain with grain and a 3-component halation mixture) is refused with
+ * SPK_ERR_INVALID_ARG before anything is enqueued.  A negative developed by this call is not
+ * cached; reprint != 0 reuses a negative cached by an earlier spk_render / spkx_render_device.
+ * Calls on one GPU share the region: the library serialises them and each returns after `stream`
+ * has drained, so they do not race; work the caller enqueues elsewhere must not read the region. */
+SPK_API spk_status spkx_render_device_static(spk_session* session, const char* tier, int32_t reprint, void* dev_out,
+                                             uint32_t row_stride_px, void* stream, uint32_t* out_w, uint32_t* out_h);
 /* Waits for the session's stream. */
 SPK_API spk_status spkx_session_sync(spk_session* session);
 

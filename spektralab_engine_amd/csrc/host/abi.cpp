@@ -410,6 +410,17 @@ SPK_API spk_status spkx_render_device(spk_session* session, const char* tier, in
     });
 }
 
+SPK_API spk_status spkx_render_device_static(spk_session* session, const char* tier, int32_t reprint, void* dev_out,
+                                             uint32_t row_stride_px, void* stream, uint32_t* out_w, uint32_t* out_h) {
+    le::clear();
+    if (!session || !tier || !dev_out) return invalid("a required argument is NULL");
+    return guarded([&] {
+        S(session)->render_device_static(tier, reprint != 0, dev_out, row_stride_px, static_cast<hipStream_t>(stream),
+                                         out_w, out_h);
+        return SPK_OK;
+    });
+}
+
 SPK_API spk_status spkx_session_sync(spk_session* session) {
     le::clear();
     if (!session) return invalid("session is NULL");

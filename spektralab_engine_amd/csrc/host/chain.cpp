@@ -750,6 +750,43 @@ void develop_post(const spk_plan& p, const DevelopStage& st, float* negative, gp
     mark("grain_blur");
 }
 
+size_t scratch_bound(const spk_plan& p, bool develop, bool tables, bool negative) {
+    auto blk = [](size_t bytes) { return ((bytes ? bytes : 1) + 255) & ~size_t(255); };
+    const size_t np = size_t(p.width) * p.height;
+    const size_t plane = np * sizeof(float);
+    // blur3: one temporary of 1 or 3 planes, and one more plane when a recursive column pass has
+    // to go through a transpose
+    const size_t b3 = blk(3 * plane) + blk(plane);
+    size_t n = 0;
+    if (tables) {
+        const size_t gk = std::max<size_t>(p.grain_k, 1);
+        n += blk(size_t(p.L) * p.L * 3 * sizeof(float)) + blk(8 * size_t(p.K) * sizeof(float));
+        n += 2 * blk(3 * gk * sizeof(float)) + blk(9 * gk * sizeof(float));
+        n += blk(4 * size_t(k::kHintBins) * sizeof(uint16_t)) + blk(spkm::kLgammaTable * sizeof(float));
+        n += blk(5 * size_t(std::max<uint32_t>(p.paper_k, 1)) * sizeof(float)) + blk(SPK_CMAX_N * sizeof(float));
+        n += blk(sizeof(k::PrintConsts)) + blk(sizeof(k::SpecMfma));
+    }
+    if (negative) n += blk(3 * plane);
+    if (develop) {
+        n += blk(3 * plane);  // H
+        if (p.geometry_active) n += blk(2 * (size_t(p.width) + p.height) * sizeof(double)) + blk(3 * plane);
+        if (p.boost_active) n += blk((k::kMaxBlocks + 1) * sizeof(float));
+        if (p.headroom_active) n += blk((k::kMaxBlocks + 1) * sizeof(float));
+        if (p.lens_blur_px > 0.0) n += b3;
+        if (p.scatter_active || p.bounce_active) n += blk(3 * plane);
+        if (p.scatter_active) n += p.n_scatter * b3;
+        if (p.bounce_active) n += p.n_bounce * b3;
+        if (p.couplers_active) n += 3 * blk(3 * plane) + p.n_dir * b3;
+        if (p.grain_active) n += 2 * blk(3 * plane) + b3;  // G, the pre-grain density, the final blur
+    }
+    if (p.glare_active) n += 2 * blk(plane) + 2 * blk(plane);
+    n += blk(3 * plane);  // the scanned RGB
+    if (p.spectral_fp16) n += blk(sizeof(k::SpecMfma));
+    if (p.scanner_lens_blur_px > 0.0) n += b3;
+    n += blk(3 * plane) + b3;  // the unsharp blur and its temporaries
+    return n;
+}
+
 void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
              std::vector<gpu::DevBuf>& keep, Profiler* prof) {
     DevelopStage st;

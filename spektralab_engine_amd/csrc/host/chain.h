@@ -53,6 +53,12 @@ void set_hints(k::DevelopArgs& a, const Tables& t);
 // returns false, leaving lo and scale 0, for an axis that is not non-decreasing or too short.
 bool axis_hints(const float* a, size_t st, size_t n, uint16_t* hint, float& lo, float& scale);
 
+// An upper bound on the device bytes develop() (develop: true) and print() acquire from their pool
+// for plan p, each acquisition rounded up to the pool's 256-byte granule, plus upload_tables()'s
+// when tables is true and one negative when negative is true.  A pool that never reuses a block
+// within a render (an arena) needs no more than this.
+size_t scratch_bound(const spk_plan& p, bool develop, bool tables, bool negative);
+
 // rgb: interleaved w x h x 3 on the device; negative: 3 planar planes (w*h*3 floats).
 void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
              std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr);

@@ -139,6 +139,13 @@ public:
     // The same into a caller device buffer on a caller stream.
     void render_device(const char* tier, bool reprint, void* dev_out, uint32_t stride, hipStream_t stream, uint32_t* w,
                        uint32_t* h);
+    // render_device with every device buffer of the render carved from the library's static
+    // scratch region of the engine's GPU (no device allocation).  A negative developed here is not
+    // cached (the region is reused by the next call); reprint reuses a negative cached by an
+    // earlier render_device / render.  Calls on one GPU are serialised and each returns after its
+    // stream has drained.
+    void render_device_static(const char* tier, bool reprint, void* dev_out, uint32_t stride, hipStream_t stream,
+                              uint32_t* w, uint32_t* h);
     void sync();
     void set_overrides(const char* json);
     size_t last_plan(void* out, size_t cap);
@@ -174,13 +181,20 @@ private:
     // The output (and negative) size of tier t: its tier image size through the geometry.
     tiers::Size out_size(int t) const;
     uint32_t tier_long_edge(int t) const;
-    const TierImage& tier_image(int t, hipStream_t s);
-    // auto_exposure: the EV of the configured method (meters once, session.cpp:296-319)
-    std::optional<double> session_ev(hipStream_t s);
+    // arena: build a missing downscaled image in this pool for the call only (not cached).
+    const TierImage& tier_image(int t, hipStream_t s, gpu::Pool* arena = nullptr, TierImage* scratch = nullptr);
+    // auto_exposure: the EV of the configured method (meters once, session.cpp:296-319); the meter
+    // image's buffers come from `pool` and are dead when this returns.
+    std::optional<double> session_ev(hipStream_t s, gpu::Pool& pool);
+    // The device bytes a render of tier t through an arena pool needs at most (the chain's buffers,
+    // the tables when the pipeline has none on the device, a missing downscaled tier image).
+    size_t static_need(int t, bool reprint);
     // Develops (unless reusing) and prints into `out`; returns the plan's reprint flag.
     // lut: the print-preview table to apply instead of the print (no print seed is drawn).
+    // arena: every device buffer of the render from this pool; the negative is then not cached and
+    // the plan's tables are uploaded into the arena when the pipeline has none on the device yet.
     bool run(int tier, bool reprint, uint16_t* out, uint32_t stride, hipStream_t s, std::vector<gpu::DevBuf>& keep,
-             const res::PrintLutTable* lut = nullptr);
+             const res::PrintLutTable* lut = nullptr, gpu::Pool* arena = nullptr);
     void node(const char* label);
 
     Engine* engine_ = nullptr;

@@ -19,6 +19,18 @@ Pool::~Pool() { trim(); }
 DevBuf Pool::acquire(size_t bytes) {
     if (bytes == 0) bytes = 256;
     bytes = (bytes + 255) & ~size_t(255);
+    if (arena_) {
+        std::lock_guard<std::mutex> g(m_);
+        if (bytes > arena_cap_ - arena_top_)
+            fail(SPK_ERR_INVALID_ARG, "the static scratch region of " + std::to_string(arena_cap_) +
+                                          " bytes is too small for this render (" + std::to_string(arena_top_ + bytes) +
+                                          " bytes needed so far)");
+        Block* b = new Block{arena_ + arena_top_, bytes, this};
+        arena_top_ += bytes;
+        live_ += bytes;
+        high_ = std::max(high_, arena_top_);
+        return DevBuf(b, release_block);
+    }
     {
         std::lock_guard<std::mutex> g(m_);
         auto it = idle_.lower_bound(bytes);
@@ -45,14 +57,31 @@ DevBuf Pool::acquire(size_t bytes) {
     return DevBuf(new Block{p, bytes, this}, release_block);
 }
 
+void Pool::reset() {
+    std::lock_guard<std::mutex> g(m_);
+    arena_top_ = 0;
+}
+
+size_t Pool::top() const {
+    std::lock_guard<std::mutex> g(m_);
+    return arena_top_;
+}
+
+void Pool::rewind(size_t offset) {
+    std::lock_guard<std::mutex> g(m_);
+    if (offset < arena_top_) arena_top_ = offset;
+}
+
 void Pool::give_back(Block* b) {
     std::lock_guard<std::mutex> g(m_);
     live_ -= b->bytes;
+    if (arena_) return;
     idle_bytes_ += b->bytes;
     idle_.emplace(b->bytes, b->ptr);
 }
 
 size_t Pool::trim() {
+    if (arena_) return 0;
     std::multimap<size_t, void*> drop;
     {
         std::lock_guard<std::mutex> g(m_);

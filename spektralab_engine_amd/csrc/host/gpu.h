@@ -33,8 +33,20 @@ using DevBuf = std::shared_ptr<Block>;
 class Pool : public std::enable_shared_from_this<Pool> {
 public:
     explicit Pool(int device) : device_(device) {}
+    // A pool over a fixed device region [base, base + bytes) that the caller owns: acquire carves
+    // 256-byte-aligned blocks off the region in order and never allocates; a released block is not
+    // handed out again until reset().  Asking for more than the region holds throws
+    // SPK_ERR_INVALID_ARG.
+    Pool(int device, void* base, size_t bytes) : device_(device), arena_(static_cast<char*>(base)), arena_cap_(bytes) {}
     ~Pool();
     DevBuf acquire(size_t bytes);
+    void reset();  // arena pools: every block free again (callers make sure none is still in use)
+    bool is_arena() const { return arena_ != nullptr; }
+    size_t capacity() const { return arena_cap_; }
+    // arena pools: the offset of the next block, and a return to an earlier offset (the blocks
+    // above it must be dead: released and drained by their stream)
+    size_t top() const;
+    void rewind(size_t offset);
     size_t trim();  // free every idle block; returns bytes freed
     size_t live_bytes() const;
     size_t idle_bytes() const;
@@ -48,6 +60,8 @@ private:
     mutable std::mutex m_;
     std::multimap<size_t, void*> idle_;
     size_t live_ = 0, idle_bytes_ = 0, high_ = 0;
+    char* arena_ = nullptr;
+    size_t arena_cap_ = 0, arena_top_ = 0;
 };
 
 template <class T>

@@ -51,6 +51,15 @@ AxisSamples axis_samples(uint32_t n, uint32_t m) {
     return s;
 }
 
+size_t downscale_bound(uint32_t w, uint32_t h, uint32_t long_edge) {
+    const Size o = downscale_size(w, h, long_edge);
+    if (o.w == w && o.h == h) return 0;
+    auto blk = [](size_t bytes) { return ((bytes ? bytes : 1) + 255) & ~size_t(255); };
+    const size_t ntab = axis_taps(h, o.h).size() + axis_taps(w, o.w).size() + size_t(3) * (size_t(o.w) + o.h);
+    const size_t px = size_t(w) * h;
+    return blk(ntab * 4) + 2 * blk(3 * px * sizeof(float)) + blk(3 * size_t(o.w) * o.h * sizeof(float));
+}
+
 gpu::DevBuf downscale(const float* src, uint32_t w, uint32_t h, uint32_t long_edge, gpu::Pool& pool, hipStream_t s,
                       Size* out) {
     const Size o = downscale_size(w, h, long_edge);

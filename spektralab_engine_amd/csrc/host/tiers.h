@@ -35,6 +35,8 @@ struct AxisSamples {
 };
 AxisSamples axis_samples(uint32_t n, uint32_t m);
 
+// The device bytes downscale() acquires (each block rounded up to 256 bytes); 0 when no downscale.
+size_t downscale_bound(uint32_t w, uint32_t h, uint32_t long_edge);
 // Downscales an interleaved w x h x 3 device image to downscale_size(w, h, long_edge) on stream s
 // (returns once enqueued; the result is valid after the stream reaches it).
 gpu::DevBuf downscale(const float* src, uint32_t w, uint32_t h, uint32_t long_edge, gpu::Pool& pool, hipStream_t s,

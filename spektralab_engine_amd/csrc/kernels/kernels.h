@@ -320,6 +320,14 @@ struct LutArgs {
 };
 void print_lut(const float* neg, const float* lut, uint16_t* out, uint32_t stride, const LutArgs& a, hipStream_t s);
 
+// The static scratch region (scratch.hip): kStaticScratchBytes of device memory per GPU, held from
+// the code object's load to the end of the process.  static_scratch_init(device) looks up the
+// region's address on that device once (hipGetSymbolAddress with the device current);
+// static_scratch(device) returns it afterwards (null before the look-up).
+inline constexpr size_t kStaticScratchBytes = size_t(512) << 20;
+hipError_t static_scratch_init(int device);
+void* static_scratch(int device);
+
 // Tier downscale passes on interleaved 3-channel images (session.hip).
 void ds_blur_v(const float* src, float* dst, const float* taps, uint32_t w, uint32_t h, int r, hipStream_t s);
 void ds_blur_h(const float* src, float* dst, const float* taps, uint32_t w, uint32_t h, int r, hipStream_t s);
