@@ -27,6 +27,7 @@ extern "C" {
  *   "spectral_precision": "fp32" | "fp16"            the print's two 81-bin spectral integrals as
  *                                                    FP32 sums, or FP16 matrix-core products with
  *                                                    FP32 accumulation (keeps the negatives)
+ *   "record_grain": bool                             keep each develop's grain draws (spkx_grain_record)
  * Returns SPK_OK | SPK_ERR_INVALID_ARG | SPK_ERR_USER. */
 SPK_API spk_status spkx_set_overrides(spk_session* session, const char* json);
 
@@ -61,6 +62,12 @@ SPK_API spk_status spkx_stage_times(spk_session* session, int32_t reset, char** 
 SPK_API size_t spkx_last_plan(spk_session* session, void* out, size_t cap);
 /* The cached negative of `tier` as 3 planar float32 planes (w*h*3 floats) into host memory. */
 SPK_API spk_status spkx_negative(spk_session* session, const char* tier, float* out, size_t count);
+/* The grain draws of the tier's last develop, kept when the override "record_grain" is true:
+ * lambdas[(3 c + l) w h + i] is the Poisson mean of sub-layer l, channel c at pixel i and counts the
+ * count drawn (9 planes each, count >= 9 w h floats; either output may be NULL).  For checkers of
+ * the integer Poisson counts. */
+SPK_API spk_status spkx_grain_record(spk_session* session, const char* tier, float* lambdas, float* counts,
+                                     size_t count);
 
 /* Host-only: the plan a session opened on `resources_dir` with `params_delta_json` (and the
  * overrides of spkx_set_overrides, NULL = none) would use for a w x h full-tier render whose film
@@ -79,6 +86,7 @@ SPK_API size_t spkx_plan(const char* resources_dir, const char* params_delta_jso
  * candidate graphs), "schedule": "phased" (one develop and one print launch for all frames
  * between per-frame phases; the default) | "streams" (each frame's whole chain on its stream;
  * with autotune off, and no session in FP16 spectral precision), "streams": 1..4,
+ * "record_grain": bool (keep each frame's grain draws, spkx_batch_grain_record),
  * "grid_per_cu": 1..64, "block": 64|128|256}.  Parameter changes made to the
  * sessions after creation are not seen by the batch.  Returns NULL with the reason in
  * spk_last_error on failure.
@@ -91,6 +99,10 @@ SPK_API spkx_batch* spkx_batch_create(spk_engine* engine, spk_session* const* se
 SPK_API spk_status spkx_batch_run(spkx_batch* batch, void* stream);
 SPK_API spk_status spkx_batch_sync(spkx_batch* batch);
 SPK_API spk_status spkx_batch_info(spkx_batch* batch, char** out_json);
+/* With the option "record_grain": true, the grain draws of frame `frame` in its last run, laid out
+ * as spkx_grain_record's (9 planes of Poisson means, 9 of counts). */
+SPK_API spk_status spkx_batch_grain_record(spkx_batch* batch, int32_t frame, float* lambdas, float* counts,
+                                           size_t count);
 SPK_API void spkx_batch_destroy(spkx_batch* batch);
 
 /* Differentiable render (no counterpart in the reference, which renders forward only).
@@ -135,6 +147,10 @@ SPK_API spk_status spkx_poisson(spk_engine* engine, const float* lam, uint32_t n
                                 float* out);
 SPK_API spk_status spkx_gaussian(spk_engine* engine, const float* in, float* out, uint32_t width, uint32_t height,
                                  const double* sigma3, double fir_crossover);
+/* The transpose of spkx_gaussian's operator, per channel (the reverse pass's blur of the
+ * differentiable render: the FIR pass, which is symmetric, or the transposed recursion). */
+SPK_API spk_status spkx_gaussian_adjoint(spk_engine* engine, const float* in, float* out, uint32_t width,
+                                         uint32_t height, const double* sigma3, double fir_crossover);
 /* The tier downscale (3-channel interleaved host image -> long edge `long_edge`, 0 = none): writes
  * *out_w, *out_h always and the image when out is not NULL and cap_floats allows. */
 SPK_API spk_status spkx_downscale(spk_engine* engine, const float* in, uint32_t width, uint32_t height,

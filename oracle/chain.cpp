@@ -212,6 +212,48 @@ void yvv_line(const T* in, T* out, size_t n, size_t stride, const Yvv& k) {
     }
 }
 
+// The transpose of yvv_line's linear map, derived from its structure (no counterpart in the
+// reference, which renders forward only).  The line is y = Bk F x with the causal pass
+// F = L^-1 (B I + E) (L unit lower-triangular Toeplitz with -b1, -b2, -b3 below the diagonal; E puts
+// the start w[-1..-3] = x[0] into rows 0..2 with weights b1 + b2 + b3, b2 + b3, b3 on column 0) and
+// the anticausal pass Bk = U^-1 (B I + D) (U = L^T; D puts y[n..n+2] = u[n-1] into rows n-1, n-2,
+// n-3 with the same weights on column n-1).  So y^T = F^T Bk^T: z = L^-1 v (a causal pass from a
+// zero start), Bk^T v = B z + e_{n-1} ((b1 + b2 + b3) z[n-1] + (b2 + b3) z[n-2] + b3 z[n-3]); then
+// t = U^-1 s (an anticausal pass from a zero start), F^T s = B t + e_0 ((b1 + b2 + b3) t[0] +
+// (b2 + b3) t[1] + b3 t[2]).  State float64, storage T between the passes (as yvv_line).
+template <class T>
+void yvv_line_t(const T* in, T* out, size_t n, size_t stride, const Yvv& k) {
+    std::vector<double> z(n);
+    double z1 = 0.0, z2 = 0.0, z3 = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const double t = ((double(in[i * stride]) + k.B1 * z1) + k.B2 * z2) + k.B3 * z3;
+        z3 = z2;
+        z2 = z1;
+        z1 = t;
+        z[i] = t;
+    }
+    std::vector<T> s(n);
+    for (size_t i = 0; i < n; ++i) s[i] = T(k.B * z[i]);
+    {
+        const double e1 = z[n - 1], e2 = n >= 2 ? z[n - 2] : 0.0, e3 = n >= 3 ? z[n - 3] : 0.0;
+        s[n - 1] = T(k.B * z[n - 1] + (((k.B1 + k.B2) + k.B3) * e1 + (k.B2 + k.B3) * e2 + k.B3 * e3));
+    }
+    std::vector<double> t(n);
+    double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    for (size_t i = n; i-- > 0;) {
+        const double v = ((double(s[i]) + k.B1 * t1) + k.B2 * t2) + k.B3 * t3;
+        t3 = t2;
+        t2 = t1;
+        t1 = v;
+        t[i] = v;
+    }
+    for (size_t i = 1; i < n; ++i) out[i * stride] = T(k.B * t[i]);
+    {
+        const double e1 = t[0], e2 = n >= 2 ? t[1] : 0.0, e3 = n >= 3 ? t[2] : 0.0;
+        out[0] = T(k.B * t[0] + (((k.B1 + k.B2) + k.B3) * e1 + (k.B2 + k.B3) * e2 + k.B3 * e3));
+    }
+}
+
 template <class T>
 void iir_plane(const T* in, T* out, const T* acc, int mode, T wt, double sigma, int w, int h) {
     const Yvv k = yvv(sigma);
@@ -713,18 +755,20 @@ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t seed) {
 }
 
 float sublayer_draw(float d, float D, float n, float u, uint32_t pixel, uint32_t stream, uint32_t seed,
-                    float* count_out = nullptr) {
+                    float* count_out = nullptr, float* lam_out = nullptr) {
     float q = d / D;
     q = q < 1e-6f ? 1e-6f : (q > 0.999999f ? 0.999999f : q);
     const float sat = 1.0f - ((q * u) * 0.999999f);
     const float lam = (n / sat) * q;
     const float k = poisson(lam, pixel, stream, seed);
     if (count_out) *count_out = k;
+    if (lam_out) *lam_out = lam;
     return k * ((D / n) * sat);
 }
 
 // Grain SUM draw of one pixel and channel from its density D (binary32 by definition).
-float grain_sum(const spk_plan& p, float D, uint32_t pixel, uint32_t c, float* counts9 = nullptr) {
+float grain_sum(const spk_plan& p, float D, uint32_t pixel, uint32_t c, float* counts9 = nullptr,
+                float* lams9 = nullptr) {
     const float v = p.grain_positive ? -D : D;
     const uint32_t K = p.grain_k;
     const float* axis = p.grain_axis;
@@ -754,7 +798,7 @@ float grain_sum(const spk_plan& p, float D, uint32_t pixel, uint32_t c, float* c
     for (uint32_t l = 0; l < 3; ++l) {
         const uint32_t lc = l * 3 + c;
         val[l] = sublayer_draw(d[l] + p.dmin_l[lc], p.sat_d[lc], p.count[lc], p.uniformity[c], pixel, c + 10u * l, p.grain_seed,
-                               counts9 ? counts9 + 3 * c + l : nullptr);
+                               counts9 ? counts9 + 3 * c + l : nullptr, lams9 ? lams9 + 3 * c + l : nullptr);
     }
     float G = ((0.0f + val[0]) + val[1]) + val[2];
     return G * 1.0f + (-p.dmin_c[c]);
@@ -805,10 +849,11 @@ T grain_frozen(const spk_plan& p, T D, const float* k9, uint32_t c) {
 
 // rgb: the tier image (In = float, or Dual for derivatives with respect to the pixels).
 // thp: the parameters (null: the plan's).  counts_in: frozen Poisson counts, 9 per pixel
-// ([i * 9 + 3 c + l]); counts_out: records the counts the binary32 draw makes.
+// ([i * 9 + 3 c + l]); counts_out: records the counts the binary32 draw makes; lams_out their
+// Poisson means (same layout).
 template <class T, class In = float>
 void develop(const spk_plan& p, const In* rgb, T* neg, const Theta<T>* thp = nullptr, const float* counts_in = nullptr,
-             float* counts_out = nullptr) {
+             float* counts_out = nullptr, float* lams_out = nullptr) {
     const int w = int(p.width), h = int(p.height);
     const size_t np = size_t(w) * h;
     std::unique_ptr<Theta<T>> own;
@@ -1039,7 +1084,8 @@ void develop(const spk_plan& p, const In* rgb, T* neg, const Theta<T>* thp = nul
                 if (counts_in)
                     G[o] = grain_frozen<T>(p, Dv, counts_in + 9 * size_t(i), uint32_t(c));
                 else
-                    G[o] = T(grain_sum(p, float(Dv), uint32_t(i), uint32_t(c), counts_out ? counts_out + 9 * size_t(i) : nullptr));
+                    G[o] = T(grain_sum(p, float(Dv), uint32_t(i), uint32_t(c), counts_out ? counts_out + 9 * size_t(i) : nullptr,
+                                       lams_out ? lams_out + 9 * size_t(i) : nullptr));
             }
         }
     }
@@ -1486,6 +1532,19 @@ void oracle_grain_counts(const spk_plan* p, const float* rgb, float* counts) {
     develop<float>(*p, rgb, neg.data(), nullptr, nullptr, counts);
 }
 
+// The binary32 develop with its grain draws: the negative (3 planes), and per pixel the 9 counts
+// and their Poisson means ([i * 9 + 3 c + l]).
+void oracle_develop_draws(const spk_plan* p, const float* rgb, float* neg, float* counts, float* lams) {
+    develop<float>(*p, rgb, neg, nullptr, nullptr, counts, lams);
+}
+
+// Poisson(lam[i]) of the grain sampler for draw i of pixel pixel[i] on stream[i] (seed shared).
+void oracle_poisson_at(const float* lam, const uint32_t* pixel, const uint32_t* stream, size_t n, uint32_t seed,
+                       float* out) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < long(n); ++i) out[i] = poisson(lam[i], pixel[i], stream[i], seed);
+}
+
 // The frozen-count render in float64 with quantity idx moved by h (idx < 0: none; 40 + j moves the
 // pixels along in_dir); returns the L2 loss against target and writes the encoded output (3 planes)
 // when enc is not null.
@@ -1645,6 +1704,35 @@ void oracle_gaussian_f32(const float* in, float* out, int w, int h, const double
     for (size_t i = 0; i < np; ++i)
         for (int c = 0; c < 3; ++c) out[3 * i + c] = b[c * np + i];
 }
+// The transposed Gaussian of oracle_gaussian_f32: per channel, the FIR pass itself (with
+// half-sample reflection and symmetric taps it is a symmetric operator) or the transposed
+// recursion on rows, then columns.
+void oracle_gaussian_adj_f32(const float* in, float* out, int w, int h, const double* sigma3, double crossover) {
+    const size_t np = size_t(w) * h;
+    std::vector<float> a(3 * np), b(3 * np);
+    for (size_t i = 0; i < np; ++i)
+        for (int c = 0; c < 3; ++c) a[c * np + i] = in[3 * i + c];
+    for (int c = 0; c < 3; ++c) {
+        const float* ic = a.data() + c * np;
+        float* oc = b.data() + c * np;
+        const double sg = sigma3[c];
+        if (!(sg > 0.0) || (sg < crossover && fir_radius(sg) == 0)) {
+            std::copy(ic, ic + np, oc);
+        } else if (sg < crossover) {
+            fir_plane<float>(ic, oc, nullptr, 0, 1.0f, sg, w, h);
+        } else {
+            const Yvv k = yvv(sg);
+            std::vector<float> tmp(np);
+#pragma omp parallel for schedule(static)
+            for (int y = 0; y < h; ++y) yvv_line_t(ic + size_t(y) * w, tmp.data() + size_t(y) * w, size_t(w), 1, k);
+#pragma omp parallel for schedule(static)
+            for (int x = 0; x < w; ++x) yvv_line_t(tmp.data() + x, oc + x, size_t(h), size_t(w), k);
+        }
+    }
+    for (size_t i = 0; i < np; ++i)
+        for (int c = 0; c < 3; ++c) out[3 * i + c] = b[c * np + i];
+}
+
 void oracle_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, uint32_t n) {
     for (uint32_t i = 0; i < n; ++i) {
         uint32_t c[4] = {ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3]};

@@ -35,6 +35,8 @@ def lib():
         l.oracle_print_f32.argtypes = [V, F, U16, F]
         l.oracle_print_f64.argtypes = [V, D, U16, D]
         l.oracle_gaussian_f32.argtypes = [F, F, C.c_int, C.c_int, D, C.c_double]
+        l.oracle_gaussian_adj_f32.argtypes = [F, F, C.c_int, C.c_int, D, C.c_double]
+        l.oracle_gaussian_adj_f32.restype = None
         l.oracle_philox.argtypes = [U, U, U, C.c_uint32]
         l.oracle_poisson.argtypes = [F, C.c_uint32, C.c_uint32, C.c_uint32, F]
         l.oracle_uniform.argtypes = [C.c_uint32]
@@ -52,6 +54,10 @@ def lib():
         l.oracle_plan_offset.restype = C.c_long
         l.oracle_grain_counts.argtypes = [V, F, F]
         l.oracle_grain_counts.restype = None
+        l.oracle_develop_draws.argtypes = [V, F, F, F, F]
+        l.oracle_develop_draws.restype = None
+        l.oracle_poisson_at.argtypes = [F, U, U, C.c_size_t, C.c_uint32, F]
+        l.oracle_poisson_at.restype = None
         l.oracle_frozen_f64.argtypes = [V, F, F, F, C.c_int, C.c_double, D, D, D]
         l.oracle_frozen_f64.restype = C.c_double
         l.oracle_frozen_grad.argtypes = [V, F, F, F, D, D, C.c_int, D]
@@ -150,6 +156,16 @@ def gaussian(image, sigma3, crossover=3.0):
     return out
 
 
+def gaussian_adjoint(image, sigma3, crossover=3.0):
+    """The transpose of gaussian() (the FIR passes are symmetric; the recursion is transposed)."""
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    h, w, _ = img.shape
+    out = np.zeros_like(img)
+    s = (C.c_double * 3)(*[float(v) for v in sigma3])
+    lib().oracle_gaussian_adj_f32(_p(img, C.c_float), _p(out, C.c_float), w, h, s, crossover)
+    return out
+
+
 def print_lut(negative, table, axes):
     """negative: 3 x H x W float32; table: (S, S, S, 3); axes: (3, S).  Returns H x W x 4 uint16."""
     neg = np.ascontiguousarray(negative, dtype=np.float32)
@@ -186,6 +202,31 @@ def grain_counts(plan, rgb):
     rgb = np.ascontiguousarray(rgb, dtype=np.float32)
     out = np.zeros((h, w, 9), np.float32)
     lib().oracle_grain_counts(C.cast(plan, C.c_void_p), _p(rgb, C.c_float), _p(out, C.c_float))
+    return out
+
+
+def develop_draws(plan, rgb):
+    """The binary32 develop with its grain draws: (negative 3 x H x W, lambdas 9 x H x W,
+    counts 9 x H x W), draw 3 c + l (channel c, sub-layer l) -- the layout spkx_grain_record uses."""
+    w, h = plan_size(plan)
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    neg = np.zeros(3 * w * h, np.float32)
+    cnt = np.zeros((h * w, 9), np.float32)
+    lam = np.zeros((h * w, 9), np.float32)
+    lib().oracle_develop_draws(C.cast(plan, C.c_void_p), _p(rgb, C.c_float), _p(neg, C.c_float), _p(cnt, C.c_float),
+                               _p(lam, C.c_float))
+    return (neg.reshape(3, h, w), np.ascontiguousarray(lam.T).reshape(9, h, w),
+            np.ascontiguousarray(cnt.T).reshape(9, h, w))
+
+
+def poisson_at(lam, pixel, stream, seed):
+    """The grain sampler's Poisson(lam[i]) for pixel[i] on stream[i] (1-D arrays of one length)."""
+    lam = np.ascontiguousarray(lam, dtype=np.float32).ravel()
+    pixel = np.ascontiguousarray(pixel, dtype=np.uint32).ravel()
+    stream = np.ascontiguousarray(stream, dtype=np.uint32).ravel()
+    out = np.zeros_like(lam)
+    lib().oracle_poisson_at(_p(lam, C.c_float), _p(pixel, C.c_uint32), _p(stream, C.c_uint32), len(lam), seed,
+                            _p(out, C.c_float))
     return out
 
 

@@ -469,6 +469,16 @@ SPK_API spk_status spkx_negative(spk_session* session, const char* tier, float* 
     });
 }
 
+SPK_API spk_status spkx_grain_record(spk_session* session, const char* tier, float* lambdas, float* counts,
+                                     size_t count) {
+    le::clear();
+    if (!session || !tier) return invalid("a required argument is NULL");
+    return guarded([&] {
+        S(session)->grain_record(tier, lambdas, counts, count);
+        return SPK_OK;
+    });
+}
+
 SPK_API size_t spkx_plan(const char* resources_dir, const char* params_delta_json, const char* overrides_json,
                          uint32_t width, uint32_t height, uint32_t film_seed, void* out, size_t cap) {
     le::clear();
@@ -611,6 +621,31 @@ SPK_API spk_status spkx_gaussian(spk_engine* engine, const float* in, float* out
     });
 }
 
+SPK_API spk_status spkx_gaussian_adjoint(spk_engine* engine, const float* in, float* out, uint32_t width,
+                                         uint32_t height, const double* sigma3, double fir_crossover) {
+    le::clear();
+    if (!engine || !in || !out || !sigma3) return invalid("a required argument is NULL");
+    return guarded([&] {
+        Engine& e = *E(engine);
+        spk::gpu::DeviceScope ds(e.device());
+        const size_t np = size_t(width) * height;
+        std::vector<float> planar(3 * np);
+        for (size_t i = 0; i < np; ++i)
+            for (int c = 0; c < 3; ++c) planar[size_t(c) * np + i] = in[3 * i + size_t(c)];
+        spk::gpu::DevBuf a = e.pool().acquire(3 * np * 4), b = e.pool().acquire(3 * np * 4);
+        spk::gpu::check(hipMemcpy(a->ptr, planar.data(), 3 * np * 4, hipMemcpyHostToDevice), "hipMemcpy");
+        std::vector<spk::gpu::DevBuf> keep;
+        for (int c = 0; c < 3; ++c)
+            spk::chain::blur_planes(static_cast<float*>(a->ptr) + size_t(c) * np, static_cast<float*>(b->ptr) + size_t(c) * np,
+                                    1, sigma3[c], sigma3[c], fir_crossover, true, width, height, e.pool(), nullptr, keep);
+        spk::gpu::check(hipDeviceSynchronize(), "gaussian adjoint");
+        spk::gpu::check(hipMemcpy(planar.data(), b->ptr, 3 * np * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        for (size_t i = 0; i < np; ++i)
+            for (int c = 0; c < 3; ++c) out[3 * i + size_t(c)] = planar[size_t(c) * np + i];
+        return SPK_OK;
+    });
+}
+
 SPK_API spk_status spkx_downscale(spk_engine* engine, const float* in, uint32_t width, uint32_t height,
                                   uint32_t long_edge, float* out, size_t cap_floats, uint32_t* out_w, uint32_t* out_h) {
     le::clear();
@@ -670,6 +705,16 @@ SPK_API spk_status spkx_batch_sync(spkx_batch* batch) {
     if (!batch) return invalid("batch is NULL");
     return guarded([&] {
         reinterpret_cast<spk::Batch*>(batch)->sync();
+        return SPK_OK;
+    });
+}
+
+SPK_API spk_status spkx_batch_grain_record(spkx_batch* batch, int32_t frame, float* lambdas, float* counts,
+                                           size_t count) {
+    le::clear();
+    if (!batch) return invalid("batch is NULL");
+    return guarded([&] {
+        reinterpret_cast<spk::Batch*>(batch)->grain_record(frame, lambdas, counts, count);
         return SPK_OK;
     });
 }

@@ -384,7 +384,8 @@ void Diff::backward(int slot, const float* input, const float* g_enc, float* g_i
             if (tile_fd) {
                 const k::FirArgs ap = chain::fir_args_at(sg[0] * (1.0 + kSigmaStep), sg[0], w, h);
                 const k::FirArgs am = chain::fir_args_at(sg[0] * (1.0 - kSigmaStep), sg[0], w, h);
-                k::adj_fir2d_dot(H, gAcc, ap, am, sc, 3, 0, k::adj_blocks(n), part + rg.bs[kk], s);
+                gpu::check(k::adj_fir2d_dot(H, gAcc, ap, am, sc, 3, 0, k::adj_blocks(n), part + rg.bs[kk], s),
+                           "adj_fir2d_dot (radius outside the fused tile's 1..8)");
                 continue;
             }
             // a separable FIR component: the vertical passes of both blurs over one read of H, then

@@ -16,9 +16,10 @@ namespace {
 constexpr int kMaxStreams = 4;  // the device's hardware queues per process
 constexpr int kEvents = 32;  // 3 forks + 4 joins of up to 4 streams
 
-BatchConfig parse_options(const char* text, bool* autotune) {
+BatchConfig parse_options(const char* text, bool* autotune, bool* record) {
     BatchConfig c;
     *autotune = true;
+    *record = false;
     if (!text || !*text) return c;
     json::Value v;
     try {
@@ -31,6 +32,9 @@ BatchConfig parse_options(const char* text, bool* autotune) {
         if (k == "autotune") {
             if (!val.is_bool()) fail_user("autotune expects bool");
             *autotune = val.as_bool();
+        } else if (k == "record_grain") {
+            if (!val.is_bool()) fail_user("record_grain expects bool");
+            *record = val.as_bool();
         } else if (k == "schedule") {
             if (!val.is_string() || (val.as_string() != "phased" && val.as_string() != "streams"))
                 fail_user("schedule expects \"phased\" or \"streams\"");
@@ -72,8 +76,8 @@ std::unique_ptr<Batch> Batch::create(Engine& e, Session** sessions, int n, void*
                                      const char* options) {
     if (n <= 0) fail(SPK_ERR_INVALID_ARG, "a batch needs at least one session");
     if (!sessions || !outs || !strides) fail(SPK_ERR_INVALID_ARG, "sessions, outputs and strides must not be NULL");
-    bool autotune = true;
-    BatchConfig fixed = parse_options(options, &autotune);
+    bool autotune = true, record = false;
+    BatchConfig fixed = parse_options(options, &autotune, &record);
     if (autotune) fixed.mode = 0;
     std::unique_ptr<Batch> b(new Batch);
     b->e_ = &e;
@@ -97,6 +101,7 @@ std::unique_ptr<Batch> Batch::create(Engine& e, Session** sessions, int n, void*
         f.out = static_cast<uint16_t*>(outs[i]);
         f.stride = strides[i];
         f.negative = e.pool().acquire(size_t(f.bf.w) * f.bf.h * 3 * sizeof(float));
+        if (record && pl.grain_active) f.grain = e.pool().acquire(18 * size_t(f.bf.w) * f.bf.h * sizeof(float));
         if (f.bf.plan->spectral_fp16) {
             if (fixed.mode == 1)
                 fail_user("the streams schedule does not take FP16-spectral sessions (session " + std::to_string(i) +
@@ -221,8 +226,9 @@ std::unique_ptr<Batch::Graph> Batch::capture(const BatchConfig& c) {
         std::vector<chain::DevelopStage> dst(n);
         for (uint32_t i : order_) {
             Frame& f = frames_[i];
+            const chain::GrainRecord rec = f.record();
             chain::develop_pre(*f.bf.plan, f.bf.tables, gpu::as<const float>(f.bf.src), gpu::as<float>(f.negative), pool,
-                               st(i), g->keep, &dst[i]);
+                               st(i), g->keep, &dst[i], nullptr, &rec);
         }
         join();
         {
@@ -314,8 +320,9 @@ void Batch::capture_streams(const BatchConfig& c, Graph& g, int S, const std::ve
         Frame& f = frames_[i];
         hipStream_t s = streams_[size_t(lane[i])]->get();
         chain::DevelopStage ds;
+        const chain::GrainRecord rec = f.record();
         chain::develop_pre(*f.bf.plan, f.bf.tables, gpu::as<const float>(f.bf.src), gpu::as<float>(f.negative), pool, s,
-                           g.keep, &ds);
+                           g.keep, &ds, nullptr, &rec);
         jobs[i] = ds.job;
         const uint64_t px = uint64_t(f.bf.w) * f.bf.h;
         prefix[2 * i] = 0;
@@ -359,6 +366,19 @@ void Batch::run(hipStream_t stream) {
     gpu::check(hipGraphLaunch(graph_->exec, s), "hipGraphLaunch");
     gpu::check(hipEventRecord(done_, s), "hipEventRecord");
     launched_ = true;
+}
+
+void Batch::grain_record(int i, float* lambdas, float* counts, size_t count) {
+    if (i < 0 || size_t(i) >= frames_.size()) fail(SPK_ERR_INVALID_ARG, "frame index out of range");
+    const Frame& f = frames_[size_t(i)];
+    if (!f.grain) fail_user("the batch records no grain draws for frame " + std::to_string(i));
+    const size_t n = size_t(f.bf.w) * f.bf.h * 9;
+    if (count < n) fail(SPK_ERR_INVALID_ARG, "output too small for the grain record");
+    gpu::DeviceScope ds(e_->device());
+    sync();
+    const float* base = gpu::as<const float>(f.grain);
+    if (lambdas) gpu::check(hipMemcpy(lambdas, base, n * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy(lambdas)");
+    if (counts) gpu::check(hipMemcpy(counts, base + n, n * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy(counts)");
 }
 
 void Batch::sync() {

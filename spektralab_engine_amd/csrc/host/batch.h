@@ -48,6 +48,9 @@ public:
     void run(hipStream_t stream);
     void sync();
     std::string info() const;
+    // option "record_grain": the grain draws of frame i's last run (9 planar planes of Poisson
+    // means, 9 of counts, as Session::grain_record).
+    void grain_record(int i, float* lambdas, float* counts, size_t count);
 
 private:
     struct Frame {
@@ -55,8 +58,17 @@ private:
         Session::BatchFrame bf;
         gpu::DevBuf negative;
         gpu::DevBuf enl_mfma;  // FP16-spectral frames: the enlarger operands, uploaded before capture
+        gpu::DevBuf grain;     // record_grain: 9 planes of Poisson means, then 9 of counts
         uint16_t* out = nullptr;
         uint32_t stride = 0;
+        chain::GrainRecord record() const {
+            chain::GrainRecord r;
+            if (grain) {
+                r.lambdas = gpu::as<float>(grain);
+                r.counts = r.lambdas + 9 * size_t(bf.w) * bf.h;
+            }
+            return r;
+        }
     };
     struct Graph {
         hipGraph_t graph = nullptr;

@@ -564,7 +564,7 @@ float device_max(const float* p, size_t n, gpu::Pool& pool, hipStream_t s, std::
 }
 
 void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
-                 std::vector<gpu::DevBuf>& keep, DevelopStage* st, Profiler* prof) {
+                 std::vector<gpu::DevBuf>& keep, DevelopStage* st, Profiler* prof, const GrainRecord* rec) {
     auto mark = [&](const char* l) {
         if (prof && prof->on) prof->mark(l, s);
     };
@@ -665,6 +665,10 @@ void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* ne
     std::memcpy(da.count, p.count, sizeof da.count);
     k::set_grain_scale(da);
     set_hints(da, t);
+    if (rec && p.grain_active) {
+        da.lambdas = rec->lambdas;
+        da.counts = rec->counts;
+    }
     std::memcpy(da.uniformity, p.uniformity, sizeof da.uniformity);
     std::memcpy(da.dmin_c, p.dmin_c, sizeof da.dmin_c);
     for (int c = 0; c < 3; ++c) {
@@ -788,9 +792,9 @@ size_t scratch_bound(const spk_plan& p, bool develop, bool tables, bool negative
 }
 
 void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
-             std::vector<gpu::DevBuf>& keep, Profiler* prof) {
+             std::vector<gpu::DevBuf>& keep, Profiler* prof, const GrainRecord* rec) {
     DevelopStage st;
-    develop_pre(p, t, rgb, negative, pool, s, keep, &st, prof);
+    develop_pre(p, t, rgb, negative, pool, s, keep, &st, prof, rec);
     const k::DevelopJob& j = st.job;
     k::develop(j.H, j.acc, j.out, j.density, j.curves, j.gaxis, j.ginv, j.glayers, j.lgam, j.a, s);
     if (prof && prof->on) prof->mark("develop", s);

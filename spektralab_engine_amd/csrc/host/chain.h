@@ -59,9 +59,16 @@ bool axis_hints(const float* a, size_t st, size_t n, uint16_t* hint, float& lo, 
 // within a render (an arena) needs no more than this.
 size_t scratch_bound(const spk_plan& p, bool develop, bool tables, bool negative);
 
+// Device planes that receive the grain draws of a develop: lambdas[(3 c + l) n + i] the Poisson
+// mean of draw 3 c + l of pixel i, counts the same layout for the count drawn (either may be null).
+struct GrainRecord {
+    float* lambdas = nullptr;
+    float* counts = nullptr;
+};
+
 // rgb: interleaved w x h x 3 on the device; negative: 3 planar planes (w*h*3 floats).
 void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
-             std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr);
+             std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr, const GrainRecord* rec = nullptr);
 // out: RGBA16 with a row pitch of `stride` pixels.
 void print(const spk_plan& p, const Tables& t, const float* negative, uint16_t* out, uint32_t stride, gpu::Pool& pool,
            hipStream_t s, std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr);
@@ -76,7 +83,8 @@ struct DevelopStage {
     bool blend = false;
 };
 void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
-                 std::vector<gpu::DevBuf>& keep, DevelopStage* st, Profiler* prof = nullptr);
+                 std::vector<gpu::DevBuf>& keep, DevelopStage* st, Profiler* prof = nullptr,
+                 const GrainRecord* rec = nullptr);
 void develop_post(const spk_plan& p, const DevelopStage& st, float* negative, gpu::Pool& pool, hipStream_t s,
                   std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr);
 struct PrintStage {

@@ -109,6 +109,7 @@ struct Pipeline {
 
 struct TierState {
     gpu::DevBuf negative;  // 3 planar planes of `size`
+    gpu::DevBuf grain;     // record_grain: 9 planes of Poisson means, then 9 of counts
     tiers::Size size;
     double pitch_um = 0.0;
     uint32_t film_seed = 0;
@@ -150,6 +151,8 @@ public:
     void set_overrides(const char* json);
     size_t last_plan(void* out, size_t cap);
     void negative(const char* tier, float* out, size_t count);
+    // The grain draws of the tier's last develop (override record_grain): 9 planar planes each.
+    void grain_record(const char* tier, float* lambdas, float* counts, size_t count);
 
     void cancel(const char* progress_id);
     std::string progress(const char* progress_id);

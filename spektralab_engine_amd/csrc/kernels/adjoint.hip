@@ -622,8 +622,9 @@ void adj_dot_diff(const float* g, const float* fp, const float* fm, const float 
 
 uint32_t fir2d_tiles(uint32_t w, uint32_t h) { return ((w + kFdTX - 1) / kFdTX) * ((h + kFdTY - 1) / kFdTY); }
 
-void adj_fir2d_dot(const float* in, const float* g, const FirArgs& plus, const FirArgs& minus, const float scale[3],
-                   uint32_t nplanes, uint32_t c0, uint32_t nrows, double* part, hipStream_t s) {
+hipError_t adj_fir2d_dot(const float* in, const float* g, const FirArgs& plus, const FirArgs& minus,
+                         const float scale[3], uint32_t nplanes, uint32_t c0, uint32_t nrows, double* part,
+                         hipStream_t s) {
     const dim3 grid(xcd_grid(fir2d_tiles(plus.w, plus.h) * nplanes)), block(kFdNT);
     switch (plus.r) {
 #define SPK_FD_CASE(RR)                                                                                          \
@@ -640,8 +641,9 @@ void adj_fir2d_dot(const float* in, const float* g, const FirArgs& plus, const F
         SPK_FD_CASE(7)
         SPK_FD_CASE(8)
 #undef SPK_FD_CASE
-        default: break;
+        default: return hipErrorInvalidValue;  // no kernel for this radius: nothing was written
     }
+    return hipGetLastError();
 }
 
 void adj_reduce(const AdjTerm* terms, uint32_t nq, const double* part, double* gq, hipStream_t s) {

@@ -446,25 +446,26 @@ __device__ inline void yvv_chunks(float (&v)[MC], uint32_t j, uint32_t nch, uint
     if (j > 0 && has) e = get(j - 1);
     __syncthreads();
     const St ein = e;
+    // kAdj: a[n-1], a[n-2], a[n-3] of the last chunk in float64 (from its entry state for len < 3).
+    // The end term divides them by B (small for wide Gaussians) with weights that nearly cancel,
+    // so their binary32 roundings would be amplified there.
+    double a1 = 0.0, a2 = len == 1 ? ein.a : 0.0, a3 = len == 2 ? ein.a : (len == 1 ? ein.b : 0.0);
 #pragma unroll
     for (int k = 0; k < MC; ++k) {
         const double t = yvv_step(q, v[k], e);
         e = {t, e.a, e.b};
         v[k] = float(t);
+        if constexpr (kAdj) {
+            if (uint32_t(k) + 1 == len) a1 = t;
+            if (uint32_t(k) + 2 == len) a2 = t;
+            if (uint32_t(k) + 3 == len) a3 = t;
+        }
     }
 
     // ---- backward ----
     St eb{0.0, 0.0, 0.0};
     if constexpr (kAdj) {
         if (j == last) {
-            // a[n-1], a[n-2], a[n-3]: from the chunk, or from its entry state for len < 3
-            double a1 = 0.0, a2 = len == 1 ? ein.a : 0.0, a3 = len == 2 ? ein.a : (len == 1 ? ein.b : 0.0);
-#pragma unroll
-            for (int k = 0; k < MC; ++k) {
-                if (uint32_t(k) + 1 == len) a1 = v[k];
-                if (uint32_t(k) + 2 == len) a2 = v[k];
-                if (uint32_t(k) + 3 == len) a3 = v[k];
-            }
             const double S1 = (q.B1 + q.B2) + q.B3, S2 = q.B2 + q.B3, S3 = q.B3;
             const double end = a1 + ((S1 * a1 + S2 * a2) + S3 * a3) / q.B;
 #pragma unroll

@@ -550,6 +550,10 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
     }
     if (!grain) return;
 
+    if (q.lambdas && valid) {
+#pragma unroll
+        for (uint32_t jd = 0; jd < 9; ++jd) q.lambdas[size_t(jd) * n + i] = gs.v[jd][lane];
+    }
     // list the draws by method, then drain each list with the whole wavefront
     wave_sync();
     uint32_t n_large = 0, n_small = 0;

@@ -90,6 +90,7 @@ struct DevelopArgs {
     float hint_lo[4], hint_scale[4];
     float* counts;            // grain: the Poisson counts drawn, 9 planes: draw 3 c + l of pixel i at
                               // [(3 c + l) n + i] (or null)
+    float* lambdas;           // grain: the Poisson means of those draws, same layout (or null)
 };
 
 // The per-draw density of one grain of sub-layer l, channel c: D_lc / n_lc rounded once in
@@ -282,8 +283,10 @@ void fir_v2(const float* in, float* outp, float* outm, const FirArgs& plus, cons
             hipStream_t s);
 void fir_h_dot(const float* inp, const float* inm, const float* g, const FirArgs& plus, const FirArgs& minus,
                const float scale[3], uint32_t nplanes, uint32_t nrows, double* part, hipStream_t s);
-void adj_fir2d_dot(const float* in, const float* g, const FirArgs& plus, const FirArgs& minus, const float scale[3],
-                   uint32_t nplanes, uint32_t c0, uint32_t nrows, double* part, hipStream_t s);
+// Returns hipErrorInvalidValue, launching nothing, for a radius outside 1..kFir2dMaxR.
+hipError_t adj_fir2d_dot(const float* in, const float* g, const FirArgs& plus, const FirArgs& minus,
+                         const float scale[3], uint32_t nplanes, uint32_t c0, uint32_t nrows, double* part,
+                         hipStream_t s);
 
 // enc (interleaved) := transfer(m_nid (ua lin + ub blur)) (blur null: lin), pre-quantisation.
 void adj_encode_fwd(const float* lin, const float* blur, float* enc, const PrintConsts* pc, float ua, float ub,

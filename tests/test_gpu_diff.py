@@ -17,7 +17,7 @@ from conftest import delta
 
 pytestmark = pytest.mark.gpu
 
-W, H = 96, 64
+W, H = 96, 64  # the size of the cases that name none
 MAPS = {"film_gamma": [1.05, 0.97, 1.0], "film_offset": [0.01, -0.02, 0.0],
         "paper_gamma": [0.98, 1.02, 1.0], "paper_offset": [-0.01, 0.0, 0.015]}
 CASES = {
@@ -34,15 +34,29 @@ CASES = {
                       fir_crossover=16.5)),
     # no grain, no halation
     "plain": (delta(grain_active=False, halation_active=False), dict(MAPS)),
+    # a separable FIR component (sigma 6: radius 18, above the 2-D tile's 8) at a width where the
+    # reverse pass fuses both sigma blurs with the dot product (k_fir_v2 + k_fir_h_dot)
+    "fir_separable": (delta(grain_active=True, halation_active=True, film_format_mm=10.0),
+                      dict(MAPS, halation_mixture=[[6.0, 0.6], [1.4, 0.3], [20.0, 0.1]], halation_scatter=False,
+                           fir_crossover=16.5), (256, 160)),
+    # the stock's own halation: per-channel sigmas (no mixture override)
+    "per_channel": (delta(grain_active=False, halation_active=True, film_format_mm=4.0),
+                    dict(MAPS, halation_scatter=False)),
 }
-GRAIN = {"fir_iir", "iir_glare"}
+GRAIN = {"fir_iir", "iir_glare", "fir_separable"}
+
+
+def size_of(name):
+    c = CASES[name]
+    return c[2] if len(c) > 2 else (W, H)
 
 
 @pytest.fixture(scope="module", params=sorted(CASES))
 def case(request, engine, oracle):
     import torch
-    dl, ov = CASES[request.param]
-    img = synth.make_frame(W, H, 11)
+    dl, ov = CASES[request.param][:2]
+    w, h = size_of(request.param)
+    img = synth.make_frame(w, h, 11)
     s = engine.open(img, dl)
     s.set_overrides(ov)
     d = spk.Diff(s, slots=2)
@@ -55,7 +69,7 @@ def case(request, engine, oracle):
 
 
 def _enc_ref(o, plan, img, counts, idx=-1, h=0.0, wt_dir=None):
-    _, enc = o.frozen_f64(plan, img, counts, np.zeros((H, W, 3), np.float32), idx, h, wt_dir, want_enc=True)
+    _, enc = o.frozen_f64(plan, img, counts, np.zeros(img.shape, np.float32), idx, h, wt_dir, want_enc=True)
     return enc.transpose(1, 2, 0)
 
 
@@ -106,7 +120,7 @@ def test_gradients_match_forward_mode(case):
     th_t[39] = 0.0
     target = spk.diff_render(d, x, torch.from_numpy(th_t)).detach().cpu().numpy()
     wt_dir = rng.normal(0, 1, 243) * np.abs(o.field(plan, "enl_wt64", 243)).mean()
-    in_dirs = rng.normal(0, 1, (2, H, W, 3)) * img[None] * 0.1
+    in_dirs = rng.normal(0, 1, (2,) + img.shape) * img[None] * 0.1
     loss_ref, g = o.frozen_grad(plan, img, counts, target, wt_dir, in_dirs)
 
     xg = x.clone().requires_grad_(True)

@@ -111,3 +111,18 @@ def test_poisson_statistics(oracle):
 
 def test_poisson_nan_draws_zero(oracle):
     assert oracle.poisson(np.array([np.nan, 0.0], np.float32), 0, 1).tolist() == [0.0, 0.0]
+
+
+def test_transposed_recursion_is_the_adjoint(oracle):
+    """<G x, y> = <x, G^T y> for the oracle's transposed Gaussian (recursive and FIR channels, lines
+    of 1 to 300 samples): the identity fixes every edge term of the transposed recursion; a wrong
+    weight or index at either end moves the two sides apart by far more than binary32 rounding."""
+    rng = np.random.default_rng(0)
+    for w, h, sig in [(37, 29, (5.0, 8.0, 32.0)), (64, 300, (3.0, 3.0, 3.0)), (7, 5, (4.0, 40.0, 3.5)),
+                      (2, 2, (10.0, 10.0, 10.0)), (1, 1, (5.0, 5.0, 5.0)), (19, 3, (2.0, 1.0, 6.0))]:
+        x = rng.normal(size=(h, w, 3)).astype(np.float32)
+        y = rng.normal(size=(h, w, 3)).astype(np.float32)
+        a = np.sum(oracle.gaussian(x, sig, 3.0).astype(np.float64) * y)
+        b = np.sum(x.astype(np.float64) * oracle.gaussian_adjoint(y, sig, 3.0))
+        scale = np.sum(np.abs(x)) * np.abs(y).max()
+        assert abs(a - b) <= 1e-6 * scale, (w, h, sig, a, b)
