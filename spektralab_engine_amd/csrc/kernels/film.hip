@@ -75,26 +75,58 @@ __device__ inline float curve_lookup(const float* __restrict__ x, const float* _
     return y[l] + t * (y[l + 1] - y[l]);
 }
 
-// Hormann's PTRS constants for one lam >= 10 (reference kernels/fx_random.h:52-58).
+// a / b rounded to nearest, as IEEE binary32 division: the sequence the compiler emits for it
+// (reciprocal estimate, one refinement of the reciprocal, two corrections of the quotient by
+// fused multiply-adds) without its range scaling and special-value fix-up, which change nothing
+// while b, a / b and the intermediates are finite normal numbers.  That holds at every call below
+// but one case: a PTRS try with U = 1/2 exactly (us = 0) gets NaN for (2a) / us where the division
+// gives +inf, and the try is rejected either way (us < 0.013 and V > 0), before a / us^2.
+__device__ inline float fdiv_rn(float a, float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e0 = fmaf(-b, y0, 1.0f);
+    const float y1 = fmaf(e0, y0, y0);
+    const float q0 = a * y1;
+    const float r0 = fmaf(-b, q0, a);
+    const float q1 = fmaf(r0, y1, q0);
+    const float r1 = fmaf(-b, q1, a);
+    return fmaf(r1, y1, q1);
+}
+
+// Hormann's PTRS constants for one lam >= 10 (reference kernels/fx_random.h:52-58), with
+// log2(1/alpha) estimated for the screened acceptance test below.
 struct Ptrs {
-    float lam, loglam, b, a, vr, loginv;
+    float lam, loglam, b, a, vr, log2inv;
 };
 
+// Margin of the screened acceptance test: the estimate of L differs from the binary32 L of the
+// exact test by less than 3e-5 (hardware log2, about 1 ulp, of arguments whose logarithms are at
+// most ~51 in magnitude, plus binary32 roundings of L itself); any estimate farther than this
+// from Rt decides as the exact test does.
+constexpr float kPtrsMargin = 1e-3f;
+
 // One PTRS try from the uniform pair (u, v): 1 = accept *k, 0 = next try.  lgamma(k + 1) comes
-// from the LDS table for k < 1024 (the table holds spkm::lgamma_k exactly).
+// from the LDS table for k < 1024 (the table holds spkm::lgamma_k exactly).  The rejection test
+// L <= Rt (L = (log V + log(1/alpha)) - log(a / us^2 + b)) is screened with hardware log2 first;
+// the exact binary32 evaluation of L runs only when the estimate lies within kPtrsMargin of Rt.
 __device__ inline bool ptrs_try(const Ptrs& P, float u, float v, const float* __restrict__ lgam, float hl2p,
                                 float* k_out) {
     const float U = u - 0.5f;
     const float V = v;
     const float us = 0.5f - fabsf(U);
-    const float k = floorf(((((2.0f * P.a) / us + P.b) * U + P.lam) + 0.43f));
+    const float k = floorf(((fdiv_rn(2.0f * P.a, us) + P.b) * U + P.lam) + 0.43f);
     *k_out = k;
     if (us >= 0.07f && V <= P.vr) return true;
     if (k < 0.0f || (us < 0.013f && V > us)) return false;
-    // V in (0, 1] and a / us^2 + b >= b > 0 (us >= 2^-25): logf_pos is logf_ there
-    const float Lh = ((spkm::logf_pos(V) + P.loginv) - spkm::logf_pos(P.a / (us * us) + P.b));
     const float lg = k < float(spkm::kLgammaTable) ? lgam[int(k)] : spkm::lgamma_k(k + 1.0f, hl2p);
     const float Rt = ((-P.lam + k * P.loglam) - lg);
+    const float uu = us * us;
+    const float Qe = fmaf(P.a, __builtin_amdgcn_rcpf(uu), P.b);
+    const float Le = ((__builtin_amdgcn_logf(V) + P.log2inv) - __builtin_amdgcn_logf(Qe)) * 0.69314718f;
+    if (Le <= Rt - kPtrsMargin) return true;
+    if (Le > Rt + kPtrsMargin) return false;
+    // V in (0, 1] and a / us^2 + b >= b > 0 (us >= 2^-25): logf_pos is logf_ there
+    const float invalpha = 1.1239f + fdiv_rn(1.1328f, P.b - 3.4f);
+    const float Lh = ((spkm::logf_pos(V) + spkm::logf_pos(invalpha)) - spkm::logf_pos(fdiv_rn(P.a, uu) + P.b));
     return Lh <= Rt;
 }
 
@@ -107,11 +139,10 @@ __device__ inline void ptrs_setup(Ptrs& P, float lam) {
     const float slam = sqrtf(lam);
     P.b = 0.931f + (2.53f * slam);
     P.a = -0.059f + (0.02483f * P.b);
-    const float invalpha = 1.1239f + (1.1328f / (P.b - 3.4f));
-    P.vr = 0.9277f - (3.6224f / (P.b - 2.0f));
-    // lam >= 10 and invalpha > 1.1239 are finite and positive (lam is at most n / 2e-6 here)
+    P.vr = 0.9277f - fdiv_rn(3.6224f, P.b - 2.0f);
+    // lam >= 10 is finite and positive (lam is at most n / 2e-6 here)
     P.loglam = spkm::logf_pos(lam);
-    P.loginv = spkm::logf_pos(invalpha);
+    P.log2inv = __builtin_amdgcn_logf(fmaf(1.1328f, __builtin_amdgcn_rcpf(P.b - 3.4f), 1.1239f));
 }
 
 __device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t seed, float hl2p,
@@ -147,16 +178,6 @@ __device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t se
     return X;
 }
 
-// q := clamp(d / D, 1e-6, 0.999999) (NaN stays); sat := 1 - q u 0.999999; lam := (n / sat) q;
-// draw := Poisson(lam) (D / n) sat (kernels/fx_random.h:91-99).
-__device__ inline float sublayer_draw(float d, float D, float n, float u, uint32_t pixel, uint32_t stream,
-                                      uint32_t seed, float hl2p, const float* __restrict__ lgam) {
-    float q = d / D;
-    q = q < 1e-6f ? 1e-6f : (q > 0.999999f ? 0.999999f : q);
-    const float sat = 1.0f - ((q * u) * 0.999999f);
-    const float lam = (n / sat) * q;
-    return poisson(lam, pixel, stream, seed, hl2p, lgam) * ((D / n) * sat);
-}
 
 // ---- the grain draws of one wavefront, compacted ---------------------------------------------
 // A wavefront's 64 pixels make 576 draws (3 channels x 3 sub-layers).  Drawn where they arise,
@@ -184,6 +205,8 @@ __device__ inline uint32_t lanes_below(uint64_t mask) {
 // makes one try per pass of the loop: the (x, y) pair of a fresh Philox block, or the (z, w) pair
 // the lane kept from its previous pass.  A try accepts 75-87 % of the time (lam 10..300), so
 // running both tries of a block in every pass would leave most lanes idle through the second one.
+// (One loop over both methods, the few small draws filling the large ones' idle tail, needs more
+// registers and measured slower: 2.25 against 2.02 ms at 24 MP.)
 template <bool kLarge>
 __device__ __forceinline__ void drain(GrainScratch& gs, uint32_t count, uint32_t lane, uint32_t base, uint32_t seed,
                                       float hl2p, const float* __restrict__ lgam) {
@@ -532,10 +555,10 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
             const uint32_t lc = l * 3 + c;
             const float dl = d[l] + q.dmin_l[lc];
             const float Dl = q.sat_d[lc], nl = q.count[lc];
-            float qq = dl / Dl;
+            float qq = fdiv_rn(dl, Dl);
             qq = qq < 1e-6f ? 1e-6f : (qq > 0.999999f ? 0.999999f : qq);
             const float sat = 1.0f - ((qq * q.uniformity[c]) * 0.999999f);
-            gs.v[3 * c + l][lane] = (nl / sat) * qq;
+            gs.v[3 * c + l][lane] = fdiv_rn(nl, sat) * qq;
             mult[3 * c + l] = q.d_per_n[lc] * sat;
         }
     }

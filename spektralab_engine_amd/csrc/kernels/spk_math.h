@@ -146,7 +146,8 @@ SPKM_FN double fma_d(double a, double b, double c) { return __builtin_fma(a, b, 
 // (truncation < 2^-48 / 6).  Relative error before the final rounding < 2^-50: the float result
 // is the correctly rounded one except within that distance of a rounding midpoint.
 // logf_pos: the same function for a finite x > 0 (no special-case tests).
-SPKM_FN float logf_pos(float x) {
+// ln_pos_d: the same logarithm before its final rounding (binary64, for log10f_).
+SPKM_FN double ln_pos_d(float x) {
     SPKM_NO_CONTRACT
     const uint64_t u = bits_of(double(x));  // float subnormals are normal in binary64
     const int e = int((u >> 52) & 0x7FF) - 1023;
@@ -159,8 +160,9 @@ SPKM_FN float logf_pos(float x) {
     p = fma_d(r, p, -0.5);
     const double lp = fma_d(r * r, p, r);
     const double t = fma_d(double(e), 0x1.62e42fefa39efp-1, logc);
-    return float(t + lp);
+    return t + lp;
 }
+SPKM_FN float logf_pos(float x) { return float(ln_pos_d(x)); }
 SPKM_FN float logf_(float x) {
     if (!(x > 0.0f)) return x == 0.0f ? -__builtin_inff() : __builtin_nanf("");
     if (x == __builtin_inff()) return x;
@@ -216,9 +218,13 @@ SPKM_FN float lgamma_k(float x, float hl2p) {
 }
 SPKM_FN float half_log_2pi() { return 0.5f * logf_(6.283185307179586f); }
 inline constexpr int kLgammaTable = 1024;
+// log10 of the film log: for finite x > 0 the table logarithm above times 1 / ln 10 in binary64
+// (relative error < 2^-49 before the one rounding to binary32, no division); other x through
+// log_d (0 -> -inf, +inf -> +inf, negative or NaN -> NaN).
 SPKM_FN float log10f_(float x) {
     SPKM_NO_CONTRACT
-    return float(log_d(double(x)) * 0.43429448190325182765);  // 1 / ln 10
+    if (x > 0.0f && x < __builtin_inff()) return float(ln_pos_d(x) * 0.43429448190325182765);  // 1 / ln 10
+    return float(log_d(double(x)) * 0.43429448190325182765);
 }
 
 }  // namespace spkm
