@@ -76,6 +76,14 @@ SPK_API spk_status spkx_grain_record(spk_session* session, const char* tier, flo
 SPK_API size_t spkx_plan(const char* resources_dir, const char* params_delta_json, const char* overrides_json,
                          uint32_t width, uint32_t height, uint32_t film_seed, void* out, size_t cap);
 
+/* spk_output_transform computed on the host from a resource folder, without an engine or device
+ * (for hosts that only need the matrices and constants, and for checking them without a GPU).
+ * The reply and its errors are spk_output_transform's; out_cmax (may be NULL) receives the
+ * 64 x 720 chroma-limit table when cap >= 46080. */
+SPK_API spk_status spkx_output_transform_host(const char* resources_dir, const char* src_cs, const char* dst_cs,
+                                              const char* gamut_compress_json, char** out_json, float* out_cmax,
+                                              size_t cap);
+
 /* Batched rendering.  spkx_batch_create snapshots the parameters of n open sessions of one engine
  * and captures one hipGraph that renders every session's full tier (develop + print, fresh seeds
  * from each session's own sequence at every run, as spk_render draws them) into dev_out[i]

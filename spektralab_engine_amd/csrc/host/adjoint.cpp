@@ -22,11 +22,11 @@ int parse_frames(const char* text, int frames) {
     json::Value v;
     try {
         v = json::parse(text);
-    } catch (const json::ParseError& e) {
+    } catch (const json::SyntaxError& e) {
         fail(SPK_ERR_INVALID_ARG, e.what());
     }
-    if (!v.is_object()) fail(SPK_ERR_INVALID_ARG, "differentiable render options must be a JSON object");
-    if (!v.as_object().empty()) fail_user("unknown differentiable render option '" + v.as_object().begin()->first + "'");
+    if (!v.is_map()) fail(SPK_ERR_INVALID_ARG, "differentiable render options must be a JSON object");
+    if (!v.members().empty()) fail_user("unknown differentiable render option '" + v.members().begin()->first + "'");
     return frames;
 }
 

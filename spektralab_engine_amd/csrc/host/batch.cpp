@@ -24,24 +24,24 @@ BatchConfig parse_options(const char* text, bool* autotune, bool* record) {
     json::Value v;
     try {
         v = json::parse(text);
-    } catch (const json::ParseError& e) {
+    } catch (const json::SyntaxError& e) {
         fail(SPK_ERR_INVALID_ARG, e.what());
     }
-    if (!v.is_object()) fail(SPK_ERR_INVALID_ARG, "batch options must be a JSON object");
-    for (const auto& [k, val] : v.as_object()) {
+    if (!v.is_map()) fail(SPK_ERR_INVALID_ARG, "batch options must be a JSON object");
+    for (const auto& [k, val] : v.members()) {
         if (k == "autotune") {
             if (!val.is_bool()) fail_user("autotune expects bool");
-            *autotune = val.as_bool();
+            *autotune = val.boolean();
         } else if (k == "record_grain") {
             if (!val.is_bool()) fail_user("record_grain expects bool");
-            *record = val.as_bool();
+            *record = val.boolean();
         } else if (k == "schedule") {
-            if (!val.is_string() || (val.as_string() != "phased" && val.as_string() != "streams"))
+            if (!val.is_text() || (val.text() != "phased" && val.text() != "streams"))
                 fail_user("schedule expects \"phased\" or \"streams\"");
-            c.mode = val.as_string() == "streams" ? 1 : 0;
+            c.mode = val.text() == "streams" ? 1 : 0;
         } else if (k == "streams" || k == "grid_per_cu" || k == "block") {
             if (!val.is_number()) fail_user(k + " expects a number");
-            const double x = val.as_number();
+            const double x = val.number();
             if (k == "streams") {
                 if (!(x >= 1 && x <= kMaxStreams) || x != double(int(x))) fail_user("streams expects 1..4");
                 c.streams = int(x);

@@ -91,9 +91,9 @@ struct IirPowers {
     double last[9];
 };
 
-IirPowers iir_powers(const setup::IirCoefficients& c, uint32_t m, uint32_t len_last) {
+IirPowers iir_powers(const setup::YvvCoeffs& c, uint32_t m, uint32_t len_last) {
     IirPowers r{};
-    const long double b1 = c.B1, b2 = c.B2, b3 = c.B3;
+    const long double b1 = c.b1, b2 = c.b2, b3 = c.b3;
     const uint64_t kmax = uint64_t(m) << 6;
     for (int col = 0; col < 3; ++col) {
         long double s0 = col == 0, s1 = col == 1, s2 = col == 2;
@@ -124,7 +124,7 @@ IirPowers iir_powers(const setup::IirCoefficients& c, uint32_t m, uint32_t len_l
 // Recursive-pass arguments for lines of length n with nch chunks of m (last chunk n - (nch-1) m).
 // The powers depend only on (sigma, m, len_last) and are cached: a render reuses them every frame.
 k::IirArgs iir_args(double sigma, uint32_t w, uint32_t h, uint32_t n, uint32_t m, uint32_t nch) {
-    const setup::IirCoefficients c = setup::iir_coefficients(sigma);
+    const setup::YvvCoeffs c = setup::yvv(sigma);
     k::IirArgs a{};
     a.w = w;
     a.h = h;
@@ -132,9 +132,9 @@ k::IirArgs iir_args(double sigma, uint32_t w, uint32_t h, uint32_t n, uint32_t m
     a.m = m;
     a.nch = nch;
     a.B = c.B;
-    a.B1 = c.B1;
-    a.B2 = c.B2;
-    a.B3 = c.B3;
+    a.B1 = c.b1;
+    a.B2 = c.b2;
+    a.B3 = c.b3;
     const uint32_t len_last = n - (nch - 1) * m;
     using Key = std::tuple<double, uint32_t, uint32_t>;
     static std::mutex mu;
@@ -507,7 +507,7 @@ Tables upload_tables(const spk_plan& p, gpu::Pool& pool) {
     return t;
 }
 
-const float* preprocess(const float* src, uint32_t sw, uint32_t sh, const setup::GeometryPlan& g, uint32_t decode,
+const float* preprocess(const float* src, uint32_t sw, uint32_t sh, const setup::GeometryMap& g, uint32_t decode,
                         float bt709_bp, float bt709_exp, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep,
                         uint32_t* ow, uint32_t* oh) {
     *ow = sw;
@@ -579,7 +579,7 @@ void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* ne
     const uint32_t decode = p.decode_active ? 1u + p.decode_mode : 0u;
     if (p.geometry_active) {  // decode (on the taps) and geometry in one pass (develop.cpp:117-129)
         uint32_t gw = 0, gh = 0;
-        rgb = preprocess(rgb, p.src_w, p.src_h, setup::plan_geometry(p), decode, p.bt709_bp, p.bt709_exp, pool, s,
+        rgb = preprocess(rgb, p.src_w, p.src_h, setup::geometry_map(p), decode, p.bt709_bp, p.bt709_exp, pool, s,
                          keep, &gw, &gh);
         if (gw != w || gh != h) fail_internal("geometry: output size differs from the plan");
         mark("geometry");

@@ -105,7 +105,7 @@ void print_post(const spk_plan& p, const Tables& t, const PrintStage& st, uint16
 
 // Input decode (decode: 0 none, 1 + mode) and geometry of an interleaved 3-channel image: the
 // processed image (src itself when both are off) and its size.
-const float* preprocess(const float* src, uint32_t sw, uint32_t sh, const setup::GeometryPlan& g, uint32_t decode,
+const float* preprocess(const float* src, uint32_t sw, uint32_t sh, const setup::GeometryMap& g, uint32_t decode,
                         float bt709_bp, float bt709_exp, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep,
                         uint32_t* ow, uint32_t* oh);
 k::EnlargerArgs enlarger_args(const spk_plan& p);

@@ -28,6 +28,7 @@ DevBuf Pool::acquire(size_t bytes) {
         Block* b = new Block{arena_ + arena_top_, bytes, this};
         arena_top_ += bytes;
         live_ += bytes;
+        ++live_n_;
         high_ = std::max(high_, arena_top_);
         return DevBuf(b, release_block);
     }
@@ -38,6 +39,14 @@ DevBuf Pool::acquire(size_t bytes) {
             Block* b = new Block{it->second, it->first, this};
             idle_bytes_ -= it->first;
             live_ += it->first;
+            ++live_n_;
+            ++reuses_;
+            reuse_req_ += bytes;
+            reuse_taken_ += it->first;
+            if (it->first > reuse_max_) {
+                reuse_max_ = it->first;
+                reuse_max_for_ = bytes;
+            }
             high_ = std::max(high_, live_);
             idle_.erase(it);
             return DevBuf(b, release_block);
@@ -53,6 +62,8 @@ DevBuf Pool::acquire(size_t bytes) {
     }
     std::lock_guard<std::mutex> g(m_);
     live_ += bytes;
+    ++live_n_;
+    ++allocs_;
     high_ = std::max(high_, live_);
     return DevBuf(new Block{p, bytes, this}, release_block);
 }
@@ -75,6 +86,7 @@ void Pool::rewind(size_t offset) {
 void Pool::give_back(Block* b) {
     std::lock_guard<std::mutex> g(m_);
     live_ -= b->bytes;
+    --live_n_;
     if (arena_) return;
     idle_bytes_ += b->bytes;
     idle_.emplace(b->bytes, b->ptr);
@@ -111,6 +123,22 @@ size_t Pool::idle_bytes() const {
 size_t Pool::high_water() const {
     std::lock_guard<std::mutex> g(m_);
     return high_;
+}
+Pool::Stats Pool::stats() const {
+    std::lock_guard<std::mutex> g(m_);
+    Stats s;
+    s.live_bytes = live_;
+    s.idle_bytes = idle_bytes_;
+    s.high_water = high_;
+    s.live_buffers = live_n_;
+    s.idle_buffers = idle_.size();
+    s.allocations = allocs_;
+    s.reuses = reuses_;
+    s.reuse_requested = reuse_req_;
+    s.reuse_taken = reuse_taken_;
+    s.reuse_max_taken = reuse_max_;
+    s.reuse_max_taken_for = reuse_max_for_;
+    return s;
 }
 
 PinnedBuf::PinnedBuf(size_t n) : bytes(n) { check(hipHostMalloc(&ptr, n ? n : 1, hipHostMallocDefault), "hipHostMalloc"); }

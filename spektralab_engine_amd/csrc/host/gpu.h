@@ -51,6 +51,14 @@ public:
     size_t live_bytes() const;
     size_t idle_bytes() const;
     size_t high_water() const;
+    // A consistent snapshot of the pool's accounting.
+    struct Stats {
+        size_t live_bytes = 0, idle_bytes = 0, high_water = 0;
+        size_t live_buffers = 0, idle_buffers = 0;
+        uint64_t allocations = 0, reuses = 0, reuse_requested = 0, reuse_taken = 0;
+        size_t reuse_max_taken = 0, reuse_max_taken_for = 0;
+    };
+    Stats stats() const;
     int device() const { return device_; }
 
 private:
@@ -59,7 +67,9 @@ private:
     int device_;
     mutable std::mutex m_;
     std::multimap<size_t, void*> idle_;
-    size_t live_ = 0, idle_bytes_ = 0, high_ = 0;
+    size_t live_ = 0, idle_bytes_ = 0, high_ = 0, live_n_ = 0;
+    uint64_t allocs_ = 0, reuses_ = 0, reuse_req_ = 0, reuse_taken_ = 0;
+    size_t reuse_max_ = 0, reuse_max_for_ = 0;
     char* arena_ = nullptr;
     size_t arena_cap_ = 0, arena_top_ = 0;
 };

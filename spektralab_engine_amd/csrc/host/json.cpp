@@ -1,250 +1,363 @@
-// Strict JSON reader and compact writer; behaviour stated in json.h.
+// The JSON document model; rules stated in json.h.
 #include "json.h"
 
-#include <clocale>
+#include <algorithm>
+#include <charconv>
 #include <cmath>
-#include <cstdio>
-#include <cstdlib>
-#include <locale.h>
+#include <cstdint>
+#include <cstring>
 
 namespace spk::json {
 
-const Value* Value::find(std::string_view key) const {
-    if (!is_object()) return nullptr;
-    const Object& o = as_object();
-    auto it = o.find(key);
-    return it == o.end() ? nullptr : &it->second;
+// ---- the model ------------------------------------------------------------------------------
+Value Value::list(std::vector<Value> items) {
+    Value v;
+    v.kind_ = Kind::List;
+    v.list_ = std::move(items);
+    return v;
 }
 
-Value& Value::operator[](std::string_view key) {
-    if (is_null()) v_ = Object{};
-    Object& o = as_object();
-    auto it = o.find(key);
-    if (it == o.end()) it = o.emplace(std::string(key), Value()).first;
-    return it->second;
+Value Value::map() {
+    Value v;
+    v.kind_ = Kind::Map;
+    return v;
 }
 
-void Value::push_back(Value v) {
-    if (is_null()) v_ = Array{};
-    as_array().push_back(std::move(v));
+namespace {
+[[noreturn]] void wrong_kind(const char* wanted) { throw std::logic_error(std::string("JSON value is not ") + wanted); }
+
+bool key_less(const Member& m, std::string_view k) { return std::string_view(m.first) < k; }
+}  // namespace
+
+bool Value::boolean() const {
+    if (kind_ != Kind::Boolean) wrong_kind("a boolean");
+    return flag_;
+}
+double Value::number() const {
+    if (kind_ != Kind::Number) wrong_kind("a number");
+    return num_;
+}
+const std::string& Value::text() const {
+    if (kind_ != Kind::Text) wrong_kind("a string");
+    return text_;
+}
+const std::vector<Value>& Value::items() const {
+    if (kind_ != Kind::List) wrong_kind("an array");
+    return list_;
+}
+const std::vector<Member>& Value::members() const {
+    if (kind_ != Kind::Map) wrong_kind("an object");
+    return map_;
+}
+
+const Value* Value::get(std::string_view key) const {
+    if (kind_ != Kind::Map) return nullptr;
+    auto it = std::lower_bound(map_.begin(), map_.end(), key, key_less);
+    return (it != map_.end() && it->first == key) ? &it->second : nullptr;
+}
+
+Value& Value::set(std::string_view key, Value v) {
+    if (kind_ == Kind::Null) kind_ = Kind::Map;
+    if (kind_ != Kind::Map) wrong_kind("an object");
+    auto it = std::lower_bound(map_.begin(), map_.end(), key, key_less);
+    if (it != map_.end() && it->first == key) {
+        it->second = std::move(v);
+        return it->second;
+    }
+    return map_.insert(it, Member(std::string(key), std::move(v)))->second;
+}
+
+Value& Value::append(Value v) {
+    if (kind_ == Kind::Null) kind_ = Kind::List;
+    if (kind_ != Kind::List) wrong_kind("an array");
+    list_.push_back(std::move(v));
+    return list_.back();
+}
+
+size_t Value::size() const { return kind_ == Kind::List ? list_.size() : kind_ == Kind::Map ? map_.size() : 0; }
+
+bool Value::operator==(const Value& o) const {
+    if (kind_ != o.kind_) return false;
+    switch (kind_) {
+        case Kind::Null: return true;
+        case Kind::Boolean: return flag_ == o.flag_;
+        case Kind::Number: return num_ == o.num_;
+        case Kind::Text: return text_ == o.text_;
+        case Kind::List: return list_ == o.list_;
+        case Kind::Map: return map_ == o.map_;
+    }
+    return false;
+}
+
+SyntaxError::SyntaxError(size_t at, const std::string& why)
+    : std::runtime_error("invalid JSON at offset " + std::to_string(at) + ": " + why), at_(at) {}
+
+// ---- reading ---------------------------------------------------------------------------------
+namespace {
+
+struct Reader {
+    std::string_view src;
+    size_t pos = 0;
+
+    [[noreturn]] void stop(const std::string& why) const { throw SyntaxError(pos, why); }
+    bool done() const { return pos >= src.size(); }
+    char peek() const { return done() ? '\0' : src[pos]; }
+
+    void blanks() {
+        while (!done()) {
+            const char c = src[pos];
+            if (c != ' ' && c != '\t' && c != '\n' && c != '\r') break;
+            ++pos;
+        }
+    }
+
+    void word(std::string_view w) {
+        if (src.substr(pos, w.size()) != w) stop("unrecognised token");
+        pos += w.size();
+    }
+
+    Value value(int depth) {
+        blanks();
+        switch (peek()) {
+            case '{': return map(depth + 1);
+            case '[': return list(depth + 1);
+            case '"': return Value(text());
+            case 't': word("true"); return Value(true);
+            case 'f': word("false"); return Value(false);
+            case 'n': word("null"); return Value();
+            case '\0':
+                if (done()) stop("the text ends where a value should begin");
+                stop("unexpected character");
+            default: return Value(number());
+        }
+    }
+
+    Value map(int depth) {
+        if (depth > kMaxDepth) stop("containers nested more than " + std::to_string(kMaxDepth) + " deep");
+        ++pos;  // '{'
+        Value out = Value::map();
+        blanks();
+        if (peek() == '}') {
+            ++pos;
+            return out;
+        }
+        for (;;) {
+            blanks();
+            if (peek() != '"') stop("an object key must be a string");
+            std::string key = text();
+            blanks();
+            if (peek() != ':') stop("':' expected after an object key");
+            ++pos;
+            out.set(key, value(depth));
+            blanks();
+            const char c = peek();
+            ++pos;
+            if (c == '}') return out;
+            if (c != ',') {
+                --pos;
+                stop("',' or '}' expected in an object");
+            }
+        }
+    }
+
+    Value list(int depth) {
+        if (depth > kMaxDepth) stop("containers nested more than " + std::to_string(kMaxDepth) + " deep");
+        ++pos;  // '['
+        Value out = Value::list();
+        blanks();
+        if (peek() == ']') {
+            ++pos;
+            return out;
+        }
+        for (;;) {
+            out.append(value(depth));
+            blanks();
+            const char c = peek();
+            ++pos;
+            if (c == ']') return out;
+            if (c != ',') {
+                --pos;
+                stop("',' or ']' expected in an array");
+            }
+        }
+    }
+
+    // Numbers: -? (0 | [1-9][0-9]*) (. [0-9]+)? ([eE] [+-]? [0-9]+)?
+    double number() {
+        const size_t start = pos;
+        auto digits = [&] {
+            const size_t d0 = pos;
+            while (!done() && src[pos] >= '0' && src[pos] <= '9') ++pos;
+            return pos - d0;
+        };
+        if (peek() == '-') ++pos;
+        if (peek() == '0') {
+            ++pos;
+        } else if (digits() == 0) {
+            pos = start;
+            stop("unexpected character");
+        }
+        if (peek() == '.') {
+            ++pos;
+            if (digits() == 0) stop("a digit must follow the decimal point");
+        }
+        if (peek() == 'e' || peek() == 'E') {
+            ++pos;
+            if (peek() == '+' || peek() == '-') ++pos;
+            if (digits() == 0) stop("an exponent needs digits");
+        }
+        double d = 0.0;
+        const std::string_view lexeme = src.substr(start, pos - start);
+        const auto r = std::from_chars(lexeme.data(), lexeme.data() + lexeme.size(), d);
+        if (r.ec == std::errc::result_out_of_range) {
+            // beyond the doubles: the nearest one, +-inf for a large magnitude, +-0 for a tiny one
+            const size_t e = lexeme.find_first_of("eE");
+            const bool tiny = e != std::string_view::npos && e + 1 < lexeme.size() && lexeme[e + 1] == '-';
+            const bool neg = lexeme.front() == '-';
+            d = tiny ? (neg ? -0.0 : 0.0) : (neg ? -HUGE_VAL : HUGE_VAL);
+        } else if (r.ec != std::errc()) {
+            stop("malformed number");
+        }
+        return d;
+    }
+
+    uint32_t hex4() {
+        if (src.size() - pos < 4) stop("\\u needs four hex digits");
+        uint32_t v = 0;
+        for (int i = 0; i < 4; ++i) {
+            const char c = src[pos++];
+            v <<= 4;
+            if (c >= '0' && c <= '9') v |= uint32_t(c - '0');
+            else if (c >= 'a' && c <= 'f') v |= uint32_t(c - 'a' + 10);
+            else if (c >= 'A' && c <= 'F') v |= uint32_t(c - 'A' + 10);
+            else stop("\\u needs four hex digits");
+        }
+        return v;
+    }
+
+    static void utf8(std::string& out, uint32_t cp) {
+        if (cp < 0x80) {
+            out += char(cp);
+        } else if (cp < 0x800) {
+            out += char(0xC0 | (cp >> 6));
+            out += char(0x80 | (cp & 0x3F));
+        } else if (cp < 0x10000) {
+            out += char(0xE0 | (cp >> 12));
+            out += char(0x80 | ((cp >> 6) & 0x3F));
+            out += char(0x80 | (cp & 0x3F));
+        } else {
+            out += char(0xF0 | (cp >> 18));
+            out += char(0x80 | ((cp >> 12) & 0x3F));
+            out += char(0x80 | ((cp >> 6) & 0x3F));
+            out += char(0x80 | (cp & 0x3F));
+        }
+    }
+
+    // One raw UTF-8 sequence, validated (no overlongs, no surrogates, at most U+10FFFF).
+    void raw_utf8(std::string& out) {
+        const unsigned char c = static_cast<unsigned char>(src[pos]);
+        int extra = 0;
+        uint32_t cp = 0, lo = 0;
+        if (c >= 0xC2 && c <= 0xDF) {
+            extra = 1;
+            cp = c & 0x1F;
+            lo = 0x80;
+        } else if (c >= 0xE0 && c <= 0xEF) {
+            extra = 2;
+            cp = c & 0x0F;
+            lo = 0x800;
+        } else if (c >= 0xF0 && c <= 0xF4) {
+            extra = 3;
+            cp = c & 0x07;
+            lo = 0x10000;
+        } else {
+            stop("invalid UTF-8 in a string");
+        }
+        if (src.size() - pos <= size_t(extra)) stop("invalid UTF-8 in a string");
+        for (int i = 1; i <= extra; ++i) {
+            const unsigned char d = static_cast<unsigned char>(src[pos + size_t(i)]);
+            if ((d & 0xC0) != 0x80) stop("invalid UTF-8 in a string");
+            cp = (cp << 6) | (d & 0x3F);
+        }
+        if (cp < lo || cp > 0x10FFFF || (cp >= 0xD800 && cp <= 0xDFFF)) stop("invalid UTF-8 in a string");
+        out.append(src.data() + pos, size_t(extra) + 1);
+        pos += size_t(extra) + 1;
+    }
+
+    std::string text() {
+        ++pos;  // '"'
+        std::string out;
+        for (;;) {
+            if (done()) stop("a string is not closed");
+            const unsigned char c = static_cast<unsigned char>(src[pos]);
+            if (c == '"') {
+                ++pos;
+                return out;
+            }
+            if (c < 0x20) stop("a control character inside a string");
+            if (c >= 0x80) {
+                raw_utf8(out);
+                continue;
+            }
+            if (c != '\\') {
+                out += char(c);
+                ++pos;
+                continue;
+            }
+            ++pos;
+            if (done()) stop("a string is not closed");
+            const char e = src[pos++];
+            switch (e) {
+                case '"': out += '"'; break;
+                case '\\': out += '\\'; break;
+                case '/': out += '/'; break;
+                case 'b': out += '\b'; break;
+                case 'f': out += '\f'; break;
+                case 'n': out += '\n'; break;
+                case 'r': out += '\r'; break;
+                case 't': out += '\t'; break;
+                case 'u': {
+                    uint32_t cp = hex4();
+                    if (cp >= 0xDC00 && cp <= 0xDFFF) stop("a low surrogate without a high one");
+                    if (cp >= 0xD800 && cp <= 0xDBFF) {
+                        if (src.substr(pos, 2) != "\\u") stop("a high surrogate without a low one");
+                        pos += 2;
+                        const uint32_t low = hex4();
+                        if (low < 0xDC00 || low > 0xDFFF) stop("a high surrogate without a low one");
+                        cp = 0x10000 + ((cp - 0xD800) << 10) + (low - 0xDC00);
+                    }
+                    utf8(out, cp);
+                    break;
+                }
+                default: --pos; stop("unknown escape in a string");
+            }
+        }
+    }
+};
+
+}  // namespace
+
+Value parse(std::string_view text) {
+    Reader r{text};
+    Value v = r.value(0);
+    r.blanks();
+    if (!r.done()) r.stop("text continues after the value");
+    return v;
+}
+
+// ---- writing ---------------------------------------------------------------------------------
+std::string number_text(double d) {
+    if (!std::isfinite(d)) return "null";
+    if (d == 0.0) return std::signbit(d) ? "-0" : "0";
+    char buf[64];
+    const auto r = std::to_chars(buf, buf + sizeof buf, d);
+    return std::string(buf, r.ptr);
 }
 
 namespace {
 
-locale_t c_locale() {
-    static locale_t loc = newlocale(LC_ALL_MASK, "C", nullptr);
-    return loc;
-}
-
-class Parser {
-public:
-    explicit Parser(std::string_view s) : s_(s) {}
-
-    Value run() {
-        check_utf8();
-        ws();
-        Value v = value(0);
-        ws();
-        if (i_ != s_.size()) err("unexpected data after the JSON value");
-        return v;
-    }
-
-private:
-    std::string_view s_;
-    size_t i_ = 0;
-
-    [[noreturn]] void err(const std::string& m) const { throw ParseError(i_, m); }
-    [[noreturn]] void err_at(size_t at, const std::string& m) const { throw ParseError(at, m); }
-    bool eof() const { return i_ >= s_.size(); }
-
-    void check_utf8() const {
-        size_t i = 0;
-        const auto* p = reinterpret_cast<const unsigned char*>(s_.data());
-        while (i < s_.size()) {
-            const unsigned c = p[i];
-            size_t n = 0;
-            unsigned cp = 0;
-            if (c < 0x80) { ++i; continue; }
-            if ((c & 0xE0) == 0xC0) { n = 1; cp = c & 0x1F; }
-            else if ((c & 0xF0) == 0xE0) { n = 2; cp = c & 0x0F; }
-            else if ((c & 0xF8) == 0xF0) { n = 3; cp = c & 0x07; }
-            else throw ParseError(i, "invalid UTF-8");
-            for (size_t k = 1; k <= n; ++k) {
-                if (i + k >= s_.size() || (p[i + k] & 0xC0) != 0x80) throw ParseError(i, "invalid UTF-8");
-                cp = (cp << 6) | (p[i + k] & 0x3F);
-            }
-            if ((n == 1 && cp < 0x80) || (n == 2 && cp < 0x800) || (n == 3 && cp < 0x10000) || cp > 0x10FFFF ||
-                (cp >= 0xD800 && cp <= 0xDFFF))
-                throw ParseError(i, "invalid UTF-8");
-            i += n + 1;
-        }
-    }
-
-    void ws() {
-        while (!eof() && (s_[i_] == ' ' || s_[i_] == '\t' || s_[i_] == '\n' || s_[i_] == '\r')) ++i_;
-    }
-
-    void expect_word(const char* w) {
-        for (const char* p = w; *p; ++p) {
-            if (eof() || s_[i_] != *p) err(std::string("invalid literal, expected '") + w + "'");
-            ++i_;
-        }
-    }
-
-    Value value(int depth) {
-        if (eof()) err("unexpected end of input");
-        const char c = s_[i_];
-        if (c == '{') return object(depth + 1);
-        if (c == '[') return array(depth + 1);
-        if (c == '"') return Value(string());
-        if (c == 't') { expect_word("true"); return Value(true); }
-        if (c == 'f') { expect_word("false"); return Value(false); }
-        if (c == 'n') { expect_word("null"); return Value(); }
-        if (c == '-' || (c >= '0' && c <= '9')) return Value(number());
-        err("unexpected character");
-    }
-
-    Value object(int depth) {
-        if (depth > 64) err("nesting deeper than 64");
-        ++i_;
-        Object o;
-        ws();
-        if (!eof() && s_[i_] == '}') { ++i_; return Value(std::move(o)); }
-        for (;;) {
-            ws();
-            if (eof() || s_[i_] != '"') err("expected a string key");
-            std::string k = string();
-            ws();
-            if (eof() || s_[i_] != ':') err("expected ':'");
-            ++i_;
-            ws();
-            Value v = value(depth);
-            o[k] = std::move(v);
-            ws();
-            if (eof()) err("unexpected end of input in an object");
-            if (s_[i_] == ',') { ++i_; continue; }
-            if (s_[i_] == '}') { ++i_; break; }
-            err("expected ',' or '}'");
-        }
-        return Value(std::move(o));
-    }
-
-    Value array(int depth) {
-        if (depth > 64) err("nesting deeper than 64");
-        ++i_;
-        Array a;
-        ws();
-        if (!eof() && s_[i_] == ']') { ++i_; return Value(std::move(a)); }
-        for (;;) {
-            ws();
-            a.push_back(value(depth));
-            ws();
-            if (eof()) err("unexpected end of input in an array");
-            if (s_[i_] == ',') { ++i_; continue; }
-            if (s_[i_] == ']') { ++i_; break; }
-            err("expected ',' or ']'");
-        }
-        return Value(std::move(a));
-    }
-
-    static void put_utf8(std::string& o, unsigned cp) {
-        if (cp < 0x80) o += char(cp);
-        else if (cp < 0x800) { o += char(0xC0 | (cp >> 6)); o += char(0x80 | (cp & 0x3F)); }
-        else if (cp < 0x10000) {
-            o += char(0xE0 | (cp >> 12)); o += char(0x80 | ((cp >> 6) & 0x3F)); o += char(0x80 | (cp & 0x3F));
-        } else {
-            o += char(0xF0 | (cp >> 18)); o += char(0x80 | ((cp >> 12) & 0x3F));
-            o += char(0x80 | ((cp >> 6) & 0x3F)); o += char(0x80 | (cp & 0x3F));
-        }
-    }
-
-    unsigned hex4() {
-        unsigned v = 0;
-        for (int k = 0; k < 4; ++k) {
-            if (eof()) err("truncated \\u escape");
-            const char c = s_[i_++];
-            v <<= 4;
-            if (c >= '0' && c <= '9') v |= unsigned(c - '0');
-            else if (c >= 'a' && c <= 'f') v |= unsigned(c - 'a' + 10);
-            else if (c >= 'A' && c <= 'F') v |= unsigned(c - 'A' + 10);
-            else err("invalid \\u escape");
-        }
-        return v;
-    }
-
-    std::string string() {
-        ++i_;
-        std::string o;
-        for (;;) {
-            if (eof()) err("unterminated string");
-            const unsigned char c = static_cast<unsigned char>(s_[i_]);
-            if (c == '"') { ++i_; break; }
-            if (c < 0x20) err("control character in a string");
-            if (c != '\\') { o += char(c); ++i_; continue; }
-            ++i_;
-            if (eof()) err("unterminated escape");
-            const char e = s_[i_++];
-            switch (e) {
-                case '"': o += '"'; break;
-                case '\\': o += '\\'; break;
-                case '/': o += '/'; break;
-                case 'b': o += '\b'; break;
-                case 'f': o += '\f'; break;
-                case 'n': o += '\n'; break;
-                case 'r': o += '\r'; break;
-                case 't': o += '\t'; break;
-                case 'u': {
-                    unsigned cp = hex4();
-                    if (cp >= 0xD800 && cp <= 0xDBFF) {
-                        if (i_ + 1 >= s_.size() || s_[i_] != '\\' || s_[i_ + 1] != 'u') err("unpaired surrogate");
-                        i_ += 2;
-                        const unsigned lo = hex4();
-                        if (lo < 0xDC00 || lo > 0xDFFF) err("unpaired surrogate");
-                        cp = 0x10000 + ((cp - 0xD800) << 10) + (lo - 0xDC00);
-                    } else if (cp >= 0xDC00 && cp <= 0xDFFF) {
-                        err("unpaired surrogate");
-                    }
-                    put_utf8(o, cp);
-                    break;
-                }
-                default: err("invalid escape");
-            }
-        }
-        return o;
-    }
-
-    double number() {
-        const size_t start = i_;
-        if (s_[i_] == '-') ++i_;
-        if (eof()) err("invalid number");
-        if (s_[i_] == '0') {
-            ++i_;
-        } else if (s_[i_] >= '1' && s_[i_] <= '9') {
-            while (!eof() && s_[i_] >= '0' && s_[i_] <= '9') ++i_;
-        } else {
-            err("invalid number");
-        }
-        if (!eof() && s_[i_] == '.') {
-            ++i_;
-            if (eof() || s_[i_] < '0' || s_[i_] > '9') err("invalid number: digit expected after '.'");
-            while (!eof() && s_[i_] >= '0' && s_[i_] <= '9') ++i_;
-        }
-        if (!eof() && (s_[i_] == 'e' || s_[i_] == 'E')) {
-            ++i_;
-            if (!eof() && (s_[i_] == '+' || s_[i_] == '-')) ++i_;
-            if (eof() || s_[i_] < '0' || s_[i_] > '9') err("invalid number: digit expected in exponent");
-            while (!eof() && s_[i_] >= '0' && s_[i_] <= '9') ++i_;
-        }
-        const std::string tmp(s_.substr(start, i_ - start));
-        char* end = nullptr;
-        const double v = strtod_l(tmp.c_str(), &end, c_locale());
-        if (end != tmp.c_str() + tmp.size()) err_at(start, "invalid number");
-        if (std::isinf(v)) err_at(start, "number out of range");
-        return v;
-    }
-};
-
-void append_string(std::string& out, const std::string& s) {
+void quote(std::string& out, const std::string& s) {
+    static const char* hex = "0123456789abcdef";
     out += '"';
     for (const char ch : s) {
         const unsigned char c = static_cast<unsigned char>(ch);
@@ -258,9 +371,9 @@ void append_string(std::string& out, const std::string& s) {
             case '\f': out += "\\f"; break;
             default:
                 if (c < 0x20) {
-                    char buf[8];
-                    std::snprintf(buf, sizeof buf, "\\u%04x", c);
-                    out += buf;
+                    out += "\\u00";
+                    out += hex[c >> 4];
+                    out += hex[c & 15];
                 } else {
                     out += ch;
                 }
@@ -269,32 +382,32 @@ void append_string(std::string& out, const std::string& s) {
     out += '"';
 }
 
-void append_value(std::string& out, const Value& v) {
-    switch (v.type()) {
-        case Type::Null: out += "null"; break;
-        case Type::Bool: out += v.as_bool() ? "true" : "false"; break;
-        case Type::Number: out += format_number(v.as_number()); break;
-        case Type::String: append_string(out, v.as_string()); break;
-        case Type::Array: {
+void emit(std::string& out, const Value& v) {
+    switch (v.kind()) {
+        case Value::Kind::Null: out += "null"; break;
+        case Value::Kind::Boolean: out += v.boolean() ? "true" : "false"; break;
+        case Value::Kind::Number: out += number_text(v.number()); break;
+        case Value::Kind::Text: quote(out, v.text()); break;
+        case Value::Kind::List: {
             out += '[';
             bool first = true;
-            for (const Value& e : v.as_array()) {
+            for (const Value& e : v.items()) {
                 if (!first) out += ',';
                 first = false;
-                append_value(out, e);
+                emit(out, e);
             }
             out += ']';
             break;
         }
-        case Type::Object: {
+        case Value::Kind::Map: {
             out += '{';
             bool first = true;
-            for (const auto& [k, e] : v.as_object()) {  // std::map: ascending bytewise order
+            for (const Member& m : v.members()) {
                 if (!first) out += ',';
                 first = false;
-                append_string(out, k);
+                quote(out, m.first);
                 out += ':';
-                append_value(out, e);
+                emit(out, m.second);
             }
             out += '}';
             break;
@@ -304,20 +417,9 @@ void append_value(std::string& out, const Value& v) {
 
 }  // namespace
 
-Value parse(std::string_view text) { return Parser(text).run(); }
-
-std::string format_number(double d) {
-    if (!std::isfinite(d)) return "null";
-    char buf[40];
-    const locale_t old = uselocale(c_locale());
-    std::snprintf(buf, sizeof buf, "%.17g", d);
-    uselocale(old);
-    return buf;
-}
-
-std::string write(const Value& v) {
+std::string dump(const Value& v) {
     std::string out;
-    append_value(out, v);
+    emit(out, v);
     return out;
 }
 

@@ -1,84 +1,94 @@
-// Strict JSON (RFC 8259) reader and compact writer.
+// A small JSON document model for the engine's option strings, parameter deltas, resource files
+// and replies.
 //
-// Wire behaviour follows the reference's src/util/json.hpp:1-30: strict reader (no NaN/Infinity,
-// depth <= 64, duplicate keys: last wins, whitespace only after the value, byte offset in
-// errors); writer with object keys in ascending bytewise order, numbers as "%.17g", non-finite
-// numbers as null.
+// Reading is strict RFC 8259: one value, whitespace around it, no comments, no NaN or Infinity
+// literals, strings valid UTF-8 after escapes, at most kMaxDepth nested containers.  A repeated
+// object key keeps its last value.  Writing is compact, objects in ascending bytewise key order,
+// numbers in the shortest form that reads back to the same double (non-finite numbers as null).
 #pragma once
 
 #include <cstddef>
-#include <map>
 #include <stdexcept>
 #include <string>
 #include <string_view>
-#include <variant>
+#include <utility>
 #include <vector>
 
 namespace spk::json {
 
-class Value;
-using Array = std::vector<Value>;
-using Object = std::map<std::string, Value, std::less<>>;
+inline constexpr int kMaxDepth = 64;
 
-enum class Type : unsigned char { Null, Bool, Number, String, Array, Object };
+class Value;
+using Member = std::pair<std::string, Value>;
 
 class Value {
 public:
+    enum class Kind : unsigned char { Null, Boolean, Number, Text, List, Map };
+
     Value() = default;
     Value(std::nullptr_t) {}
-    Value(bool b) : v_(b) {}
-    Value(double d) : v_(d) {}
-    Value(int i) : v_(double(i)) {}
-    Value(unsigned i) : v_(double(i)) {}
-    Value(long i) : v_(double(i)) {}
-    Value(unsigned long i) : v_(double(i)) {}
-    Value(long long i) : v_(double(i)) {}
-    Value(unsigned long long i) : v_(double(i)) {}
-    Value(const char* s) : v_(std::string(s)) {}
-    Value(std::string s) : v_(std::move(s)) {}
-    Value(std::string_view s) : v_(std::string(s)) {}
-    Value(Array a) : v_(std::move(a)) {}
-    Value(Object o) : v_(std::move(o)) {}
+    Value(bool b) : kind_(Kind::Boolean), flag_(b) {}
+    Value(double d) : kind_(Kind::Number), num_(d) {}
+    Value(int i) : Value(double(i)) {}
+    Value(unsigned i) : Value(double(i)) {}
+    Value(long i) : Value(double(i)) {}
+    Value(unsigned long i) : Value(double(i)) {}
+    Value(long long i) : Value(double(i)) {}
+    Value(unsigned long long i) : Value(double(i)) {}
+    Value(const char* s) : kind_(Kind::Text), text_(s) {}
+    Value(std::string s) : kind_(Kind::Text), text_(std::move(s)) {}
+    Value(std::string_view s) : kind_(Kind::Text), text_(s) {}
 
-    Type type() const { return Type(v_.index()); }
-    bool is_null() const { return type() == Type::Null; }
-    bool is_bool() const { return type() == Type::Bool; }
-    bool is_number() const { return type() == Type::Number; }
-    bool is_string() const { return type() == Type::String; }
-    bool is_array() const { return type() == Type::Array; }
-    bool is_object() const { return type() == Type::Object; }
+    static Value list(std::vector<Value> items = {});
+    static Value map();
 
-    bool as_bool() const { return std::get<bool>(v_); }
-    double as_number() const { return std::get<double>(v_); }
-    const std::string& as_string() const { return std::get<std::string>(v_); }
-    const Array& as_array() const { return std::get<Array>(v_); }
-    Array& as_array() { return std::get<Array>(v_); }
-    const Object& as_object() const { return std::get<Object>(v_); }
-    Object& as_object() { return std::get<Object>(v_); }
+    Kind kind() const { return kind_; }
+    bool is_null() const { return kind_ == Kind::Null; }
+    bool is_bool() const { return kind_ == Kind::Boolean; }
+    bool is_number() const { return kind_ == Kind::Number; }
+    bool is_text() const { return kind_ == Kind::Text; }
+    bool is_list() const { return kind_ == Kind::List; }
+    bool is_map() const { return kind_ == Kind::Map; }
 
-    // nullptr when this is not an object or the key is absent.
-    const Value* find(std::string_view key) const;
-    // A null value becomes an empty object first.
-    Value& operator[](std::string_view key);
-    void push_back(Value v);
-    bool operator==(const Value& o) const { return v_ == o.v_; }
+    // Typed reads; a read of the wrong kind throws std::logic_error.
+    bool boolean() const;
+    double number() const;
+    const std::string& text() const;
+    const std::vector<Value>& items() const;     // list elements
+    const std::vector<Member>& members() const;  // map members, ascending by key
+
+    // Maps: the member's value or null when absent (or when this is not a map).
+    const Value* get(std::string_view key) const;
+    // Maps (a null value becomes an empty map): inserts or replaces, keeping keys sorted.
+    Value& set(std::string_view key, Value v);
+    // Lists (a null value becomes an empty list).
+    Value& append(Value v);
+    size_t size() const;  // elements of a list, members of a map, else 0
+
+    bool operator==(const Value& o) const;
 
 private:
-    std::variant<std::monostate, bool, double, std::string, Array, Object> v_;
+    Kind kind_ = Kind::Null;
+    bool flag_ = false;
+    double num_ = 0.0;
+    std::string text_;
+    std::vector<Value> list_;
+    std::vector<Member> map_;
 };
 
-class ParseError : public std::runtime_error {
+// A syntax error with the byte offset where reading stopped.
+class SyntaxError : public std::runtime_error {
 public:
-    ParseError(size_t offset, const std::string& what)
-        : std::runtime_error("JSON parse error at byte " + std::to_string(offset) + ": " + what), offset_(offset) {}
-    size_t offset() const { return offset_; }
+    SyntaxError(size_t at, const std::string& why);
+    size_t at() const { return at_; }
 
 private:
-    size_t offset_;
+    size_t at_;
 };
 
 Value parse(std::string_view text);
-std::string write(const Value& v);
-std::string format_number(double d);
+std::string dump(const Value& v);
+// The shortest decimal form that reads back to d; "null" when d is not finite.
+std::string number_text(double d);
 
 }  // namespace spk::json

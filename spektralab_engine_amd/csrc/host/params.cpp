@@ -1,4 +1,4 @@
-// Schema rows, ParamSet and delta validation; rules stated in params.h.
+// The parameter schema, values and delta validation; rules stated in params.h.
 #include "params.h"
 
 #include <cmath>
@@ -10,202 +10,214 @@ namespace spk::params {
 
 namespace {
 
-using R = std::optional<std::pair<double, double>>;
+constexpr bool S = true, PR = false;  // layer: shoot / print
+constexpr double kNo = 0.0;
 
-FieldDecl row(const char* name, const char* path, Type t, R range, json::Value def, Layer l, bool live = false) {
-    FieldDecl f;
-    f.name = name;
-    f.path = path;
-    f.type = t;
-    f.range = range;
-    f.def = std::move(def);
-    f.layer = l;
-    f.live = live;
-    return f;
+constexpr Row num(const char* name, const char* path, Kind k, double lo, double hi, double def, bool shoot,
+                  bool live = false) {
+    return Row{name, path, k, true, lo, hi, def, nullptr, shoot, live};
+}
+constexpr Row flag(const char* name, const char* path, bool def, bool shoot) {
+    return Row{name, path, Kind::Bool, false, kNo, kNo, def ? 1.0 : 0.0, nullptr, shoot, false};
+}
+constexpr Row str(const char* name, const char* path, const char* def, bool shoot) {
+    return Row{name, path, Kind::Str, false, kNo, kNo, kNo, def, shoot, false};
 }
 
-const char* type_name(Type t) {
-    switch (t) {
-        case Type::Float: return "float";
-        case Type::Int: return "int";
-        case Type::Bool: return "bool";
-        case Type::Str: return "str";
+// docs/API_SPEC.md section 3, row for row.
+constexpr std::array<Row, kFields> kRows = {{
+    str("film_stock", "film.info.stock", "kodak_portra_400", S),
+    str("print_stock", "print.info.stock", "kodak_portra_endura", PR),
+    num("exposure_compensation_ev", "camera.exposure_compensation_ev", Kind::Float, -8, 8, 0, S),
+    flag("auto_exposure", "camera.auto_exposure", true, S),
+    str("auto_exposure_method", "camera.auto_exposure_method", "center_weighted", S),
+    num("film_format_mm", "camera.film_format_mm", Kind::Float, 4, 200, 35, S),
+    num("lens_blur_um", "camera.lens_blur_um", Kind::Float, 0, 200, 0, S),
+    flag("halation_active", "film_render.halation.active", true, S),
+    num("halation_amount", "film_render.halation.halation_amount", Kind::Float, 0, 4, 1, S),
+    num("halation_boost_ev", "film_render.halation.boost_ev", Kind::Float, 0, 8, 0, S),
+    flag("grain_active", "film_render.grain.active", true, S),
+    flag("grain_sublayers_active", "film_render.grain.sublayers_active", true, S),
+    flag("dir_couplers_active", "film_render.dir_couplers.active", true, S),
+    num("dir_couplers_amount", "film_render.dir_couplers.amount", Kind::Float, 0, 4, 1, S),
+    num("density_curve_gamma", "film_render.density_curve_gamma", Kind::Float, 0.2, 4, 1, S),
+    str("input_color_space", "io.input_color_space", "ProPhoto RGB", S),
+    flag("input_cctf_decoding", "io.input_cctf_decoding", false, S),
+    num("geometry_crop_x", "io.geometry.crop_x", Kind::Float, 0, 1, 0, S),
+    num("geometry_crop_y", "io.geometry.crop_y", Kind::Float, 0, 1, 0, S),
+    num("geometry_crop_w", "io.geometry.crop_w", Kind::Float, 1e-4, 1, 1, S),
+    num("geometry_crop_h", "io.geometry.crop_h", Kind::Float, 1e-4, 1, 1, S),
+    num("geometry_rotation_deg", "io.geometry.rotation_deg", Kind::Float, -45, 45, 0, S),
+    num("geometry_quarter_turns", "io.geometry.quarter_turns", Kind::Int, 0, 3, 0, S),
+    flag("geometry_flip_h", "io.geometry.flip_h", false, S),
+    flag("geometry_flip_v", "io.geometry.flip_v", false, S),
+    num("print_exposure", "enlarger.print_exposure", Kind::Float, 0.05, 20, 1, PR, true),
+    num("m_filter_shift", "enlarger.m_filter_shift", Kind::Float, -1, 1, 0, PR, true),
+    num("y_filter_shift", "enlarger.y_filter_shift", Kind::Float, -1, 1, 0, PR, true),
+    num("c_filter_neutral", "enlarger.c_filter_neutral", Kind::Float, 0, 200, 0, PR),
+    num("m_filter_neutral", "enlarger.m_filter_neutral", Kind::Float, 0, 200, 65, PR),
+    num("y_filter_neutral", "enlarger.y_filter_neutral", Kind::Float, 0, 200, 55, PR),
+    num("preflash_exposure", "enlarger.preflash_exposure", Kind::Float, 0, 1, 0, PR, true),
+    str("enlarger_illuminant", "enlarger.illuminant", "TH-KG3", PR),
+    flag("glare_active", "print_render.glare.active", true, PR),
+    flag("scanner_white_correction", "scanner.white_correction", false, PR),
+    flag("scanner_black_correction", "scanner.black_correction", false, PR),
+    num("scanner_lens_blur", "scanner.lens_blur", Kind::Float, 0, 20, 0, PR),
+    str("output_color_space", "io.output_color_space", "ProPhoto RGB", PR),
+    flag("output_cctf_encoding", "io.output_cctf_encoding", true, PR),
+    flag("scan_film", "io.scan_film", false, PR),
+    flag("extended_dynamic_range", "print_render.edr_enabled", false, PR),
+    num("halation_scatter_amount", "film_render.halation.scatter_amount", Kind::Float, 0, 1, 1, S),
+    num("grain_amount", "film_render.grain.amount", Kind::Float, 0, 2, 1, S),
+    num("glare_amount", "print_render.glare.amount", Kind::Float, 0, 30, 1, PR),
+    flag("antihalation_removed", "film_render.halation.antihalation_removed", false, S),
+    num("filter_shift_scale", "enlarger.filter_shift_scale", Kind::Float, 1, 60, 1, PR, true),
+    num("halation_headroom_ev", "film_render.halation.headroom_ev", Kind::Float, 0, 8, 4, S),
+    num("preview_long_edge", "io.preview_long_edge", Kind::Int, 800, 8192, 1600, PR),
+}};
+
+// The accepted values of auto_exposure_method, in the order its message lists them.
+constexpr std::array<const char*, 7> kMeterMethods = {"center_weighted", "average",          "median",
+                                                      "balanced",        "center",           "protect_highlights",
+                                                      "protect_shadows"};
+
+const char* kind_word(Kind k) {
+    switch (k) {
+        case Kind::Float: return "float";
+        case Kind::Int: return "int";
+        case Kind::Bool: return "bool";
+        case Kind::Str: return "str";
     }
     return "?";
 }
 
-std::vector<FieldDecl> make_rows() {
-    const R none = std::nullopt;
-    auto rg = [](double lo, double hi) { return R(std::make_pair(lo, hi)); };
-    constexpr Layer S = Layer::Shoot, Pr = Layer::Print;
-    std::vector<FieldDecl> v = {
-        row("film_stock", "film.info.stock", Type::Str, none, "kodak_portra_400", S),
-        row("print_stock", "print.info.stock", Type::Str, none, "kodak_portra_endura", Pr),
-        row("exposure_compensation_ev", "camera.exposure_compensation_ev", Type::Float, rg(-8, 8), 0, S),
-        row("auto_exposure", "camera.auto_exposure", Type::Bool, none, true, S),
-        row("auto_exposure_method", "camera.auto_exposure_method", Type::Str, none, "center_weighted", S),
-        row("film_format_mm", "camera.film_format_mm", Type::Float, rg(4, 200), 35, S),
-        row("lens_blur_um", "camera.lens_blur_um", Type::Float, rg(0, 200), 0, S),
-        row("halation_active", "film_render.halation.active", Type::Bool, none, true, S),
-        row("halation_amount", "film_render.halation.halation_amount", Type::Float, rg(0, 4), 1, S),
-        row("halation_boost_ev", "film_render.halation.boost_ev", Type::Float, rg(0, 8), 0, S),
-        row("grain_active", "film_render.grain.active", Type::Bool, none, true, S),
-        row("grain_sublayers_active", "film_render.grain.sublayers_active", Type::Bool, none, true, S),
-        row("dir_couplers_active", "film_render.dir_couplers.active", Type::Bool, none, true, S),
-        row("dir_couplers_amount", "film_render.dir_couplers.amount", Type::Float, rg(0, 4), 1, S),
-        row("density_curve_gamma", "film_render.density_curve_gamma", Type::Float, rg(0.2, 4), 1, S),
-        row("input_color_space", "io.input_color_space", Type::Str, none, "ProPhoto RGB", S),
-        row("input_cctf_decoding", "io.input_cctf_decoding", Type::Bool, none, false, S),
-        row("geometry_crop_x", "io.geometry.crop_x", Type::Float, rg(0, 1), 0, S),
-        row("geometry_crop_y", "io.geometry.crop_y", Type::Float, rg(0, 1), 0, S),
-        row("geometry_crop_w", "io.geometry.crop_w", Type::Float, rg(1e-4, 1), 1, S),
-        row("geometry_crop_h", "io.geometry.crop_h", Type::Float, rg(1e-4, 1), 1, S),
-        row("geometry_rotation_deg", "io.geometry.rotation_deg", Type::Float, rg(-45, 45), 0, S),
-        row("geometry_quarter_turns", "io.geometry.quarter_turns", Type::Int, rg(0, 3), 0, S),
-        row("geometry_flip_h", "io.geometry.flip_h", Type::Bool, none, false, S),
-        row("geometry_flip_v", "io.geometry.flip_v", Type::Bool, none, false, S),
-        row("print_exposure", "enlarger.print_exposure", Type::Float, rg(0.05, 20), 1, Pr, true),
-        row("m_filter_shift", "enlarger.m_filter_shift", Type::Float, rg(-1, 1), 0, Pr, true),
-        row("y_filter_shift", "enlarger.y_filter_shift", Type::Float, rg(-1, 1), 0, Pr, true),
-        row("c_filter_neutral", "enlarger.c_filter_neutral", Type::Float, rg(0, 200), 0, Pr),
-        row("m_filter_neutral", "enlarger.m_filter_neutral", Type::Float, rg(0, 200), 65, Pr),
-        row("y_filter_neutral", "enlarger.y_filter_neutral", Type::Float, rg(0, 200), 55, Pr),
-        row("preflash_exposure", "enlarger.preflash_exposure", Type::Float, rg(0, 1), 0, Pr, true),
-        row("enlarger_illuminant", "enlarger.illuminant", Type::Str, none, "TH-KG3", Pr),
-        row("glare_active", "print_render.glare.active", Type::Bool, none, true, Pr),
-        row("scanner_white_correction", "scanner.white_correction", Type::Bool, none, false, Pr),
-        row("scanner_black_correction", "scanner.black_correction", Type::Bool, none, false, Pr),
-        row("scanner_lens_blur", "scanner.lens_blur", Type::Float, rg(0, 20), 0, Pr),
-        row("output_color_space", "io.output_color_space", Type::Str, none, "ProPhoto RGB", Pr),
-        row("output_cctf_encoding", "io.output_cctf_encoding", Type::Bool, none, true, Pr),
-        row("scan_film", "io.scan_film", Type::Bool, none, false, Pr),
-        row("extended_dynamic_range", "print_render.edr_enabled", Type::Bool, none, false, Pr),
-        row("halation_scatter_amount", "film_render.halation.scatter_amount", Type::Float, rg(0, 1), 1, S),
-        row("grain_amount", "film_render.grain.amount", Type::Float, rg(0, 2), 1, S),
-        row("glare_amount", "print_render.glare.amount", Type::Float, rg(0, 30), 1, Pr),
-        row("antihalation_removed", "film_render.halation.antihalation_removed", Type::Bool, none, false, S),
-        row("filter_shift_scale", "enlarger.filter_shift_scale", Type::Float, rg(1, 60), 1, Pr, true),
-        row("halation_headroom_ev", "film_render.halation.headroom_ev", Type::Float, rg(0, 8), 4, S),
-        row("preview_long_edge", "io.preview_long_edge", Type::Int, rg(800, 8192), 1600, Pr),
-    };
-    static const std::vector<std::string> methods = {"center_weighted", "average", "median", "balanced",
-                                                     "center", "protect_highlights", "protect_shadows"};
-    v[id(P::auto_exposure_method)].check_enum = [](std::string_view s) -> std::optional<std::string> {
-        std::string list;
-        for (const std::string& m : methods) {
-            if (m == s) return std::nullopt;
-            list += (list.empty() ? "" : ", ") + m;
-        }
-        return "'auto_exposure_method' = '" + std::string(s) + "' is not one of " + list;
-    };
-    return v;
-}
-
-std::string fmt_f(double v) {
-    char buf[64];
+std::string fixed(double v) {  // printf's %f, the form of the range message
+    char buf[512];
     std::snprintf(buf, sizeof buf, "%f", v);
     return buf;
 }
 
-bool type_matches(Type t, const json::Value& v) {
-    switch (t) {
-        case Type::Float:
-        case Type::Int: return v.is_number();
-        case Type::Bool: return v.is_bool();
-        case Type::Str: return v.is_string();
+json::Value default_json(const Row& r) {
+    switch (r.kind) {
+        case Kind::Str: return json::Value(r.text);
+        case Kind::Bool: return json::Value(r.number != 0.0);
+        default: return json::Value(r.number);
     }
-    return false;
+}
+
+// Checks one delta entry against its row; returns the value to store.
+Slot checked(const Row& r, const json::Value& v) {
+    const std::string name = r.name;
+    const bool kind_ok = r.kind == Kind::Str ? v.is_text() : r.kind == Kind::Bool ? v.is_bool() : v.is_number();
+    if (!kind_ok) fail_user("'" + name + "' expects " + kind_word(r.kind));
+    Slot out;
+    if (r.kind == Kind::Str) {
+        out.str = v.text();
+        if (std::string_view(r.name) == "auto_exposure_method") {
+            bool known = false;
+            std::string list;
+            for (const char* m : kMeterMethods) {
+                known = known || out.str == m;
+                list += list.empty() ? m : std::string(", ") + m;
+            }
+            if (!known) fail_user("'auto_exposure_method' = '" + out.str + "' is not one of " + list);
+        }
+        return out;
+    }
+    if (r.kind == Kind::Bool) {
+        out.num = v.boolean() ? 1.0 : 0.0;
+        return out;
+    }
+    const double x = v.number();
+    if (std::isnan(x)) fail_user("'" + name + "' expects " + kind_word(r.kind) + ", not NaN");
+    if (r.ranged && !(x >= r.lo && x <= r.hi))
+        fail_user("'" + name + "'=" + fixed(x) + " is outside [" + fixed(r.lo) + ", " + fixed(r.hi) + "]");
+    out.num = r.kind == Kind::Int ? std::round(x) : x;  // ints: half away from zero, after the range check
+    return out;
 }
 
 }  // namespace
 
-std::shared_ptr<const Schema> Schema::build() {
-    static const std::vector<FieldDecl> rows = make_rows();
-    auto s = std::make_shared<Schema>();
-    s->fields_ = rows;
-    json::Array arr;
-    for (const FieldDecl& f : s->fields_) {
-        json::Value r;
-        r["name"] = f.name;
-        r["path"] = f.path;
-        r["type"] = type_name(f.type);
-        r["layer"] = f.layer == Layer::Shoot ? "shoot" : "print";
-        r["default"] = f.def;
-        r["live"] = f.live;
-        if (f.range) r["range"] = json::Array{f.range->first, f.range->second};
-        arr.push_back(std::move(r));
-    }
-    json::Value doc;
-    doc["schema_version"] = 1;
-    doc["fields"] = std::move(arr);
-    s->json_ = json::write(doc);
-    return s;
-}
+const std::array<Row, kFields>& rows() { return kRows; }
 
-std::optional<size_t> Schema::find(std::string_view name) const {
-    for (size_t i = 0; i < fields_.size(); ++i)
-        if (fields_[i].name == name) return i;
+std::optional<P> lookup(std::string_view name) {
+    for (size_t i = 0; i < kFields; ++i)
+        if (name == kRows[i].name) return P(i);
     return std::nullopt;
 }
 
-ParamSet::ParamSet(std::shared_ptr<const Schema> s) : schema_(std::move(s)) {
-    for (size_t i = 0; i < schema_->size(); ++i) values_.push_back(schema_->field(i).def);
+const std::string& schema_json() {
+    static const std::string text = [] {
+        json::Value fields = json::Value::list();
+        for (const Row& r : kRows) {
+            json::Value f = json::Value::map();
+            f.set("default", default_json(r));
+            f.set("layer", r.shoot ? "shoot" : "print");
+            f.set("live", r.live);
+            f.set("name", r.name);
+            f.set("path", r.path);
+            if (r.ranged) f.set("range", json::Value::list({r.lo, r.hi}));
+            f.set("type", kind_word(r.kind));
+            fields.append(std::move(f));
+        }
+        json::Value doc = json::Value::map();
+        doc.set("fields", std::move(fields));
+        doc.set("schema_version", 1);
+        return json::dump(doc);
+    }();
+    return text;
 }
 
-json::Value ParamSet::to_json() const {
-    json::Value o = json::Object{};
-    for (size_t i = 0; i < values_.size(); ++i) o[schema_->field(i).name] = values_[i];
+Values::Values() {
+    for (size_t i = 0; i < kFields; ++i) {
+        slots_[i].num = kRows[i].number;
+        if (kRows[i].text) slots_[i].str = kRows[i].text;
+    }
+}
+
+json::Value Values::to_json() const {
+    json::Value o = json::Value::map();
+    for (size_t i = 0; i < kFields; ++i) {
+        const Row& r = kRows[i];
+        if (r.kind == Kind::Str) o.set(r.name, slots_[i].str);
+        else if (r.kind == Kind::Bool) o.set(r.name, slots_[i].num != 0.0);
+        else o.set(r.name, slots_[i].num);
+    }
     return o;
 }
 
-Delta validate_delta(const Schema& schema, std::string_view text, DeltaContext ctx) {
+Delta parse_delta(std::string_view text, Origin origin) {
     Delta d;
-    if (text.empty() && ctx == DeltaContext::Open) return d;
+    if (text.empty()) {
+        if (origin == Origin::Open) return d;
+        fail_user("params_delta must be an object");
+    }
     json::Value doc;
     try {
         doc = json::parse(text);
-    } catch (const json::ParseError& e) {
+    } catch (const json::SyntaxError& e) {
         fail_user(e.what());
     }
-    if (!doc.is_object()) fail_user("params_delta must be an object");
-    json::Object obj = doc.as_object();
-    if (ctx == DeltaContext::Open) {
-        if (auto it = obj.find("product_defaults"); it != obj.end()) {
-            if (!it->second.is_bool()) fail_user("'product_defaults' expects bool");
-            d.product_defaults = it->second.as_bool();
-            obj.erase(it);
+    if (!doc.is_map()) fail_user("params_delta must be an object");
+    for (const json::Member& m : doc.members()) {  // ascending bytewise by key
+        if (origin == Origin::Open && m.first == "product_defaults") {
+            if (!m.second.is_bool()) fail_user("'product_defaults' expects bool");
+            d.product_defaults = m.second.boolean();
+            continue;
         }
+        const std::optional<P> p = lookup(m.first);
+        if (!p) fail_user("unknown parameter '" + m.first + "'");
+        const Row& r = row(*p);
+        d.sets.emplace_back(*p, checked(r, m.second));
+        if (*p == P::film_stock || *p == P::print_stock) d.names_stock = true;
+        if (r.shoot) d.touches_shoot = true;
+        if (!r.live) d.needs_rebuild = true;
     }
-    for (const auto& [name, value] : obj) {  // std::map: ascending bytewise order
-        const std::optional<size_t> fid = schema.find(name);
-        if (!fid) fail_user("unknown parameter '" + name + "'");
-        const FieldDecl& f = schema.field(*fid);
-        if (!type_matches(f.type, value)) fail_user("'" + name + "' expects " + type_name(f.type));
-        json::Value stored = value;
-        if (f.type == Type::Str && f.check_enum)
-            if (auto msg = f.check_enum(value.as_string())) fail_user(*msg);
-        if (f.type == Type::Float || f.type == Type::Int) {
-            const double v = value.as_number();
-            if (f.range) {
-                const auto [lo, hi] = *f.range;
-                if (!(v >= lo && v <= hi))
-                    fail_user("'" + name + "'=" + fmt_f(v) + " is outside [" + fmt_f(lo) + ", " + fmt_f(hi) + "]");
-            } else if (std::isnan(v)) {
-                fail_user("'" + name + "' expects " + type_name(f.type) + ", not NaN");
-            }
-            if (f.type == Type::Int) stored = json::Value(std::round(v));
-        }
-        d.fields.emplace_back(*fid, std::move(stored));
-        if (name == "film_stock" || name == "print_stock") d.has_stock = true;
-        if (f.layer == Layer::Shoot) d.any_shoot = true;
-        if (!f.live) d.any_rebuild = true;
-    }
-    if (d.has_stock) d.any_rebuild = true;
     return d;
 }
 
-void apply_delta(ParamSet& p, const Delta& d) {
-    for (const auto& [fid, v] : d.fields) p.set(fid, v);
+void apply(Values& v, const Delta& d) {
+    for (const auto& [p, s] : d.sets) v.put(p, s);
 }
 
 }  // namespace spk::params

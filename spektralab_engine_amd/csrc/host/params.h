@@ -1,13 +1,12 @@
-// The parameter schema (48 rows), the flat parameter values and delta validation.
+// Render parameters: the 48-field schema of the ABI, one session's values, and parameter deltas.
 //
-// Rows, types, ranges, defaults, layers and live flags restate the reference's
-// src/params/schema.cpp:51-113; validation restates src/params/delta.cpp:40-96 (keys in ascending
-// bytewise order, first failure wins, NaN refused, ints rounded half away from zero).
+// The schema (names, dotted paths, types, ranges, defaults, layers, live flags, in params_schema
+// order) is the data contract of reference docs/API_SPEC.md section 3; delta validation follows
+// section 2.4 of the same document.
 #pragma once
 
+#include <array>
 #include <cstdint>
-#include <functional>
-#include <memory>
 #include <optional>
 #include <string>
 #include <string_view>
@@ -18,19 +17,7 @@
 
 namespace spk::params {
 
-enum class Type : uint8_t { Float, Int, Bool, Str };
-enum class Layer : uint8_t { Shoot, Print };
-
-struct FieldDecl {
-    std::string name, path;
-    Type type = Type::Float;
-    Layer layer = Layer::Shoot;
-    bool live = false;
-    std::optional<std::pair<double, double>> range;
-    json::Value def;
-    std::function<std::optional<std::string>(std::string_view)> check_enum;
-};
-
+// One identifier per schema field, in params_schema order (the field's own name).
 enum class P : uint16_t {
     film_stock, print_stock,
     exposure_compensation_ev, auto_exposure, auto_exposure_method, film_format_mm, lens_blur_um,
@@ -46,49 +33,73 @@ enum class P : uint16_t {
     output_color_space, output_cctf_encoding, scan_film, extended_dynamic_range,
     halation_scatter_amount, grain_amount, glare_amount, antihalation_removed, filter_shift_scale,
     halation_headroom_ev, preview_long_edge,
-    kCount
 };
-inline constexpr size_t id(P p) { return size_t(p); }
+inline constexpr size_t kFields = size_t(P::preview_long_edge) + 1;
 
-class Schema {
-public:
-    static std::shared_ptr<const Schema> build();
-    size_t size() const { return fields_.size(); }
-    const FieldDecl& field(size_t i) const { return fields_[i]; }
-    std::optional<size_t> find(std::string_view name) const;
-    const std::string& json_text() const { return json_; }
+enum class Kind : uint8_t { Float, Int, Bool, Str };
 
-private:
-    std::vector<FieldDecl> fields_;
-    std::string json_;
+// One schema row.  A ranged field has lo <= hi; `text` is the default of a Str field, `number`
+// the default of the others (bools as 0 / 1).  Print-layer fields have shoot = false.
+struct Row {
+    const char* name;
+    const char* path;
+    Kind kind;
+    bool ranged;
+    double lo, hi;
+    double number;
+    const char* text;
+    bool shoot;
+    bool live;
 };
 
-class ParamSet {
+// The rows in params_schema order; row(p) is the row of field p.
+const std::array<Row, kFields>& rows();
+inline const Row& row(P p) { return rows()[size_t(p)]; }
+std::optional<P> lookup(std::string_view name);
+// {"fields": [...], "schema_version": 1} (the params_schema reply).
+const std::string& schema_json();
+
+// A field's value: numbers and bools in `num` (an int is a whole number, a bool 0 or 1), strings
+// in `str`.
+struct Slot {
+    double num = 0.0;
+    std::string str;
+};
+
+// The values of every field (schema defaults until set).
+class Values {
 public:
-    explicit ParamSet(std::shared_ptr<const Schema> s);
-    double f(P p) const { return values_[id(p)].as_number(); }
-    int64_t i(P p) const { return int64_t(values_[id(p)].as_number()); }
-    bool b(P p) const { return values_[id(p)].as_bool(); }
-    const std::string& s(P p) const { return values_[id(p)].as_string(); }
-    const json::Value& get(size_t fid) const { return values_[fid]; }
-    void set(size_t fid, json::Value v) { values_[fid] = std::move(v); }
-    void set(P p, json::Value v) { set(id(p), std::move(v)); }
+    Values();
+    double f(P p) const { return slots_[size_t(p)].num; }
+    int64_t i(P p) const { return int64_t(slots_[size_t(p)].num); }
+    bool b(P p) const { return slots_[size_t(p)].num != 0.0; }
+    const std::string& s(P p) const { return slots_[size_t(p)].str; }
+    const Slot& slot(P p) const { return slots_[size_t(p)]; }
+    void put(P p, Slot v) { slots_[size_t(p)] = std::move(v); }
+    void put_number(P p, double v) { slots_[size_t(p)].num = v; }
+    void put_text(P p, std::string v) { slots_[size_t(p)].str = std::move(v); }
+    // The flat params object of the replies: 48 keys, numbers, bools and strings.
     json::Value to_json() const;
-    const Schema& schema() const { return *schema_; }
 
 private:
-    std::shared_ptr<const Schema> schema_;
-    std::vector<json::Value> values_;
+    std::array<Slot, kFields> slots_;
 };
+using ParamSet = Values;
 
+// A validated delta: the fields it sets (ascending by name) and what they touch.
 struct Delta {
-    std::vector<std::pair<size_t, json::Value>> fields;
-    bool has_stock = false, any_shoot = false, any_rebuild = false;
+    std::vector<std::pair<P, Slot>> sets;
+    bool names_stock = false;   // film_stock or print_stock
+    bool touches_shoot = false; // some field of the shoot layer
+    bool needs_rebuild = false; // some field that is neither live nor a stock
     std::optional<bool> product_defaults;
 };
 
-enum class DeltaContext { Open, SetParams };
-Delta validate_delta(const Schema& schema, std::string_view text, DeltaContext ctx);
-void apply_delta(ParamSet& p, const Delta& d);
+// Where a delta comes from: an open delta may carry "product_defaults" (bool, not a field).
+enum class Origin { Open, SetParams };
+// Parses and validates a delta text (an empty open delta is {}); failures throw SPK_ERR_USER with
+// the message section 2.4 names.
+Delta parse_delta(std::string_view text, Origin origin);
+void apply(Values& v, const Delta& d);
 
 }  // namespace spk::params

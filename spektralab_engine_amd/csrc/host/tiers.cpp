@@ -1,4 +1,4 @@
-// Tiers, tier downscale and meter; behaviour stated in tiers.h.
+// Tiers and the tier downscale; behaviour stated in tiers.h.
 #include "tiers.h"
 
 #include <algorithm>
@@ -106,91 +106,6 @@ gpu::DevBuf downscale(const float* src, uint32_t w, uint32_t h, uint32_t long_ed
     // The scratch blocks return to the pool only after the stream has passed them.
     gpu::check(hipStreamSynchronize(s), "hipStreamSynchronize(downscale)");
     return dst;
-}
-
-// ---- meter ------------------------------------------------------------------------------------
-double MeterValues::ev(std::string_view m) const {
-    if (m == "center_weighted") return center_weighted;
-    if (m == "average") return average;
-    if (m == "median") return median;
-    if (m == "balanced") return balanced;
-    if (m == "center") return center;
-    if (m == "protect_highlights") return protect_highlights;
-    if (m == "protect_shadows") return protect_shadows;
-    fail_internal("unknown auto-exposure method '" + std::string(m) + "'");
-}
-
-namespace {
-constexpr double kMid = 0.184, kSigmaCw = 0.2, kFloor = 0.184 / 4096.0;
-double legacy_ev(double e) {
-    const double ev = -std::log2(e);
-    return std::isfinite(ev) ? ev : 0.0;
-}
-// ascending with NaN last: a strict weak order when a decode produced NaN
-bool nan_last(double a, double b) { return a < b || (!std::isnan(a) && std::isnan(b)); }
-}  // namespace
-
-// Y := row 1 of rgb_to_xyz . rgb; centre weight exp(-(x^2 + y^2) / (2 0.2^2)) on coordinates
-// centred and scaled by the long edge; legacy meters on Y, intent meters on max(Y, 0.184 / 4096)
-// within its 1st..99th per-mille percentiles (meter.cpp:44-122).
-MeterValues meter_values(const float* rgb, uint32_t sw, uint32_t sh, const double rgb_to_xyz[9]) {
-    MeterValues mv;
-    const size_t n = size_t(sw) * sh;
-    if (n == 0) return mv;
-    const double m0 = rgb_to_xyz[3], m1 = rgb_to_xyz[4], m2 = rgb_to_xyz[5];
-    std::vector<double> Y(n);
-    for (size_t i = 0; i < n; ++i)
-        Y[i] = (m0 * double(rgb[3 * i]) + m1 * double(rgb[3 * i + 1])) + m2 * double(rgb[3 * i + 2]);
-    const double M = double(std::max(sw, sh));
-    std::vector<double> wx(sw), wy(sh);
-    for (uint32_t j = 0; j < sw; ++j) wx[j] = (double(j) / double(sw) - 0.5) * (double(sw) / M);
-    for (uint32_t i = 0; i < sh; ++i) wy[i] = (double(i) / double(sh) - 0.5) * (double(sh) / M);
-    const double two_s2 = 2.0 * (kSigmaCw * kSigmaCw);
-    auto weight = [&](size_t k) {
-        const double x = wx[k % sw], y = wy[k / sw];
-        return std::exp(-(x * x + y * y) / two_s2);
-    };
-    {
-        double s = 0.0;
-        for (double y : Y) s += y;
-        mv.average = legacy_ev((s / double(n)) / kMid);
-        std::vector<double> sorted = Y;
-        std::sort(sorted.begin(), sorted.end(), nan_last);
-        const double med = (n % 2) ? sorted[n / 2] : (sorted[n / 2 - 1] + sorted[n / 2]) / 2.0;
-        mv.median = legacy_ev(med / kMid);
-        double sw_ = 0.0, swy = 0.0;
-        for (size_t k = 0; k < n; ++k) {
-            const double wk = weight(k);
-            swy += wk * Y[k];
-            sw_ += wk;
-        }
-        mv.center_weighted = legacy_ev((swy / sw_) / kMid);
-    }
-    {
-        std::vector<double> y(n);
-        for (size_t k = 0; k < n; ++k) y[k] = std::max(Y[k], kFloor);
-        std::vector<double> sorted = y;
-        std::sort(sorted.begin(), sorted.end(), nan_last);
-        auto pct = [&](uint64_t q) { return sorted[size_t(q * uint64_t(n - 1) / 1000u)]; };
-        const double p1 = pct(10), p5 = pct(50), p99 = pct(990), p995 = pct(995);
-        double sl = 0.0, swl = 0.0, sw_ = 0.0;
-        size_t cnt = 0;
-        for (size_t k = 0; k < n; ++k) {
-            if (!(y[k] >= p1 && y[k] <= p99)) continue;
-            const double l = std::log(y[k]);
-            const double wk = weight(k);
-            sl += l;
-            swl += wk * l;
-            sw_ += wk;
-            ++cnt;
-        }
-        mv.balanced = -std::log2(std::exp(sl / double(cnt)) / kMid);
-        mv.center = -std::log2(std::exp(swl / sw_) / kMid);
-        mv.protect_highlights =
-            std::min(std::max(std::log2(kMid * std::pow(2.0, 2.5) / p995), mv.balanced - 3.0), mv.balanced);
-        mv.protect_shadows = std::min(std::max(std::log2(kMid * std::pow(2.0, -3.5) / p5), mv.balanced), mv.balanced + 2.0);
-    }
-    return mv;
 }
 
 }  // namespace spk::tiers

@@ -1,13 +1,11 @@
-// Tiers, the tier downscale and the auto-exposure meter.
+// Tiers and the tier downscale.
 //
 // Tier long edges: live = preview_long_edge, preview = 3400, full = the frame (reference
 // src/session/tiers.cpp:39-55).  The downscale (tiers.cpp:136-183) runs on the GPU from host-built
-// taps and exact-integer sampling tables; the meter (src/film/meter.cpp) is float64 on the host
-// over the 1600 px meter image.
+// taps and exact-integer sampling tables.
 #pragma once
 
 #include <cstdint>
-#include <string_view>
 #include <vector>
 
 #include "gpu.h"
@@ -15,7 +13,6 @@
 namespace spk::tiers {
 
 inline constexpr uint32_t kPreviewLongEdge = 3400;
-inline constexpr uint32_t kMeterLongEdge = 1600;
 
 struct Size {
     uint32_t w = 0, h = 0;
@@ -41,13 +38,5 @@ size_t downscale_bound(uint32_t w, uint32_t h, uint32_t long_edge);
 // (returns once enqueued; the result is valid after the stream reaches it).
 gpu::DevBuf downscale(const float* src, uint32_t w, uint32_t h, uint32_t long_edge, gpu::Pool& pool, hipStream_t s,
                       Size* out);
-
-// The meter values of the auto-exposure methods (reference src/film/meter.cpp:17-122).
-struct MeterValues {
-    double center_weighted = 0, average = 0, median = 0, balanced = 0, center = 0, protect_highlights = 0,
-           protect_shadows = 0;
-    double ev(std::string_view method) const;
-};
-MeterValues meter_values(const float* rgb, uint32_t sw, uint32_t sh, const double rgb_to_xyz[9]);
 
 }  // namespace spk::tiers

@@ -189,3 +189,85 @@ def test_coupler_and_glare_plan_fields(oracle, resources):
     assert abs(comps[:, 0, 0].sum() - (0.94 + 0.06 * 0.9999)) < 1e-15
     assert oracle.field(plan, "glare_active")[0] == 1
     assert np.isclose(oracle.field(plan, "glare_sigma")[0], 0.5 * np.sqrt(2.0), rtol=1e-15)
+
+
+# ---- the plan bytes of the benchmark configurations ------------------------------------------------
+def test_plan_bytes_unchanged(resources):
+    """spkx_plan of configurations 1, 2 (600 x 400 and 6000 x 4000) and 5 hashes to the recorded
+    digests (tests/golden/make_plan_digests.py): the set-up computes exactly what it computed when
+    the digests were recorded."""
+    import hashlib
+    import json
+    from pathlib import Path
+    import sys
+    sys.path.insert(0, str(Path(__file__).parent / "golden"))
+    import make_plan_digests as mk
+    want = json.loads((Path(__file__).parent / "golden" / "plan_digests.json").read_text())
+    assert mk.digests(resources) == want
+
+
+def test_plan_is_a_function_of_its_inputs(resources):
+    """Two plans of the same inputs are byte-identical (the unused table tails are zero)."""
+    a = bytes(spk.host_plan(resources, CONFIG1, 96, 64))
+    b = bytes(spk.host_plan(resources, CONFIG1, 96, 64))
+    assert a == b
+
+
+# ---- spk_output_transform (host form) -------------------------------------------------------------
+def test_output_transform_identity(resources):
+    """sRGB -> sRGB: the identity matrix (no adaptation between equal whites), sRGB curve modes,
+    the default gamut options."""
+    r, cmax = spk.output_transform_host(resources, "sRGB", "sRGB")
+    np.testing.assert_allclose(np.array(r["matrix"]).reshape(3, 3), np.eye(3), atol=1e-12)
+    assert r["source_cctf_mode"] == 0 and r["target_cctf_mode"] == 0
+    assert r["source_color_space"] == "sRGB" and r["target_color_space"] == "sRGB"
+    g = r["gamut_compress"]
+    assert g["algorithm"] == "cam16ucs" and g["cmax_rows"] == 64 and g["cmax_cols"] == 720
+    assert g["lightness_compression_active"] is True
+    np.testing.assert_allclose(g["k"][12:18], [0, 1, 6, 0.7, 1.0, 2.2])
+    m_xyz, m_rgb = np.array(g["m_to_xyz"]).reshape(3, 3), np.array(g["m_to_rgb"]).reshape(3, 3)
+    np.testing.assert_allclose(m_rgb @ m_xyz, np.eye(3), atol=1e-6)
+    assert cmax.shape == (64, 720) and np.all(cmax >= 0) and cmax.max() > 0
+
+
+@pytest.mark.parametrize("src,dst", [("ProPhoto RGB", "sRGB"), ("sRGB", "ProPhoto RGB"), ("ACEScg", "Display P3")])
+def test_output_transform_maps_white_to_white(resources, src, dst):
+    """Between spaces of different whites, CAT02 makes RGB white (1, 1, 1) of the source land on
+    (1, 1, 1) of the target (von Kries adaptation maps white to white); the mode is the space's
+    transfer (0 sRGB, 1 ProPhoto, 4 none)."""
+    r, _ = spk.output_transform_host(resources, src, dst)
+    m = np.array(r["matrix"]).reshape(3, 3)
+    np.testing.assert_allclose(m @ np.ones(3), np.ones(3), atol=1e-9)
+    modes = {"sRGB": 0, "ProPhoto RGB": 1, "ACEScg": 4, "Display P3": 0}
+    assert r["source_cctf_mode"] == modes[src] and r["target_cctf_mode"] == modes[dst]
+
+
+def test_output_transform_options_and_refusals(resources):
+    r, _ = spk.output_transform_host(resources, "sRGB", "sRGB",
+                                     {"algorithm": "off", "knee": [0.1, 0.9, 4], "lightness_compression_active": False})
+    g = r["gamut_compress"]
+    assert g["algorithm"] == "off" and g["lightness_compression_active"] is False
+    np.testing.assert_allclose(g["k"][12:18], [0.1, 0.9, 4, 0, 1, 1])
+    r, _ = spk.output_transform_host(resources, "sRGB", "sRGB", {"knee": [1, 2], "lightness_compression": [0.5, 1, 3]})
+    np.testing.assert_allclose(r["gamut_compress"]["k"][12:18], [0, 1, 6, 0.5, 1, 3])  # a 2-array is ignored
+    for gamut, status in [({"algorithm": "reinhard"}, -2), ("[1, 2]", -1), ("{", -1)]:
+        with pytest.raises(spk.SpkError) as e:
+            spk.output_transform_host(resources, "sRGB", "sRGB", gamut)
+        assert e.value.status == status
+    with pytest.raises(spk.SpkError) as e:
+        spk.output_transform_host(resources, "sRGB", "Rec. 9999")
+    assert e.value.status == -2
+
+
+def test_plan_tables_filled_and_tails_zero(resources, oracle):
+    """Every table of a grain + halation plan holds its data (not all zero) and nothing past it: the
+    zeroing of unused tails must not reach a table another step filled."""
+    plan = spk.host_plan(resources, CONFIG2, 120, 80, 0x4ABEA221, CONFIG2_OVERRIDES)
+    L, K = int(oracle.field(plan, "L")[0]), int(oracle.field(plan, "K")[0])
+    gk, pk = int(oracle.field(plan, "grain_k")[0]), int(oracle.field(plan, "paper_k")[0])
+    assert gk == K > 0 and pk > 0
+    for name, n, cap in [("table", L * L * 3, 0), ("grain_axis", 3 * gk, 3 * 1024), ("grain_inv", 3 * gk, 3 * 1024),
+                         ("grain_layers", 9 * gk, 9 * 1024), ("paper", 5 * pk, 5 * 1024), ("cmax", 46080, 0)]:
+        v = oracle.field(plan, name, max(n, cap)).view(np.float32)
+        assert np.count_nonzero(v[:n]) > n // 2, name
+        assert not np.any(v[n:]), name
