@@ -41,13 +41,88 @@ __device__ inline float fuse(uint32_t mode, const float* __restrict__ acc, size_
     return acc[o] + weight * v;
 }
 
+// Raw buffer access moves 32-bit patterns: reinterpret them, never convert.  The resource covers
+// one plane (num_records = its bytes), so an access outside it reads 0 or is dropped.
+__device__ inline float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ inline void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+__device__ inline __amdgpu_buffer_rsrc_t plane_rsrc(const float* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, int(bytes), 0x00020000);
+}
+
 // ---- FIR ---------------------------------------------------------------------------------------
 // Both passes give each thread kT adjacent outputs and stream the 2R + kT inputs they need through a
-// register window of 2 kT values, unrolled kT taps at a time so that every window index is a
-// compile-time constant: output j takes tap i from window slot i + j.  Each output accumulates
-// its taps in ascending order from 0.  The tap loop's trip count is the only runtime bound, and
-// it is wave-uniform.
+// register window of 2 kT values used as a ring: taps come kT at a time, and the half of the ring
+// the next group of taps reaches is refilled while the other half is still read, so no value is
+// ever moved and every ring slot is a compile-time constant (output j takes window index i + j,
+// held in slot (i + j) mod 2 kT).  Each output accumulates its taps in ascending order from 0.
+// The tap loop's trip count is the only runtime bound, and it is wave-uniform.
 constexpr int kT = 16;
+
+// acc[j] = sum_{i < n} taps[i] x[i + j] (fused multiply-add from 0, i ascending), x[t] = load(t)
+// for t <= n - 2 + kT.  kRing: the ring above; otherwise the window's upper half is copied down
+// after each group of taps (fewer live registers, for windows read from LDS).
+template <bool kRing, class Load>
+__device__ __forceinline__ void fir_window(int n, const float* __restrict__ taps, Load load, float (&acc)[kT]) {
+    const int last = n - 2 + kT;
+    if (!kRing) {
+        float win[2 * kT];
+#pragma unroll
+        for (int j = 0; j < kT; ++j) {
+            acc[j] = 0.0f;
+            win[j] = load(j);
+        }
+        for (int i0 = 0; i0 < n; i0 += kT) {
+#pragma unroll
+            for (int k = 0; k < kT; ++k) win[kT + k] = i0 + kT + k <= last ? load(i0 + kT + k) : 0.0f;
+#pragma unroll
+            for (int ii = 0; ii < kT; ++ii) {
+                if (i0 + ii < n) {
+                    const float tap = taps[i0 + ii];
+#pragma unroll
+                    for (int j = 0; j < kT; ++j) acc[j] = fmaf(tap, win[ii + j], acc[j]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kT; ++k) win[k] = win[kT + k];
+        }
+        return;
+    }
+    float ring[2 * kT];
+#pragma unroll
+    for (int j = 0; j < kT; ++j) {
+        acc[j] = 0.0f;
+        ring[j] = load(j);
+    }
+    for (int i0 = 0; i0 < n; i0 += 2 * kT) {
+        // taps i0 .. i0 + kT - 1 read window indices i0 .. i0 + 2 kT - 2: slots 0 .. 2 kT - 1
+#pragma unroll
+        for (int k = 0; k < kT; ++k) ring[kT + k] = i0 + kT + k <= last ? load(i0 + kT + k) : 0.0f;
+#pragma unroll
+        for (int ii = 0; ii < kT; ++ii) {
+            if (i0 + ii < n) {
+                const float tap = taps[i0 + ii];
+#pragma unroll
+                for (int j = 0; j < kT; ++j) acc[j] = fmaf(tap, ring[ii + j], acc[j]);
+            }
+        }
+        if (i0 + kT >= n) break;
+        // taps i0 + kT .. i0 + 2 kT - 1 read indices up to i0 + 3 kT - 2: slots kT .. 2 kT - 1, then 0 ..
+#pragma unroll
+        for (int k = 0; k < kT; ++k) ring[k] = i0 + 2 * kT + k <= last ? load(i0 + 2 * kT + k) : 0.0f;
+#pragma unroll
+        for (int ii = 0; ii < kT; ++ii) {
+            if (i0 + kT + ii < n) {
+                const float tap = taps[i0 + kT + ii];
+#pragma unroll
+                for (int j = 0; j < kT; ++j) acc[j] = fmaf(tap, ring[(kT + ii + j) % (2 * kT)], acc[j]);
+            }
+        }
+    }
+}
 
 // Vertical: a wave is 64 adjacent columns of one run of kT rows; loads are coalesced rows, and
 // rows outside the frame (wave-uniform) are folded by half-sample reflection.  nplanes
@@ -64,41 +139,29 @@ __global__ __launch_bounds__(256) void k_fir_v(const float* __restrict__ in, flo
     const size_t poff = size_t(plane) * q.w * q.h;
     in += poff;
     out += poff;
+    // threadIdx.y is the wave's index in the 64 x 4 workgroup: the rows a wave reads and writes
+    // are wave-uniform, so their byte offsets (and any reflection) stay scalar, and each load or
+    // store is one buffer instruction on the lane's fixed column offset
+    const int wave = int(__builtin_amdgcn_readfirstlane(threadIdx.y));
     const int x = int(bx * 64 + threadIdx.x);
-    const int y0 = int((by * 4 + threadIdx.y) * kT);
+    const int y0 = int((by * 4 + uint32_t(wave)) * kT);
     if (y0 >= h) return;
     const int xc = x < w ? x : w - 1;
-    const float* col = in + xc;
-    auto load = [&](int t) {
-        int yy = y0 - R + t;
-        if (yy < 0 || yy >= h) yy = reflect_index(yy, h);
-        return col[size_t(yy) * w];
-    };
+    const uint32_t plane_bytes = q.w * q.h * 4u, rowb = q.w * 4u, voff = uint32_t(xc) * 4u;
+    const __amdgpu_buffer_rsrc_t rin = plane_rsrc(in, plane_bytes);
     const int n = 2 * R + 1;
-    float acc[kT], win[2 * kT];
-#pragma unroll
-    for (int j = 0; j < kT; ++j) {
-        acc[j] = 0.0f;
-        win[j] = load(j);
-    }
-    for (int i0 = 0; i0 < n; i0 += kT) {
-#pragma unroll
-        for (int k = 0; k < kT; ++k) win[kT + k] = load(i0 + kT + k);
-#pragma unroll
-        for (int ii = 0; ii < kT; ++ii) {
-            if (i0 + ii < n) {
-                const float tap = q.taps[i0 + ii];
-#pragma unroll
-                for (int j = 0; j < kT; ++j) acc[j] = fmaf(tap, win[ii + j], acc[j]);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kT; ++k) win[k] = win[kT + k];
-    }
+    const bool interior = y0 - R >= 0 && y0 + R + kT <= h;  // no row needs reflection
+    float acc[kT];
+    fir_window<true>(n, q.taps, [&](int t) {
+        int yy = y0 - R + t;
+        if (!interior && (yy < 0 || yy >= h)) yy = reflect_index(yy, h);
+        return buf_ld(rin, voff, uint32_t(yy) * rowb);
+    }, acc);
     if (x >= w) return;
+    const __amdgpu_buffer_rsrc_t rout = plane_rsrc(out, plane_bytes);
 #pragma unroll
     for (int j = 0; j < kT; ++j)
-        if (y0 + j < h) out[size_t(y0 + j) * w + x] = acc[j];
+        if (y0 + j < h) buf_st(acc[j], rout, voff, uint32_t(y0 + j) * rowb);
 }
 
 // Horizontal: a wave is one row segment of 64 kT outputs staged in LDS with one pad word per 16
@@ -113,61 +176,51 @@ __global__ __launch_bounds__(256) void k_fir_h(const float* __restrict__ in, flo
     extern __shared__ float lds_h[];
     const int w = int(q.w), R = int(q.r);
     const int h = int(q.h * nrows_planes);  // rows of all planes, contiguous
-    const int y = int(blockIdx.y * 4 + threadIdx.y);
+    // threadIdx.y is the wave's index in the 64 x 4 workgroup: its row is wave-uniform
+    const int wave = int(__builtin_amdgcn_readfirstlane(threadIdx.y));
+    const int y = int(blockIdx.y * 4) + wave;
     const int x0 = int(blockIdx.x) * kHSeg;
     const int span = kHSeg + 2 * R + kT;
     const int stride = hpad(span) + 1;
-    float* tile = lds_h + threadIdx.y * stride;
+    float* tile = lds_h + wave * stride;
     const int lane = int(threadIdx.x);
     const bool rowok = y < h;
+    // rows of all planes are contiguous: one resource over them, rows by a scalar byte offset
+    // (below 4 GiB: the engine's frames are at most 151301008 pixels, 605 MB a plane)
+    const uint32_t rowb = q.w * 4u, soff = uint32_t(y) * rowb;
+    const uint32_t all_bytes = uint32_t(h) * rowb;
     if (rowok) {
-        const float* row = in + size_t(y) * w;
+        const __amdgpu_buffer_rsrc_t rin = plane_rsrc(in, all_bytes);
+        const bool interior = x0 - R >= 0 && x0 - R + span <= w;  // no column needs reflection
         for (int t = lane; t < span; t += 64) {
             int xx = x0 - R + t;
-            if (xx < 0 || xx >= w) xx = reflect_index(xx, w);
-            tile[hpad(t)] = row[xx];
+            if (!interior && (xx < 0 || xx >= w)) xx = reflect_index(xx, w);
+            tile[hpad(t)] = buf_ld(rin, uint32_t(xx) * 4u, soff);
         }
     }
     __syncthreads();
-    const int n = 2 * R + 1;
     const int base = lane * kT;
-    float acc[kT], win[2 * kT];
-#pragma unroll
-    for (int j = 0; j < kT; ++j) {
-        acc[j] = 0.0f;
-        win[j] = tile[hpad(base + j)];
-    }
-    for (int i0 = 0; i0 < n; i0 += kT) {
-#pragma unroll
-        for (int k = 0; k < kT; ++k) win[kT + k] = tile[hpad(base + i0 + kT + k)];
-#pragma unroll
-        for (int ii = 0; ii < kT; ++ii) {
-            if (i0 + ii < n) {
-                const float tap = q.taps[i0 + ii];
-#pragma unroll
-                for (int j = 0; j < kT; ++j) acc[j] = fmaf(tap, win[ii + j], acc[j]);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kT; ++k) win[k] = win[kT + k];
-    }
+    float acc[kT];
+    fir_window<false>(2 * R + 1, q.taps, [&](int t) { return tile[hpad(base + t)]; }, acc);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kT; ++j) tile[hpad(base + j)] = acc[j];
     __syncthreads();
     if (!rowok) return;
     // 16 coalesced results per lane; the accumulation plane's loads are issued before the stores
+    const __amdgpu_buffer_rsrc_t rout = plane_rsrc(out, all_bytes);
     float r[kT], av[kT];
 #pragma unroll
     for (int u = 0; u < kT; ++u) {
         const int t = lane + 64 * u, x = x0 + t;
         r[u] = tile[hpad(t)];
-        av[u] = (q.acc_mode == 2u && x < w) ? accp[size_t(y) * w + x] : 0.0f;
+        av[u] = 0.0f;
+        if (q.acc_mode == 2u && x < w) av[u] = buf_ld(plane_rsrc(accp, all_bytes), uint32_t(x) * 4u, soff);
     }
 #pragma unroll
     for (int u = 0; u < kT; ++u) {
         const int x = x0 + lane + 64 * u;
-        if (x < w) out[size_t(y) * w + x] = fuse_v(q.acc_mode, av[u], q.weight, r[u]);
+        if (x < w) buf_st(fuse_v(q.acc_mode, av[u], q.weight, r[u]), rout, uint32_t(x) * 4u, soff);
     }
 }
 
@@ -379,13 +432,6 @@ __device__ inline St mat_apply(const double* M, const St& v) {
 
 __device__ inline St st_add(const St& x, const St& y) { return {x.a + y.a, x.b + y.b, x.c + y.c}; }
 
-// Raw buffer access moves 32-bit patterns: reinterpret them, never convert.
-__device__ inline float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
-__device__ inline void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
-}
 
 // The forward-backward recursion over one line split into nch chunks of MC samples, thread j
 // holding chunk j in v[] (the last chunk's samples past the line end padded).  Each direction:

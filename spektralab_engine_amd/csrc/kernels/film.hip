@@ -182,13 +182,15 @@ __device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t se
 // ---- the grain draws of one wavefront, compacted ---------------------------------------------
 // A wavefront's 64 pixels make 576 draws (3 channels x 3 sub-layers).  Drawn where they arise,
 // lanes whose lam lies on the other side of 10 (the two methods) or that need more uniform blocks
-// than their neighbours sit idle; here the draws are first listed by method in the wavefront's
-// LDS slice, then each list is drained by all 64 lanes, a lane taking the next draw as soon as
-// its current one is done.  Every draw consumes the same Philox blocks in the same order as
-// poisson() above, so the counts are identical; only the lane doing the work changes.
+// than their neighbours sit idle.  So a draw with lam >= 10 makes only the two tries of its first
+// Philox block where it arises (they accept about 96 % of such draws, and nearly every lane has
+// one in each slot); what is left is listed by method in the wavefront's LDS slice, and each list
+// is drained by all 64 lanes, a lane taking the next draw as soon as its current one is done.
+// Every draw consumes the same Philox blocks in the same order as poisson() above, so the counts
+// are identical; only the lane doing the work changes.
 struct GrainScratch {
     float v[9][64];     // lam of draw (j = 3 c + l, pixel lane), then its Poisson count
-    uint16_t q[576];    // draw ids (j << 6 | lane): lam >= 10 from the front, the others from the back
+    uint16_t q[576];    // draw ids (j << 6 | lane): lam >= 10 still to draw from the front, the others from the back
 };
 
 __device__ inline void wave_sync() {
@@ -201,9 +203,9 @@ __device__ inline uint32_t lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
 }
 
-// Drains `count` listed draws of one method; base: the wavefront's first pixel.  A PTRS draw
-// makes one try per pass of the loop: the (x, y) pair of a fresh Philox block, or the (z, w) pair
-// the lane kept from its previous pass.  A try accepts 75-87 % of the time (lam 10..300), so
+// Drains `count` listed draws of one method; base: the wavefront's first pixel.  The lam >= 10
+// draws listed here resume at Philox block 1.  A PTRS draw makes one try per pass of the loop:
+// the (x, y) pair of a fresh Philox block, or the (z, w) pair the lane kept from its previous pass.  A try accepts 75-87 % of the time (lam 10..300), so
 // running both tries of a block in every pass would leave most lanes idle through the second one.
 // (One loop over both methods, the few small draws filling the large ones' idle tail, needs more
 // registers and measured slower: 2.25 against 2.02 ms at 24 MP.)
@@ -227,7 +229,7 @@ __device__ __forceinline__ void drain(GrainScratch& gs, uint32_t count, uint32_t
                 src = id & 63u;
                 lam = gs.v[j][src];
                 stream = j / 3u + 10u * (j % 3u);
-                blk = 0;
+                blk = kLarge ? 1u : 0u;  // a listed lam >= 10 draw has rejected both tries of block 0
                 busy = true;
                 if (kLarge) {
                     second = false;
@@ -577,25 +579,39 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
 #pragma unroll
         for (uint32_t jd = 0; jd < 9; ++jd) q.lambdas[size_t(jd) * n + i] = gs.v[jd][lane];
     }
-    // list the draws by method, then drain each list with the whole wavefront
+    // The draws with lam >= 10 first make the two tries of their Philox block 0 where they arise:
+    // every lane has one such draw in almost every slot jd, so this runs at full width.  A draw
+    // rejected twice (a few percent) is listed for the compacted drain from block 1 on, as are
+    // the lam < 10 draws; each list is then drained by the whole wavefront.
     wave_sync();
-    uint32_t n_large = 0, n_small = 0;
-#pragma unroll
+    const float hl2p = spkm::half_log_2pi();
+    const uint32_t base = i - lane;
+    uint32_t n_retry = 0, n_small = 0;
+#pragma unroll 1
     for (uint32_t jd = 0; jd < 9; ++jd) {
         const float lam = valid ? gs.v[jd][lane] : 0.0f;
         const bool large = valid && lam >= 10.0f;
         const bool small = valid && !(lam >= 10.0f);
-        const uint64_t bl = __ballot(large), bs = __ballot(small);
+        bool pending = large;
+        if (large) {
+            Ptrs P;
+            ptrs_setup(P, lam);
+            const uint4 w = philox_block(i, jd / 3u + 10u * (jd % 3u), 0u, seed);
+            float r;
+            if (ptrs_try(P, u01(w.x), u01(w.y), s_lgam, hl2p, &r) || ptrs_try(P, u01(w.z), u01(w.w), s_lgam, hl2p, &r)) {
+                gs.v[jd][lane] = r;
+                pending = false;
+            }
+        }
+        const uint64_t bl = __ballot(pending), bs = __ballot(small);
         const uint16_t id = uint16_t((jd << 6) | lane);
-        if (large) gs.q[n_large + lanes_below(bl)] = id;
+        if (pending) gs.q[n_retry + lanes_below(bl)] = id;
         if (small) gs.q[575u - (n_small + lanes_below(bs))] = id;
-        n_large += uint32_t(__popcll(bl));
+        n_retry += uint32_t(__popcll(bl));
         n_small += uint32_t(__popcll(bs));
     }
     wave_sync();
-    const float hl2p = spkm::half_log_2pi();
-    const uint32_t base = i - lane;
-    drain<true>(gs, n_large, lane, base, seed, hl2p, s_lgam);
+    drain<true>(gs, n_retry, lane, base, seed, hl2p, s_lgam);
     drain<false>(gs, n_small, lane, base, seed, hl2p, s_lgam);
     wave_sync();
     if (!valid) return;
