@@ -640,15 +640,33 @@ void mitchell_axis(T t, uint32_t L, int* idx, T* w) {
     w[3] = mitchell(frac - T(2));
 }
 
-template <class T>
-const T* table_of(const spk_plan& p);
-template <>
-const float* table_of<float>(const spk_plan& p) {
-    return p.table;
-}
-template <>
-const double* table_of<double>(const spk_plan& p) {
-    return p.table64;
+// The Mitchell-Netravali (B = C = 1/3) bicubic sample of an L x L x 3 table (tabv(k): element k)
+// at (t0, t1) in [0, 1]^2, mirrored about the end samples and normalised by the weight sum
+// (kernels/film.metal upsample).
+template <class T, class Tab>
+void mitchell_sample(Tab tabv, uint32_t L, T t0, T t1, T out[3]) {
+    int ia[4], ib[4];
+    T wa[4], wb[4];
+    mitchell_axis(t0, L, ia, wa);
+    mitchell_axis(t1, L, ib, wb);
+    T s = 0, a0 = 0, a1 = 0, a2 = 0;
+    for (int ii = 0; ii < 4; ++ii)
+        for (int jj = 0; jj < 4; ++jj) {
+            const T wgt = wa[ii] * wb[jj];
+            const size_t tb = 3 * (size_t(ia[ii]) * L + size_t(ib[jj]));
+            s += wgt;
+            a0 += wgt * tabv(tb);
+            a1 += wgt * tabv(tb + 1);
+            a2 += wgt * tabv(tb + 2);
+        }
+    if (s != T(0)) {
+        a0 = a0 / s;
+        a1 = a1 / s;
+        a2 = a2 / s;
+    }
+    out[0] = a0;
+    out[1] = a1;
+    out[2] = a2;
 }
 
 template <class T>
@@ -851,6 +869,36 @@ T grain_frozen(const spk_plan& p, T D, const float* k9, uint32_t c) {
 // thp: the parameters (null: the plan's).  counts_in: frozen Poisson counts, 9 per pixel
 // ([i * 9 + 3 c + l]); counts_out: records the counts the binary32 draw makes; lams_out their
 // Poisson means (same layout).
+// Highlight boost of n exposure values with frame maximum M (develop.cpp:40-62, film.metal:229-238):
+// above x0 = min(0.184 2^4, M) a value v becomes v + k M (e^z - z - 1), z = a (v - x0) / M,
+// a = 28^0.7, k chosen so that M maps to 2^ev M; M == 0 zeroes the plane.
+template <class T>
+void boost_values(T* H, size_t n, double boost_ev, float M) {
+    constexpr bool f32 = std::is_same_v<T, float>;
+    if (M == 0.0f) {
+        for (size_t i = 0; i < n; ++i) H[i] = H[i] * T(0);
+        return;
+    }
+    const double Md = double(M);
+    const double x0 = std::min(std::max(0.184 * std::pow(2.0, 4.0), 0.0), Md);
+    if (x0 == Md) return;
+    const double a = std::pow(28.0, 1.0 - 0.3);
+    const double q = a * (1.0 - x0 / Md);
+    const double den = std::exp(q) - q - 1.0;
+    const double kk = (std::pow(2.0, boost_ev) - 1.0) / den;
+    const T tx0 = f32 ? T(float(x0)) : T(x0), tr = f32 ? T(float(1.0 / Md)) : T(1.0 / Md);
+    const T ta = f32 ? T(float(a)) : T(a), tk = f32 ? T(float(kk * Md)) : T(kk * Md);
+    for (size_t i = 0; i < n; ++i) {
+        T& v = H[i];
+        if (!(v > tx0)) continue;
+        const T z = ta * ((v - tx0) * tr);
+        T e;
+        if constexpr (f32) e = spkm::expf_(z);
+        else e = exp(z);
+        v = v + tk * ((e - z) - T(1));
+    }
+}
+
 template <class T, class In = float>
 void develop(const spk_plan& p, const In* rgb, T* neg, const Theta<T>* thp = nullptr, const float* counts_in = nullptr,
              float* counts_out = nullptr, float* lams_out = nullptr) {
@@ -900,26 +948,9 @@ void develop(const spk_plan& p, const In* rgb, T* neg, const Theta<T>* thp = nul
         const T t0 = fmin(fmax(o * o, T(0)), T(1));
         const T t1 = fmin(fmax(y / (o > T(1e-10) ? o : T(1e-10)), T(0)), T(1));
         const T bp = isnan(bs) ? T(0) : bs;
-        int ia[4], ib[4];
-        T wa[4], wb[4];
-        mitchell_axis(t0, L, ia, wa);
-        mitchell_axis(t1, L, ib, wb);
-        T s = 0, a0 = 0, a1 = 0, a2 = 0;
-        for (int ii = 0; ii < 4; ++ii)
-            for (int jj = 0; jj < 4; ++jj) {
-                const T wgt = wa[ii] * wb[jj];
-                const size_t tb = 3 * (size_t(ia[ii]) * L + size_t(ib[jj]));
-                s += wgt;
-                a0 += wgt * tabv(tb);
-                a1 += wgt * tabv(tb + 1);
-                a2 += wgt * tabv(tb + 2);
-            }
-        if (s != T(0)) {
-            a0 = a0 / s;
-            a1 = a1 / s;
-            a2 = a2 / s;
-        }
-        T h0 = bp * a0, h1 = bp * a1, h2 = bp * a2;
+        T av[3];
+        mitchell_sample<T>(tabv, L, t0, t1, av);
+        T h0 = bp * av[0], h1 = bp * av[1], h2 = bp * av[2];
         if (ec_active) {
             h0 = h0 * ec;
             h1 = h1 * ec;
@@ -938,31 +969,7 @@ void develop(const spk_plan& p, const In* rgb, T* neg, const Theta<T>* thp = nul
         return m;
     };
     constexpr bool f32 = std::is_same_v<T, float>;
-    if (p.boost_active) {
-        const float M = frame_max();
-        if (M == 0.0f) {
-            for (auto& v : H) v = v * T(0);
-        } else {
-            const double Md = double(M);
-            const double x0 = std::min(std::max(0.184 * std::pow(2.0, 4.0), 0.0), Md);
-            if (x0 != Md) {
-                const double a = std::pow(28.0, 1.0 - 0.3);
-                const double q = a * (1.0 - x0 / Md);
-                const double den = std::exp(q) - q - 1.0;
-                const double kk = (std::pow(2.0, p.boost_ev) - 1.0) / den;
-                const T tx0 = f32 ? T(float(x0)) : T(x0), tr = f32 ? T(float(1.0 / Md)) : T(1.0 / Md);
-                const T ta = f32 ? T(float(a)) : T(a), tk = f32 ? T(float(kk * Md)) : T(kk * Md);
-                for (auto& v : H) {
-                    if (!(v > tx0)) continue;
-                    const T z = ta * ((v - tx0) * tr);
-                    T e;
-                    if constexpr (f32) e = spkm::expf_(z);
-                    else e = exp(z);
-                    v = v + tk * ((e - z) - T(1));
-                }
-            }
-        }
-    }
+    if (p.boost_active) boost_values<T>(H.data(), H.size(), p.boost_ev, frame_max());
     if (p.headroom_active) {
         const float clip = frame_max();
         if (clip > 0.0f && std::isfinite(clip)) {
@@ -1248,6 +1255,49 @@ T paper_lookup(const spk_plan& p, uint32_t c, T v) {
     return y0 + ((v - X(j)) * R(j)) * (Y(j + 1, c) - y0);
 }
 
+// The CAM16-UCS constants of a plan's output space (float32 or float64 copies).
+template <class T>
+Cam<T> cam_of(const spk_plan& p) {
+    const bool f32 = sizeof(T) == 4;
+    auto pick = [&](float a, double b) { return f32 ? T(a) : T(b); };
+    Cam<T> q{};
+    for (int i = 0; i < 9; ++i) {
+        q.m_to_xyz[i] = pick(p.m_to_xyz[i], p.m_to_xyz64[i]);
+        q.m_to_rgb[i] = pick(p.m_to_rgb[i], p.m_to_rgb64[i]);
+        q.m16[i] = pick(p.m16[i], p.m16_64[i]);
+        q.m16_inv[i] = pick(p.m16_inv[i], p.m16_inv64[i]);
+    }
+    const float k32[22] = {p.f_l, p.n_bb, p.n_cb, p.n_, p.z, p.a_w, p.c, p.n_c, p.j_first, p.j_last, p.h_first,
+                           p.h_step, p.knee[0], p.knee[1], p.knee[2], p.lc[0], p.lc[1], p.lc[2], p.j_white,
+                           p.d_rgb[0], p.d_rgb[1], p.d_rgb[2]};
+    T kk[22];
+    for (int i = 0; i < 22; ++i) kk[i] = pick(k32[i], p.cam64[i]);
+    q.f_l = kk[0];
+    q.n_bb = kk[1];
+    q.n_cb = kk[2];
+    q.n = kk[3];
+    q.z = kk[4];
+    q.a_w = kk[5];
+    q.c = kk[6];
+    q.n_c = kk[7];
+    q.j_first = kk[8];
+    q.j_last = kk[9];
+    q.h_first = kk[10];
+    q.h_step = kk[11];
+    for (int i = 0; i < 3; ++i) {
+        q.knee[i] = kk[12 + i];
+        q.lc[i] = kk[15 + i];
+        q.d_rgb[i] = kk[19 + i];
+    }
+    q.j_white = kk[18];
+    q.cz = q.c * q.z;
+    q.inv_cz = T(1) / q.cz;
+    q.f_l_qrt = pow(q.f_l, T(0.25));
+    q.chroma_k = pow(T(1.64) - pow(T(0.29), q.n), T(0.73));
+    q.hundred_over_f_l = T(100) / q.f_l;
+    return q;
+}
+
 // thp: the parameters (null: the plan's); enc_out: the encoded value before the 16-bit
 // quantisation (3 planes).  out may be null.
 template <class T>
@@ -1285,41 +1335,7 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out, const The
     const T kg = th.kg;
     const T* pg = th.pg;
     const T* po = th.po;
-    Cam<T> q{};
-    for (int i = 0; i < 9; ++i) {
-        q.m_to_xyz[i] = pick(p.m_to_xyz[i], p.m_to_xyz64[i]);
-        q.m_to_rgb[i] = pick(p.m_to_rgb[i], p.m_to_rgb64[i]);
-        q.m16[i] = pick(p.m16[i], p.m16_64[i]);
-        q.m16_inv[i] = pick(p.m16_inv[i], p.m16_inv64[i]);
-    }
-    const float k32[22] = {p.f_l, p.n_bb, p.n_cb, p.n_, p.z, p.a_w, p.c, p.n_c, p.j_first, p.j_last, p.h_first,
-                           p.h_step, p.knee[0], p.knee[1], p.knee[2], p.lc[0], p.lc[1], p.lc[2], p.j_white,
-                           p.d_rgb[0], p.d_rgb[1], p.d_rgb[2]};
-    T kk[22];
-    for (int i = 0; i < 22; ++i) kk[i] = pick(k32[i], p.cam64[i]);
-    q.f_l = kk[0];
-    q.n_bb = kk[1];
-    q.n_cb = kk[2];
-    q.n = kk[3];
-    q.z = kk[4];
-    q.a_w = kk[5];
-    q.c = kk[6];
-    q.n_c = kk[7];
-    q.j_first = kk[8];
-    q.j_last = kk[9];
-    q.h_first = kk[10];
-    q.h_step = kk[11];
-    for (int i = 0; i < 3; ++i) {
-        q.knee[i] = kk[12 + i];
-        q.lc[i] = kk[15 + i];
-        q.d_rgb[i] = kk[19 + i];
-    }
-    q.j_white = kk[18];
-    q.cz = q.c * q.z;
-    q.inv_cz = T(1) / q.cz;
-    q.f_l_qrt = pow(q.f_l, T(0.25));
-    q.chroma_k = pow(T(1.64) - pow(T(0.29), q.n), T(0.73));
-    q.hundred_over_f_l = T(100) / q.f_l;
+    const Cam<T> q = cam_of<T>(p);
     const int rows = SPK_CMAX_ROWS, cols = SPK_CMAX_COLS;
     const T L2 = K<T>::log2_10, tiny = K<T>::tiny;
     // glare (fx/glare.cpp:22-51, fx_random.h:100-115, fx_glare.metal:13-22): one lognormal value per
@@ -1764,6 +1780,45 @@ float oracle_expf(float x) { return spkm::expf_(x); }
 int oracle_threads(void) { return omp_get_max_threads(); }
 
 // Byte offset of a named spk_plan member (for reading plans from Python); -1 when unknown.
+// ---- single-function probes (known-answer tests of the stages above) ------------------------------
+void oracle_mitchell_sample(const double* table, uint32_t L, double t0, double t1, int f32, double* out3) {
+    if (f32) {
+        float o[3];
+        mitchell_sample<float>([&](size_t k) { return float(table[k]); }, L, float(t0), float(t1), o);
+        for (int c = 0; c < 3; ++c) out3[c] = o[c];
+    } else {
+        mitchell_sample<double>([&](size_t k) { return table[k]; }, L, t0, t1, out3);
+    }
+}
+float oracle_curve_lookup(const float* x, const float* inv, const float* y, uint32_t K, float v) {
+    return curve_lookup<float>(x, inv, y, K, v);
+}
+float oracle_paper_lookup(const spk_plan* p, uint32_t c, float v) { return paper_lookup<float>(*p, c, v); }
+void oracle_boost(float* H, size_t n, double boost_ev) {
+    float m = -std::numeric_limits<float>::infinity();
+    for (size_t k = 0; k < n; ++k)
+        if (H[k] > m) m = H[k];
+    boost_values<float>(H, n, boost_ev, m);
+}
+uint16_t oracle_encode16(float v, uint32_t mode, int active) { return encode16<float>(active ? cctf_encode<float>(v, mode) : v); }
+void oracle_cam16_forward(const spk_plan* p, const double* xyz, int f32, double* out3) {
+    if (f32) {
+        const Cam<float> q = cam_of<float>(*p);
+        const float x[3] = {float(xyz[0]), float(xyz[1]), float(xyz[2])};
+        float o[3];
+        cam16_forward(x, q, o);
+        for (int c = 0; c < 3; ++c) out3[c] = o[c];
+    } else {
+        const Cam<double> q = cam_of<double>(*p);
+        cam16_forward(xyz, q, out3);
+    }
+}
+void oracle_cam16_inverse(const spk_plan* p, double Jp, double Mp, double h, double* xyz3) {
+    const Cam<double> q = cam_of<double>(*p);
+    cam16_inverse(Jp, Mp, h, q, xyz3);
+}
+double oracle_knee(double d, double tau, double lim, double pw) { return knee<double>(d, tau, lim, pw); }
+
 long oracle_plan_offset(const char* name) {
 #define F(m) \
     if (std::strcmp(name, #m) == 0) return long(offsetof(spk_plan, m));
@@ -1777,7 +1832,8 @@ long oracle_plan_offset(const char* name) {
     F(m_to_xyz64) F(m_to_rgb64) F(unsharp_sigma) F(unsharp_amount) F(cctf_active) F(cctf_mode) F(m_nid64)
     F(decode_active) F(decode_mode) F(geometry_active) F(src_w) F(src_h) F(boost_active) F(headroom_active)
     F(lens_blur_px) F(levels_active) F(levels_m) F(levels_q) F(edr_active) F(edr_n) F(edr_table) F(edr_r_y)
-    F(table) F(table64) F(curves) F(curves_x64) F(curves_y64) F(paper) F(paper_x64) F(paper_y64) F(cmax) F(cmax64)
+    F(table) F(table64) F(curves) F(curves_x64) F(curves_y64) F(curves_y064) F(paper) F(paper_x64) F(paper_y64) F(cmax) F(cmax64)
+    F(coupler_m) F(coupler_dmax) F(coupler_positive) F(boost_ev) F(headroom_ev)
 #undef F
     return -1;
 }

@@ -58,6 +58,22 @@ def lib():
         l.oracle_develop_draws.restype = None
         l.oracle_poisson_at.argtypes = [F, U, U, C.c_size_t, C.c_uint32, F]
         l.oracle_poisson_at.restype = None
+        l.oracle_mitchell_sample.argtypes = [D, C.c_uint32, C.c_double, C.c_double, C.c_int, D]
+        l.oracle_mitchell_sample.restype = None
+        l.oracle_curve_lookup.argtypes = [F, F, F, C.c_uint32, C.c_float]
+        l.oracle_curve_lookup.restype = C.c_float
+        l.oracle_paper_lookup.argtypes = [V, C.c_uint32, C.c_float]
+        l.oracle_paper_lookup.restype = C.c_float
+        l.oracle_boost.argtypes = [F, C.c_size_t, C.c_double]
+        l.oracle_boost.restype = None
+        l.oracle_encode16.argtypes = [C.c_float, C.c_uint32, C.c_int]
+        l.oracle_encode16.restype = C.c_uint16
+        l.oracle_cam16_forward.argtypes = [V, D, C.c_int, D]
+        l.oracle_cam16_forward.restype = None
+        l.oracle_cam16_inverse.argtypes = [V, C.c_double, C.c_double, C.c_double, D]
+        l.oracle_cam16_inverse.restype = None
+        l.oracle_knee.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
+        l.oracle_knee.restype = C.c_double
         l.oracle_frozen_f64.argtypes = [V, F, F, F, C.c_int, C.c_double, D, D, D]
         l.oracle_frozen_f64.restype = C.c_double
         l.oracle_frozen_grad.argtypes = [V, F, F, F, D, D, C.c_int, D]
@@ -85,7 +101,8 @@ _DT = {"width": "u4", "height": "u4", "L": "u4", "K": "u4", "scatter_active": "u
        "couplers_active": "u4", "glare_active": "u4", "glare_seed": "u4", "n_dir": "u4", "spectral_fp16": "u4",
        "decode_active": "u4", "decode_mode": "u4", "geometry_active": "u4", "src_w": "u4", "src_h": "u4",
        "boost_active": "u4", "headroom_active": "u4", "levels_active": "u4", "levels_m": "f4", "levels_q": "f4",
-       "edr_active": "u4", "edr_n": "u4", "edr_table": "f4", "edr_r_y": "f4"}
+       "edr_active": "u4", "edr_n": "u4", "edr_table": "f4", "edr_r_y": "f4", "coupler_m": "f4",
+       "coupler_dmax": "f4", "coupler_positive": "u4"}
 
 
 def field(plan, name, count=1):
@@ -285,3 +302,53 @@ def yvv(sigma):
     out = (C.c_double * 4)()
     lib().oracle_yvv(sigma, out)
     return tuple(out)
+
+
+# ---- single-stage probes (known-answer tests) -----------------------------------------------------
+def mitchell_sample(table, t0, t1, f32=True):
+    """The upsample's bicubic sample of an L x L x 3 table at (t0, t1) (float32 or float64 arithmetic)."""
+    table = np.ascontiguousarray(table, dtype=np.float64)
+    L = table.shape[0]
+    out = np.zeros(3)
+    lib().oracle_mitchell_sample(_p(table, C.c_double), L, t0, t1, int(f32), _p(out, C.c_double))
+    return out
+
+
+def curve_lookup(x, inv, y, v):
+    """The film density-curve lookup (clamped piecewise linear, float32)."""
+    x, inv, y = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, inv, y))
+    return float(lib().oracle_curve_lookup(_p(x, C.c_float), _p(inv, C.c_float), _p(y, C.c_float), len(x), v))
+
+
+def paper_lookup(plan, c, v):
+    return float(lib().oracle_paper_lookup(C.cast(plan, C.c_void_p), c, v))
+
+
+def boost(values, ev):
+    """The highlight boost of a plane of exposure values (float32), in place on a copy."""
+    h = np.array(values, dtype=np.float32)
+    lib().oracle_boost(_p(h, C.c_float), h.size, ev)
+    return h
+
+
+def encode16(v, mode, active=True):
+    """The output encode of one value: transfer function `mode` (when active), then 16 bits."""
+    return int(lib().oracle_encode16(v, mode, int(active)))
+
+
+def cam16_forward(plan, xyz, f32=False):
+    """(J', M', h) of an XYZ triple under the plan's viewing conditions."""
+    x = np.ascontiguousarray(xyz, dtype=np.float64)
+    out = np.zeros(3)
+    lib().oracle_cam16_forward(C.cast(plan, C.c_void_p), _p(x, C.c_double), int(f32), _p(out, C.c_double))
+    return out
+
+
+def cam16_inverse(plan, Jp, Mp, h):
+    out = np.zeros(3)
+    lib().oracle_cam16_inverse(C.cast(plan, C.c_void_p), Jp, Mp, h, _p(out, C.c_double))
+    return out
+
+
+def knee(d, tau, lim, pw):
+    return lib().oracle_knee(d, tau, lim, pw)
