@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import subprocess
 import os
 import sys
 import time
@@ -197,23 +198,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as orc
-        cw, ch = (int(v) for v in args.cpu_sample.split("x"))
-        log(f"[bench] CPU baseline: oracle binary32 on a {cw}x{ch} sample, {orc.lib().oracle_threads()} threads")
-        plan = spk.host_plan(res, delta, cw, ch, film_seed=0x4ABEA221, overrides=overrides)
-        sample = np.ascontiguousarray(frame[:ch, :cw])
-        orc.render(plan, sample)  # warm-up pass (page-in, thread pool)
-        c0 = time.perf_counter()
-        reps = 0
-        while True:
-            orc.render(plan, sample)
-            reps += 1
-            if time.perf_counter() - c0 > 10.0 or reps >= 5:
-                break
-        cms = (time.perf_counter() - c0) * 1000.0 / reps
-        cpu = {"value": round(cw * ch / (cms / 1000.0) / 1e6, 3), "unit": "Mpx/s", "cores": orc.lib().oracle_threads(),
-               "kind": "port", "sample": f"{cw}x{ch} crop of the same frame and stock, same chain (binary32 oracle), "
-                                         f"{reps} renders, {cms:.0f} ms each"}
+        cpu = cpu_baseline(res, delta, overrides, frame, args)
 
     line = {
         "metric": METRIC,
@@ -251,6 +236,55 @@ def main():
     print(json.dumps(line), flush=True)
     sess.close()
     eng.close()
+
+
+def host_cpu():
+    """The host's CPU as lscpu and the affinity mask describe it: model, sockets, cores per socket,
+    threads per core, the CPUs this process may use and the physical cores among them."""
+    info = {"nproc": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    info["physical_cores"] = max(1, info["nproc"] // int(info.get("threads_per_core", 1) or 1))
+    return info
+
+
+def cpu_baseline(res, delta, overrides, frame, args):
+    """The CPU oracle (the same chain in binary32 and in binary64) on a crop of the benchmark frame,
+    one OpenMP thread per physical core: one warm-up render, then the median of five."""
+    import spektralab_engine_amd as spk
+    from oracle import oracle as orc
+    host = host_cpu()
+    orc.lib().oracle_set_threads(host["physical_cores"])
+    threads = orc.lib().oracle_threads()
+    cw, ch = (int(v) for v in args.cpu_sample.split("x"))
+    plan = spk.host_plan(res, delta, cw, ch, film_seed=0x4ABEA221, overrides=overrides)
+    sample = np.ascontiguousarray(frame[:ch, :cw])
+    legs = {}
+    for name, f64 in (("f32", False), ("f64", True)):
+        log(f"[bench] CPU baseline {name}: oracle on a {cw}x{ch} sample, {threads} threads")
+        orc.render(plan, sample, f64=f64)  # warm-up (page-in, thread pool)
+        times = []
+        for _ in range(5):
+            c0 = time.perf_counter()
+            orc.render(plan, sample, f64=f64)
+            times.append((time.perf_counter() - c0) * 1000.0)
+        med = float(np.median(times))
+        legs[name] = {"mpx_s": round(cw * ch / (med / 1000.0) / 1e6, 3), "median_ms": round(med, 1),
+                      "min_ms": round(min(times), 1), "max_ms": round(max(times), 1)}
+    return {"value": legs["f32"]["mpx_s"], "unit": "Mpx/s", "cores": threads, "kind": "port",
+            "sample": f"{cw}x{ch} crop of the same frame and stock, same chain; binary32 oracle (value) and "
+                      f"binary64 oracle; median of 5 renders after 1 warm-up; {threads} OpenMP threads, "
+                      f"one per physical core",
+            "legs": legs, "host": host}
 
 
 def batch_measure(eng, res, args):

@@ -1778,8 +1778,10 @@ float oracle_log10f(float x) { return spkm::log10f_(x); }
 float oracle_logf(float x) { return spkm::logf_(x); }
 float oracle_expf(float x) { return spkm::expf_(x); }
 int oracle_threads(void) { return omp_get_max_threads(); }
+void oracle_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+}
 
-// Byte offset of a named spk_plan member (for reading plans from Python); -1 when unknown.
 // ---- single-function probes (known-answer tests of the stages above) ------------------------------
 void oracle_mitchell_sample(const double* table, uint32_t L, double t0, double t1, int f32, double* out3) {
     if (f32) {
@@ -1819,6 +1821,7 @@ void oracle_cam16_inverse(const spk_plan* p, double Jp, double Mp, double h, dou
 }
 double oracle_knee(double d, double tau, double lim, double pw) { return knee<double>(d, tau, lim, pw); }
 
+// Byte offset of a named spk_plan member (for reading plans from Python); -1 when unknown.
 long oracle_plan_offset(const char* name) {
 #define F(m) \
     if (std::strcmp(name, #m) == 0) return long(offsetof(spk_plan, m));

@@ -74,6 +74,8 @@ def lib():
         l.oracle_cam16_inverse.restype = None
         l.oracle_knee.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
         l.oracle_knee.restype = C.c_double
+        l.oracle_set_threads.argtypes = [C.c_int]
+        l.oracle_set_threads.restype = None
         l.oracle_frozen_f64.argtypes = [V, F, F, F, C.c_int, C.c_double, D, D, D]
         l.oracle_frozen_f64.restype = C.c_double
         l.oracle_frozen_grad.argtypes = [V, F, F, F, D, D, C.c_int, D]
