@@ -14,6 +14,13 @@ namespace spk {
 
 namespace {
 
+// Waits for a stream when it goes out of scope: declared after a call's scratch list, it keeps the
+// scratch from returning to the pool while the stream may still use it.
+struct StreamDrain {
+    hipStream_t s;
+    ~StreamDrain() { (void)hipStreamSynchronize(s); }
+};
+
 constexpr double kLog2Of10 = 3.32192809488736234787;
 constexpr double kSigmaStep = 0.01;  // relative step of the central differences in sigma
 
@@ -221,6 +228,7 @@ void Diff::forward(int slot, const double* theta, const float* input, float* enc
         return gpu::as<float>(b);
     };
     std::vector<gpu::DevBuf> keep;
+    const StreamDrain drain{s};  // the scratch goes back to the pool after the stream, on a failure too
 
     float* H = plane(sl.H, 3 * np);
     k::upsample(input, H, gpu::as<float>(t.table), upsample_args(th), s);
@@ -300,6 +308,7 @@ void Diff::backward(int slot, const float* input, const float* g_enc, float* g_i
     const uint32_t n = uint32_t(np);
     const std::vector<double>& th = sl.theta;
     std::vector<gpu::DevBuf> keep;
+    const StreamDrain drain{s};
     auto buf = [&](size_t floats) {
         gpu::DevBuf b = pool.acquire(floats * sizeof(float));
         keep.push_back(b);

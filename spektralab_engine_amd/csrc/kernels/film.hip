@@ -63,6 +63,46 @@ __device__ inline void mitchell_axis(float t, uint32_t L, int* idx, float* w) {
     w[3] = mitchell(frac - 2.0f);
 }
 
+// The same weights with each tap's polynomial chosen from the range its argument takes: frac + 1 and
+// 2 - frac lie in [1, 2] (0 at exactly 2), frac and 1 - frac in [0, 1] (the outer piece at exactly
+// 1).  The arguments and polynomials are mitchell()'s own, so the weights are bit-identical; only
+// the piece a lane cannot use is no longer evaluated.
+__device__ inline float mitchell_inner(float a) { return (1.0f / 6.0f) * (7.0f * a * a * a - 12.0f * a * a + 16.0f / 3.0f); }
+__device__ inline float mitchell_outer(float a) {
+    return (1.0f / 6.0f) * ((-7.0f / 3.0f) * a * a * a + 12.0f * a * a - 20.0f * a + 32.0f / 3.0f);
+}
+__device__ inline void mitchell_axis_fast(float t, uint32_t L, uint32_t* idx, float* w) {
+    const float last = float(L - 1);
+    float u = t * last;
+    int base;
+    float frac;
+    if (u <= 0.0f) u = 0.0f;
+    if (u >= last) {
+        base = int(L) - 2;
+        frac = 1.0f;
+    } else {
+        base = int(floorf(u));
+        frac = u - float(base);
+    }
+    const int n = int(L);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int i = base - 1 + k;
+        if (i < 0) i = -i;
+        if (i >= n) i = 2 * (n - 1) - i;
+        idx[k] = uint32_t(i);
+    }
+    const float a0 = fabsf(frac + 1.0f), a1 = fabsf(frac), a2 = fabsf(frac - 1.0f), a3 = fabsf(frac - 2.0f);
+    w[0] = a0 < 2.0f ? mitchell_outer(a0) : 0.0f;
+    w[3] = a3 < 2.0f ? mitchell_outer(a3) : 0.0f;
+    w[1] = mitchell_inner(a1);
+    w[2] = mitchell_inner(a2);
+    if (!(a1 < 1.0f) || !(a2 < 1.0f)) {  // frac exactly 1 or 0
+        if (!(a1 < 1.0f)) w[1] = mitchell_outer(a1);
+        if (!(a2 < 1.0f)) w[2] = mitchell_outer(a2);
+    }
+}
+
 // F(v; x, inv, y): clamped piecewise-linear between the samples around v (kernels/film.metal:88-101).
 __device__ inline float curve_lookup(const float* __restrict__ x, const float* __restrict__ inv,
                                      const float* __restrict__ y, uint32_t K, float v, const uint16_t* hint,
@@ -90,6 +130,16 @@ __device__ inline float fdiv_rn(float a, float b) {
     const float q1 = fmaf(r0, y1, q0);
     const float r1 = fmaf(-b, q1, a);
     return fmaf(r1, y1, q1);
+}
+
+// a / b correctly rounded for any operands: fdiv_rn where its conditions hold (a == 0, or a
+// quotient of normal magnitude well inside the range, which with a finite b bounds the
+// intermediates too), the full division otherwise (zero, subnormal or huge quotients, NaN, inf).
+__device__ inline float fdiv_any(float a, float b) {
+    const float q = fdiv_rn(a, b);
+    const float m = fabsf(q);
+    if (a == 0.0f || (m >= 0x1p-100f && m <= 0x1p100f && fabsf(b) >= 0x1p-100f && fabsf(b) <= 0x1p100f)) return q;
+    return a / b;
 }
 
 // Hormann's PTRS constants for one lam >= 10 (reference kernels/fx_random.h:52-58), with
@@ -451,31 +501,37 @@ __global__ __launch_bounds__(256) void k_upsample(const float* __restrict__ rgb,
     const float Z = (q.a[6] * r + q.a[7] * g) + q.a[8] * b;
     const float bs = (X + Y) + Z;
     const float d = bs > 1e-10f ? bs : 1e-10f;
-    const float x = X / d, y = Y / d, o = 1.0f - x;
+    const float x = fdiv_any(X, d), y = fdiv_any(Y, d), o = 1.0f - x;
     const float t0 = fminf(fmaxf(o * o, 0.0f), 1.0f);
-    const float t1 = fminf(fmaxf(y / (o > 1e-10f ? o : 1e-10f), 0.0f), 1.0f);
+    const float t1 = fminf(fmaxf(fdiv_any(y, o > 1e-10f ? o : 1e-10f), 0.0f), 1.0f);
     const float bp = isnan(bs) ? 0.0f : bs;
-    int ia[4], ib[4];
+    uint32_t ia[4], ib[4];
     float wa[4], wb[4];
-    mitchell_axis(t0, q.L, ia, wa);
-    mitchell_axis(t1, q.L, ib, wb);
+    mitchell_axis_fast(t0, q.L, ia, wa);
+    mitchell_axis_fast(t1, q.L, ib, wb);
+    // table entries by 32-bit byte offsets from the table's base (the table is under 1 MB)
+    struct Rgb {
+        float v[3];
+    };
+    const char* tb = reinterpret_cast<const char*>(table);
     float s = 0.0f, a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
 #pragma unroll
     for (int ii = 0; ii < 4; ++ii) {
+        const uint32_t row = ia[ii] * q.L;
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj) {
             const float w = wa[ii] * wb[jj];
-            const float* t = table + 3 * (size_t(ia[ii]) * q.L + size_t(ib[jj]));
+            const Rgb t = *reinterpret_cast<const Rgb*>(tb + (row + ib[jj]) * 12u);
             s += w;
-            a0 += w * t[0];
-            a1 += w * t[1];
-            a2 += w * t[2];
+            a0 += w * t.v[0];
+            a1 += w * t.v[1];
+            a2 += w * t.v[2];
         }
     }
     if (s != 0.0f) {
-        a0 = a0 / s;
-        a1 = a1 / s;
-        a2 = a2 / s;
+        a0 = fdiv_any(a0, s);
+        a1 = fdiv_any(a1, s);
+        a2 = fdiv_any(a2, s);
     }
     float h0 = bp * a0, h1 = bp * a1, h2 = bp * a2;
     if (q.ec_active) {
