@@ -451,6 +451,7 @@ Tables upload_tables(const spk_plan& p, gpu::Pool& pool) {
 
     k::PrintConsts q{};
     q.paper_k = p.paper_k;
+    axis_hints(p.paper, 1, p.paper_k, q.paper_hints, q.paper_hint_lo, q.paper_hint_scale);
     for (int i = 0; i < SPK_NL3; ++i) q.scan_eps[i] = num::f32(-p.scan_eps64[i] * kLog2Of10);
     for (int i = 0; i < SPK_NL; ++i) q.scan_beta[i] = num::f32(-p.scan_beta64[i] * kLog2Of10);
     std::memcpy(q.scan_wt, p.scan_wt, sizeof q.scan_wt);

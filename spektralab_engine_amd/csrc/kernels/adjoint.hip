@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void k_adj_print_bwd(const float* __restrict__
             lg[c] = flog10(fmaxf(y[c], 0.0f) + kTiny);
             z[c] = fexp2(lg[c] * kLog2Of10) * e.k_gain;
             lp[c] = flog10(fmaxf(z[c], 0.0f) + kTiny);
-            Dp[c] = paper_lookup(paper, q.paper_k, uint32_t(c), e.pg[c] * lp[c] + e.po[c], &slope[c]);
+            Dp[c] = paper_lookup(paper, q, uint32_t(c), e.pg[c] * lp[c] + e.po[c], &slope[c]);
         }
         // scan integral
         float X[3] = {0.0f, 0.0f, 0.0f};

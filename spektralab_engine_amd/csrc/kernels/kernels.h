@@ -127,6 +127,9 @@ struct PrintConsts {
     uint32_t edr_active, edr_n;  // EDR tone map before the output matrix (table of edr_n log2 targets)
     float edr_r_y[3], edr_domain_lo, edr_domain_span, edr_toe_y, edr_shoulder_y;
     float edr_table[SPK_PLAN_MAX_EDR];
+    // search hints of the paper curve axis (see kHintBins); paper_hint_scale 0: bisect instead
+    float paper_hint_lo, paper_hint_scale;
+    uint16_t paper_hints[kHintBins];
 };
 
 // One 81-bin spectral integral R_c = sum_l exp2(e(l) . d + beta(l)) wt_c(l) with the products and

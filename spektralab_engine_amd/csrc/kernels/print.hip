@@ -6,7 +6,12 @@
 //                      (:249-258), CAM16-UCS gamut compression (:267-317).  The 81-bin loops run
 //                      over wavelength ascending with the tables read through the scalar cache
 //                      (uniform addresses); the density tables arrive pre-scaled by -log2(10), so
-//                      each bin is one exp2 of a 3-term dot product on the hardware exp2.
+//                      each bin is one exp2 of a 3-term dot product on the hardware exp2: three
+//                      FMAs from the bin's base density, the exp2 (quarter rate) and three FMAs
+//                      into the response, about 41 cycles a bin for a wavefront.  The matrix cores
+//                      do not shorten it: both contractions as chains of v_mfma_f32_4x4x1_16b_f32
+//                      (one pixel a lane, identical bits) measured 44-46 cycles a bin, the MFMA
+//                      issue not overlapping the exp2 and FMAs of the other wavefronts.
 //   k_unsharp_encode   rgb := (1 + a) rgb - a G(rgb) (src/fx/scanner.cpp:16-33) fused with the
 //                      near-identity matrix, transfer function and 16-bit encode (:350-366).
 // Float contraction is allowed here (the output is compared to the CPU oracle within 1 LSB).
@@ -41,16 +46,15 @@ __global__ __launch_bounds__(256) void k_glare_field(float* __restrict__ out, ui
 
 // Enlarger response R -> log10 exposure on the paper -> print gain -> paper densities
 // (print.metal:192-232).
-__device__ inline F3 expose_paper(float r0, float r1, float r2, CEA& e, const float* __restrict__ paper,
-                                  uint32_t K) {
+__device__ inline F3 expose_paper(float r0, float r1, float r2, CEA& e, const float* __restrict__ paper, CPC& q) {
     float l0 = flog10(fmaxf(r0 * e.g[0] + e.o[0], 0.0f) + kTiny);
     float l1 = flog10(fmaxf(r1 * e.g[1] + e.o[1], 0.0f) + kTiny);
     float l2 = flog10(fmaxf(r2 * e.g[2] + e.o[2], 0.0f) + kTiny);
     l0 = flog10(fmaxf(fexp2(l0 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
     l1 = flog10(fmaxf(fexp2(l1 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
     l2 = flog10(fmaxf(fexp2(l2 * kLog2Of10) * e.k_gain, 0.0f) + kTiny);
-    return {paper_lookup(paper, K, 0, e.pg[0] * l0 + e.po[0]), paper_lookup(paper, K, 1, e.pg[1] * l1 + e.po[1]),
-            paper_lookup(paper, K, 2, e.pg[2] * l2 + e.po[2])};
+    return {paper_lookup(paper, q, 0, e.pg[0] * l0 + e.po[0]), paper_lookup(paper, q, 1, e.pg[1] * l1 + e.po[1]),
+            paper_lookup(paper, q, 2, e.pg[2] * l2 + e.po[2])};
 }
 
 
@@ -100,12 +104,12 @@ __device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __
         float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
 #pragma unroll 9
         for (uint32_t l = 0; l < SPK_NL; ++l) {
-            const float T = fexp2(((d0 * e.eps[3 * l] + d1 * e.eps[3 * l + 1]) + d2 * e.eps[3 * l + 2]) + e.beta[l]);
+            const float T = fexp2(fmaf(d2, e.eps[3 * l + 2], fmaf(d1, e.eps[3 * l + 1], fmaf(d0, e.eps[3 * l], e.beta[l]))));
             r0 += T * e.wt[3 * l];
             r1 += T * e.wt[3 * l + 1];
             r2 += T * e.wt[3 * l + 2];
         }
-        const F3 d = expose_paper(r0, r1, r2, e, paper, q.paper_k);
+        const F3 d = expose_paper(r0, r1, r2, e, paper, q);
         d0 = d.x;
         d1 = d.y;
         d2 = d.z;
@@ -116,8 +120,8 @@ __device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __
         float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
 #pragma unroll 9
         for (uint32_t l = 0; l < SPK_NL; ++l) {
-            const float T = fexp2(((d0 * q.scan_eps[3 * l] + d1 * q.scan_eps[3 * l + 1]) + d2 * q.scan_eps[3 * l + 2]) +
-                                  q.scan_beta[l]);
+            const float T =
+                fexp2(fmaf(d2, q.scan_eps[3 * l + 2], fmaf(d1, q.scan_eps[3 * l + 1], fmaf(d0, q.scan_eps[3 * l], q.scan_beta[l]))));
             r0 += T * q.scan_wt[3 * l];
             r1 += T * q.scan_wt[3 * l + 1];
             r2 += T * q.scan_wt[3 * l + 2];
@@ -213,7 +217,7 @@ __global__ __launch_bounds__(256) void k_print16(const float* __restrict__ neg, 
     const float lo = enl->lo, hi = enl->hi;
     const F3 r = spectral_mfma(enl, cref(e).eps, fmaxf(fminf(neg[p], hi), lo),
                                fmaxf(fminf(neg[size_t(n) + p], hi), lo), fmaxf(fminf(neg[2 * size_t(n) + p], hi), lo), lane);
-    const F3 d = expose_paper(r.x, r.y, r.z, cref(e), paper, q.paper_k);
+    const F3 d = expose_paper(r.x, r.y, r.z, cref(e), paper, q);
     const float sl = scn->lo, sh = scn->hi;
     const F3 x = spectral_mfma(scn, q.scan_eps, fmaxf(fminf(d.x, sh), sl), fmaxf(fminf(d.y, sh), sl),
                                fmaxf(fminf(d.z, sh), sl), lane);

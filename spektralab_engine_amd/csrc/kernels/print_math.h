@@ -208,10 +208,11 @@ __device__ inline S3<S> cam16_inverse(S Jp, S Mp, S h, CPC& q, const Cam16Derive
     return {x100.x / 100.0f, x100.y / 100.0f, x100.z / 100.0f};
 }
 
-// Clamped paper-curve lookup with a right bisection; NaN passes (print.metal:201-232).  slope
-// (when given) receives dP/dv: 0 outside the axis, the segment's slope inside.
-__device__ inline float paper_lookup(const float* __restrict__ tab, uint32_t K, uint32_t c, float v,
-                                     float* slope = nullptr) {
+// Clamped paper-curve lookup; NaN passes (print.metal:201-232).  The segment is the one a right
+// bisection finds (upper bound of v less one), reached from the search hint of v's bin when the
+// axis has hints.  slope (when given) receives dP/dv: 0 outside the axis, the segment's slope inside.
+__device__ inline float paper_lookup(const float* __restrict__ tab, uint32_t K, uint32_t c, float v, float* slope,
+                                     const uint16_t SPK_CONST* hint, float hint_lo, float hint_scale) {
     const float* x = tab;
     const float* r = tab + K;
     const float* y = tab + 2 * K;
@@ -219,16 +220,28 @@ __device__ inline float paper_lookup(const float* __restrict__ tab, uint32_t K, 
     if (isnan(v)) return v;
     if (v <= x[0]) return y[c];
     if (v >= x[K - 1]) return y[3 * (K - 1) + c];
-    uint32_t lo = 0, hi = K;
-    while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (x[mid] <= v) lo = mid + 1;
-        else hi = mid;
+    uint32_t lo;
+    if (hint_scale > 0.0f) {  // hinted: start at the hint of v's bin, walk to the upper bound
+        const float f = fminf(fmaxf((v - hint_lo) * hint_scale, 0.0f), float(kHintBins - 1));
+        lo = min(uint32_t(hint[uint32_t(f)]), K - 1);
+        while (lo < K - 1 && x[lo] <= v) ++lo;
+        while (lo > 0 && x[lo - 1] > v) --lo;
+    } else {
+        uint32_t b = 0, e = K;
+        while (b < e) {
+            const uint32_t mid = b + (e - b) / 2;
+            if (x[mid] <= v) b = mid + 1;
+            else e = mid;
+        }
+        lo = b;
     }
     const uint32_t j = lo > 0 ? lo - 1 : 0;  // lo == 0 only on a non-monotone axis
     const float y0 = y[3 * j + c];
     if (slope) *slope = r[j] * (y[3 * (j + 1) + c] - y0);
     return y0 + ((v - x[j]) * r[j]) * (y[3 * (j + 1) + c] - y0);
+}
+__device__ inline float paper_lookup(const float* __restrict__ tab, CPC& q, uint32_t c, float v, float* slope = nullptr) {
+    return paper_lookup(tab, q.paper_k, c, v, slope, q.paper_hints, q.paper_hint_lo, q.paper_hint_scale);
 }
 
 __device__ inline float cctf_encode(float v, uint32_t mode) {
