@@ -24,7 +24,7 @@ extern "C" {
 #endif
 
 #define SPK_PLAN_MAGIC 0x504B5053u /* "SPKP" */
-#define SPK_PLAN_VERSION 5u
+#define SPK_PLAN_VERSION 6u
 
 #define SPK_NL 81        /* wavelength samples 380..780 nm, 5 nm */
 #define SPK_NL3 243      /* [lambda][channel] */
@@ -99,11 +99,18 @@ typedef struct spk_plan {
     uint32_t grain_active;  /* draws happen */
     uint32_t grain_mode;    /* SPK_GRAIN_* */
     uint32_t grain_positive, grain_shift, grain_seed, grain_k, grain_n_sub;
-    uint32_t grain_dye;     /* dye-cloud planes (not produced by the configured stocks) */
+    uint32_t grain_dye;     /* dye clouds: PLANES draw, plane l blurred with grain_dye_sigma[l*3 + c], summed */
     float grain_inv_n_sub;
-    float dmin_l[9], sat_d[9], count[9], uniformity[3], dmin_c[3];  /* [l*3 + c] */
+    float dmin_l[9], sat_d[9], count[9], uniformity[3], dmin_c[3];  /* [l*3 + c]; SIMPLE uses [c] */
     double grain_final_blur;  /* sigma px, 0 = none */
     double grain_amount;
+    /* micro-structure: G *= phi, phi := lognormal(mean 1, std grain_micro_sigma) per pixel and
+     * channel c on stream 100 + c with the film seed, blurred with grain_micro_blur px when that
+     * exceeds 0.4; with dye clouds or micro-structure the D_min shift follows: G += -dmin_c */
+    uint32_t grain_micro;
+    float grain_micro_sigma;
+    double grain_micro_blur;
+    double grain_dye_sigma[9];  /* [l*3 + c] px, 0 = that plane passes unblurred */
 
     /* ---- print side: enlarger, gain, paper, scan, levels, to_rgb, gamut, unsharp, encode ---- */
     uint32_t printed;       /* !scan_film */

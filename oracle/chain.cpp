@@ -14,7 +14,8 @@
 //                        accumulation fused into the last pass), mixtures :254-273
 //   halation             src/fx/halation.cpp:307-329 (scatter lincomb, renormalised bounce mix)
 //   log, curves          kernels/film.metal:88-101, :242-261
-//   grain                kernels/fx_grain.metal:39-97 (SUM mode + shift), kernels/fx_random.h:19-99
+//   grain                kernels/fx_grain.metal:39-116 (SUM, PLANES, SIMPLE; lognormal), kernels/fx_random.h:19-115
+//                        src/fx/grain.cpp:213-244 (dye clouds, micro-structure, the D_min shift)
 //                        (lgamma, Poisson PTRS / multiplication, sub-layer draw),
 //                        kernels/spk_common.h:23-67 (Philox4x32-10, word stream, uniform map),
 //                        src/fx/grain.cpp:245-266 (final 0.65 px blur, strength mix)
@@ -784,9 +785,10 @@ float sublayer_draw(float d, float D, float n, float u, uint32_t pixel, uint32_t
     return k * ((D / n) * sat);
 }
 
-// Grain SUM draw of one pixel and channel from its density D (binary32 by definition).
-float grain_sum(const spk_plan& p, float D, uint32_t pixel, uint32_t c, float* counts9 = nullptr,
-                float* lams9 = nullptr) {
+// The three sub-layer draws of one pixel and channel from its density D (binary32 by definition):
+// val[l] = sublayer_draw of layer curve l at D (kernels/fx_grain.metal:66-85).
+void grain_layers(const spk_plan& p, float D, uint32_t pixel, uint32_t c, float val[3], float* counts9 = nullptr,
+                  float* lams9 = nullptr) {
     const float v = p.grain_positive ? -D : D;
     const uint32_t K = p.grain_k;
     const float* axis = p.grain_axis;
@@ -812,14 +814,34 @@ float grain_sum(const spk_plan& p, float D, uint32_t pixel, uint32_t c, float* c
             d[l] = y0 + t * (y1 - y0);
         }
     }
-    float val[3];
     for (uint32_t l = 0; l < 3; ++l) {
         const uint32_t lc = l * 3 + c;
         val[l] = sublayer_draw(d[l] + p.dmin_l[lc], p.sat_d[lc], p.count[lc], p.uniformity[c], pixel, c + 10u * l, p.grain_seed,
                                counts9 ? counts9 + 3 * c + l : nullptr, lams9 ? lams9 + 3 * c + l : nullptr);
     }
+}
+
+// Grain SUM draw (kernels/fx_grain.metal:90-95): the three sub-layer values summed, shifted by
+// -D_min,c when the plan says so (without dye clouds or micro-structure).
+float grain_sum(const spk_plan& p, float D, uint32_t pixel, uint32_t c, float* counts9 = nullptr,
+                float* lams9 = nullptr) {
+    float val[3];
+    grain_layers(p, D, pixel, c, val, counts9, lams9);
     float G = ((0.0f + val[0]) + val[1]) + val[2];
-    return G * 1.0f + (-p.dmin_c[c]);
+    if (p.grain_shift) G = G * 1.0f + (-p.dmin_c[c]);
+    return G;
+}
+
+// Grain SIMPLE draw (kernels/fx_grain.metal:55-64): N_sub draws on D + D_min,c on streams c + 10 l,
+// G := (sum) (1 / N_sub) - D_min,c.
+float grain_simple(const spk_plan& p, float D, uint32_t pixel, uint32_t c, float* counts9 = nullptr,
+                   float* lams9 = nullptr) {
+    const float d = D + p.dmin_c[c];
+    float G = 0.0f;
+    for (uint32_t l = 0; l < p.grain_n_sub; ++l)
+        G = G + sublayer_draw(d, p.sat_d[c], p.count[c], p.uniformity[c], pixel, c + 10u * l, p.grain_seed,
+                              counts9 && l < 3 ? counts9 + 3 * c + l : nullptr, lams9 && l < 3 ? lams9 + 3 * c + l : nullptr);
+    return G * p.grain_inv_n_sub + (-p.dmin_c[c]);
 }
 
 // The grain sum with the Poisson counts frozen (k9[3 c + l], as a render drew them): each
@@ -1070,6 +1092,9 @@ void develop(const spk_plan& p, const In* rgb, T* neg, const Theta<T>* thp = nul
                      p.fir_crossover);
     }
     std::vector<T> G(p.grain_active ? 3 * np : 0), D(p.grain_active ? 3 * np : 0);
+    std::vector<T> V(p.grain_active && p.grain_mode == SPK_GRAIN_PLANES ? 9 * np : 0);  // [(3 l + c) np + i]
+    if (counts_in && p.grain_active && (p.grain_mode != SPK_GRAIN_SUM || p.grain_micro))
+        throw std::runtime_error("frozen grain counts: the sub-layer SUM model only");
     for (int c = 0; c < 3; ++c) {
         const T ma = th.mix_a[c], mb = th.mix_b[c];
 #pragma omp parallel for schedule(dynamic, 4096)
@@ -1088,15 +1113,59 @@ void develop(const spk_plan& p, const In* rgb, T* neg, const Theta<T>* thp = nul
                 neg[o] = Dv;
             } else {
                 D[o] = Dv;
-                if (counts_in)
+                float* k9 = counts_out ? counts_out + 9 * size_t(i) : nullptr;
+                float* l9 = lams_out ? lams_out + 9 * size_t(i) : nullptr;
+                if (counts_in) {
                     G[o] = grain_frozen<T>(p, Dv, counts_in + 9 * size_t(i), uint32_t(c));
-                else
-                    G[o] = T(grain_sum(p, float(Dv), uint32_t(i), uint32_t(c), counts_out ? counts_out + 9 * size_t(i) : nullptr,
-                                       lams_out ? lams_out + 9 * size_t(i) : nullptr));
+                } else if (p.grain_mode == SPK_GRAIN_SIMPLE) {
+                    G[o] = T(grain_simple(p, float(Dv), uint32_t(i), uint32_t(c), k9, l9));
+                } else if (p.grain_mode == SPK_GRAIN_PLANES) {
+                    float val[3];
+                    grain_layers(p, float(Dv), uint32_t(i), uint32_t(c), val, k9, l9);
+                    for (int l = 0; l < 3; ++l) V[size_t(3 * l + c) * np + size_t(i)] = T(val[l]);
+                } else {
+                    G[o] = T(grain_sum(p, float(Dv), uint32_t(i), uint32_t(c), k9, l9));
+                }
             }
         }
     }
     if (!p.grain_active) return;
+    const double one3[3] = {1, 1, 1};
+    // dye clouds (src/fx/grain.cpp:219-229): G := sum_l G_{sigma_dye[l]}(plane l)
+    if (p.grain_dye) {
+        for (int l = 0; l < 3; ++l) {
+            const double sg[3] = {p.grain_dye_sigma[3 * l], p.grain_dye_sigma[3 * l + 1], p.grain_dye_sigma[3 * l + 2]};
+            blur3<T>(V.data() + size_t(3 * l) * np, G.data(), sg, l == 0 ? 1 : 2, G.data(), one3, w, h, p.fir_crossover);
+        }
+    }
+    // micro-structure (src/fx/grain.cpp:230-241): phi_c := lognormal(1, sigma) of pixel i on stream
+    // 100 + c, blurred above 0.4 px; G := G phi; then (dye or micro) G := G - D_min,c
+    if (p.grain_dye || p.grain_micro) {
+        std::vector<T> phi;
+        if (p.grain_micro) {
+            phi.resize(3 * np);
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < int64_t(np); ++i)
+                for (uint32_t c = 0; c < 3; ++c) {
+                    Stream g{uint32_t(i), 100u + c, p.grain_seed};
+                    const float u1 = g.uniform(), u2 = g.uniform();
+                    phi[size_t(c) * np + size_t(i)] = T(spkm::lognormal_f(1.0f, p.grain_micro_sigma, u1, u2));
+                }
+            if (p.grain_micro_blur > 0.4) {
+                const double mb[3] = {p.grain_micro_blur, p.grain_micro_blur, p.grain_micro_blur};
+                std::vector<T> t(phi);
+                blur3<T>(t.data(), phi.data(), mb, 0, nullptr, one3, w, h, p.fir_crossover);
+            }
+        }
+        for (int c = 0; c < 3; ++c) {
+            const T sh = T(-p.dmin_c[c]);
+            for (size_t i = 0; i < np; ++i) {
+                T g = G[size_t(c) * np + i];
+                if (p.grain_micro) g = g * phi[size_t(c) * np + i];
+                G[size_t(c) * np + i] = g + sh;
+            }
+        }
+    }
     std::vector<T> Gb(3 * np);
     if (p.grain_final_blur > 0.0) {
         const double sg[3] = {p.grain_final_blur, p.grain_final_blur, p.grain_final_blur};
@@ -1820,6 +1889,18 @@ void oracle_cam16_inverse(const spk_plan* p, double Jp, double Mp, double h, dou
     cam16_inverse(Jp, Mp, h, q, xyz3);
 }
 double oracle_knee(double d, double tau, double lim, double pw) { return knee<double>(d, tau, lim, pw); }
+// The binary64 cosine and the micro-structure lognormal (kernels/fx_random.h:100-115) the film side uses.
+double oracle_cos(double x) { return spkm::cos_d(x); }
+float oracle_lognormal(float m, float s, float u1, float u2) { return spkm::lognormal_f(m, s, u1, u2); }
+// The micro-structure field of pixels 0 .. n-1: 3 planes of lognormal(1, sigma) on streams 100 + c.
+void oracle_micro_field(uint32_t n, float sigma, uint32_t seed, float* out) {
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t c = 0; c < 3; ++c) {
+            Stream g{i, 100u + c, seed};
+            const float u1 = g.uniform(), u2 = g.uniform();
+            out[size_t(c) * n + i] = spkm::lognormal_f(1.0f, sigma, u1, u2);
+        }
+}
 
 // Byte offset of a named spk_plan member (for reading plans from Python); -1 when unknown.
 long oracle_plan_offset(const char* name) {
@@ -1827,7 +1908,8 @@ long oracle_plan_offset(const char* name) {
     if (std::strcmp(name, #m) == 0) return long(offsetof(spk_plan, m));
     F(width) F(height) F(pitch_um) F(rho_tier) F(fir_crossover) F(L) F(K) F(a) F(a64) F(ec_gain)
     F(scatter_active) F(n_scatter) F(scatter) F(bounce_active) F(n_bounce) F(bounce) F(mix_a) F(mix_b)
-    F(grain_active) F(grain_mode) F(grain_seed) F(grain_k) F(dmin_l) F(sat_d) F(count) F(uniformity) F(dmin_c)
+    F(grain_shift) F(grain_dye) F(grain_micro) F(grain_micro_sigma) F(grain_micro_blur) F(grain_dye_sigma) F(grain_n_sub)
+    F(grain_inv_n_sub) F(grain_active) F(grain_mode) F(grain_seed) F(grain_k) F(dmin_l) F(sat_d) F(count) F(uniformity) F(dmin_c)
     F(ae_active) F(ae_gain) F(ae_ev) F(couplers_active) F(n_dir) F(dir) F(glare_active) F(glare_seed) F(glare_sigma)
     F(grain_final_blur) F(grain_amount) F(grain_axis) F(grain_inv) F(grain_layers) F(printed) F(paper_k) F(spectral_fp16) F(film_gamma) F(film_offset) F(paper_gamma) F(paper_offset)
     F(enl_eps64) F(enl_beta64) F(enl_wt64) F(enl_g64)

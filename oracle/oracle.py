@@ -74,6 +74,12 @@ def lib():
         l.oracle_cam16_inverse.restype = None
         l.oracle_knee.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
         l.oracle_knee.restype = C.c_double
+        l.oracle_cos.argtypes = [C.c_double]
+        l.oracle_cos.restype = C.c_double
+        l.oracle_lognormal.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float]
+        l.oracle_lognormal.restype = C.c_float
+        l.oracle_micro_field.argtypes = [C.c_uint32, C.c_float, C.c_uint32, F]
+        l.oracle_micro_field.restype = None
         l.oracle_set_threads.argtypes = [C.c_int]
         l.oracle_set_threads.restype = None
         l.oracle_frozen_f64.argtypes = [V, F, F, F, C.c_int, C.c_double, D, D, D]
@@ -104,7 +110,8 @@ _DT = {"width": "u4", "height": "u4", "L": "u4", "K": "u4", "scatter_active": "u
        "decode_active": "u4", "decode_mode": "u4", "geometry_active": "u4", "src_w": "u4", "src_h": "u4",
        "boost_active": "u4", "headroom_active": "u4", "levels_active": "u4", "levels_m": "f4", "levels_q": "f4",
        "edr_active": "u4", "edr_n": "u4", "edr_table": "f4", "edr_r_y": "f4", "coupler_m": "f4",
-       "coupler_dmax": "f4", "coupler_positive": "u4"}
+       "coupler_dmax": "f4", "coupler_positive": "u4", "grain_shift": "u4", "grain_dye": "u4", "grain_micro": "u4",
+       "grain_micro_sigma": "f4", "grain_n_sub": "u4", "grain_inv_n_sub": "f4"}
 
 
 def field(plan, name, count=1):
@@ -114,6 +121,15 @@ def field(plan, name, count=1):
         raise KeyError(name)
     dt = np.dtype(_DT.get(name, "f8"))
     return np.frombuffer(bytes(plan[off:off + dt.itemsize * count]), dtype=dt)
+
+
+def set_field(plan, name, values):
+    """Overwrites a member of a writable spk_plan buffer (a test's edited copy of a plan)."""
+    off = lib().oracle_plan_offset(name.encode())
+    if off < 0:
+        raise KeyError(name)
+    v = np.atleast_1d(np.asarray(values, dtype=np.dtype(_DT.get(name, "f8"))))
+    np.frombuffer(plan, dtype=np.uint8)[off:off + v.nbytes] = np.frombuffer(v.tobytes(), dtype=np.uint8)
 
 
 def plan_size(plan) -> tuple[int, int]:
@@ -354,3 +370,20 @@ def cam16_inverse(plan, Jp, Mp, h):
 
 def knee(d, tau, lim, pw):
     return lib().oracle_knee(d, tau, lim, pw)
+
+
+def cos(x):
+    """The film side's binary64 cosine (Cody-Waite reduction by pi/2, Taylor series)."""
+    return lib().oracle_cos(float(x))
+
+
+def lognormal(m, s, u1, u2):
+    """The micro-structure lognormal draw from the uniforms u1, u2 (binary32)."""
+    return float(lib().oracle_lognormal(m, s, u1, u2))
+
+
+def micro_field(n, sigma, seed):
+    """The micro-structure field of pixels 0..n-1: (3, n) float32, lognormal(1, sigma) on streams 100 + c."""
+    out = np.zeros(3 * n, dtype=np.float32)
+    lib().oracle_micro_field(n, sigma, seed, _p(out, C.c_float))
+    return out.reshape(3, n)

@@ -59,6 +59,9 @@ std::unique_ptr<Diff> Diff::create(Session& s, int frames, const char* options) 
     if (p.spectral_fp16) fail_user("differentiable render: spectral_precision must be fp32");
     if (p.grain_active && p.grain_amount != 1.0)
         fail_user("differentiable render: grain_amount must be 1 (the blend with the density is not differentiated)");
+    if (p.grain_active && (p.grain_mode != SPK_GRAIN_SUM || p.grain_micro))
+        fail_user("differentiable render: grain is differentiated for the sub-layer model without dye clouds or "
+                  "micro-structure only (set grain_sublayers_active, or a larger pixel pitch)");
     if (p.bounce_active && p.n_bounce > 3) fail_user("differentiable render: at most 3 bounce components");
     if (p.decode_active || p.geometry_active || p.boost_active || p.headroom_active || p.lens_blur_px > 0.0 ||
         p.levels_active || p.edr_active)
@@ -175,6 +178,9 @@ k::DevelopArgs Diff::develop_args(const std::vector<double>& th) const {
         da.fo[c] = num::f32(th[k::kAdjFilmOffset + c]);
     }
     da.grain_active = p.grain_active;
+    da.grain_mode = p.grain_mode;
+    da.grain_shift = p.grain_shift;
+    da.inv_n_sub = p.grain_inv_n_sub;
     da.seed = p.grain_seed;
     da.positive = p.grain_positive;
     da.gk = p.grain_k;

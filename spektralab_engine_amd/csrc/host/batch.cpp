@@ -252,7 +252,8 @@ std::unique_ptr<Batch::Graph> Batch::capture(const BatchConfig& c) {
         std::vector<chain::PrintStage> pst(n);
         for (uint32_t i : order_) {
             Frame& f = frames_[i];
-            chain::develop_post(*f.bf.plan, dst[i], gpu::as<float>(f.negative), pool, st(i), g->keep);
+            chain::develop_post(*f.bf.plan, dst[i], gpu::as<float>(f.negative), pool, st(i), g->keep, nullptr,
+                                gpu::as<const uint32_t>(dseeds_) + 2 * i);
             chain::print_pre(*f.bf.plan, f.bf.tables, gpu::as<const float>(f.negative), pool, st(i), g->keep, &pst[i],
                              gpu::as<const uint32_t>(dseeds_) + 2 * i + 1, nullptr,
                              gpu::as<const k::SpecMfma>(f.enl_mfma));
@@ -333,7 +334,8 @@ void Batch::capture_streams(const BatchConfig& c, Graph& g, int S, const std::ve
         gpu::check(hipMemcpyAsync(dpi, &prefix[2 * i], 2 * sizeof(uint32_t), hipMemcpyHostToDevice, s), "prefix");
         k::develop_batch(dji, dpi, 1, gpu::as<const uint32_t>(dseeds_) + 2 * i,
                          std::min(grid, std::max(prefix[2 * i + 1], 1u)), c.block, s);
-        chain::develop_post(*f.bf.plan, ds, gpu::as<float>(f.negative), pool, s, g.keep);
+        chain::develop_post(*f.bf.plan, ds, gpu::as<float>(f.negative), pool, s, g.keep, nullptr,
+                            gpu::as<const uint32_t>(dseeds_) + 2 * i);
         chain::PrintStage ps;
         chain::print_pre(*f.bf.plan, f.bf.tables, gpu::as<const float>(f.negative), pool, s, g.keep, &ps,
                          gpu::as<const uint32_t>(dseeds_) + 2 * i + 1, nullptr, gpu::as<const k::SpecMfma>(f.enl_mfma));

@@ -656,6 +656,9 @@ void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* ne
         da.mix_b[c] = num::f32(p.mix_b[c]);
     }
     da.grain_active = p.grain_active;
+    da.grain_mode = p.grain_mode;
+    da.grain_shift = p.grain_shift;
+    da.inv_n_sub = p.grain_inv_n_sub;
     const bool blend = p.grain_active && p.grain_amount != 1.0;
     da.write_density = blend ? 1u : 0u;
     da.seed = p.grain_seed;
@@ -723,6 +726,12 @@ void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* ne
         st->D = gpu::as<float>(Db);
         st->blend = true;
     }
+    if (p.grain_mode == SPK_GRAIN_PLANES) {  // the sub-layer planes, blurred into G afterwards
+        gpu::DevBuf Pb = pool.acquire(9 * np * sizeof(float));
+        keep.push_back(Pb);
+        st->planes = gpu::as<float>(Pb);
+        j.a.planes = st->planes;
+    }
     j.out = st->G;
     j.density = st->D;
     j.gaxis = gpu::as<float>(t.gaxis);
@@ -731,7 +740,7 @@ void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* ne
 }
 
 void develop_post(const spk_plan& p, const DevelopStage& st, float* negative, gpu::Pool& pool, hipStream_t s,
-                  std::vector<gpu::DevBuf>& keep, Profiler* prof) {
+                  std::vector<gpu::DevBuf>& keep, Profiler* prof, const uint32_t* film_seed_dev) {
     if (!p.grain_active) return;
     auto mark = [&](const char* l) {
         if (prof && prof->on) prof->mark(l, s);
@@ -740,10 +749,36 @@ void develop_post(const spk_plan& p, const DevelopStage& st, float* negative, gp
     const size_t np = size_t(w) * h;
     float* G = st.G;
     const bool blend = st.blend;
+    const double one[3] = {1.0, 1.0, 1.0};
+    // dye clouds (grain.cpp:219-229): G := sum over sub-layers l of G_{sigma_dye[l]}(plane l)
+    if (p.grain_dye) {
+        for (uint32_t l = 0; l < 3; ++l) {
+            const double sg[3] = {p.grain_dye_sigma[3 * l], p.grain_dye_sigma[3 * l + 1], p.grain_dye_sigma[3 * l + 2]};
+            blur3(st.planes + size_t(3 * l) * np, G, sg, l == 0 ? 1u : 2u, G, one, w, h, p.fir_crossover, pool, s, keep);
+        }
+        mark("grain_dye");
+    }
+    // micro-structure (grain.cpp:230-241): G := G phi, phi lognormal, blurred above 0.4 px; then
+    // the D_min shift the draw left out
+    if (p.grain_dye || p.grain_micro) {
+        float* phi = nullptr;
+        if (p.grain_micro) {
+            gpu::DevBuf fb = pool.acquire(3 * np * sizeof(float));
+            keep.push_back(fb);
+            phi = gpu::as<float>(fb);
+            k::micro_field(phi, uint32_t(np), p.grain_micro_sigma, p.grain_seed, film_seed_dev, s);
+            if (p.grain_micro_blur > 0.4) {
+                const double mb[3] = {p.grain_micro_blur, p.grain_micro_blur, p.grain_micro_blur};
+                blur3(phi, phi, mb, 0, nullptr, one, w, h, p.fir_crossover, pool, s, keep);
+            }
+        }
+        const float shift[3] = {-p.dmin_c[0], -p.dmin_c[1], -p.dmin_c[2]};
+        k::grain_finish(G, phi, uint32_t(np), shift, s);
+        mark("grain_micro");
+    }
     float* Gout = blend ? G : negative;
     if (p.grain_final_blur > 0.0) {
         const double sg[3] = {p.grain_final_blur, p.grain_final_blur, p.grain_final_blur};
-        const double one[3] = {1.0, 1.0, 1.0};
         blur3(G, Gout, sg, 0, nullptr, one, w, h, p.fir_crossover, pool, s, keep);
     } else if (Gout != G) {
         k::copy(G, Gout, 3 * np, s);
@@ -783,6 +818,8 @@ size_t scratch_bound(const spk_plan& p, bool develop, bool tables, bool negative
         if (p.bounce_active) n += p.n_bounce * b3;
         if (p.couplers_active) n += 3 * blk(3 * plane) + p.n_dir * b3;
         if (p.grain_active) n += 2 * blk(3 * plane) + b3;  // G, the pre-grain density, the final blur
+        if (p.grain_dye) n += blk(9 * plane) + 3 * b3;    // the sub-layer planes and their blurs
+        if (p.grain_micro) n += blk(3 * plane) + b3;      // the micro-structure field and its blur
     }
     if (p.glare_active) n += 2 * blk(plane) + 2 * blk(plane);
     n += blk(3 * plane);  // the scanned RGB

@@ -80,13 +80,15 @@ struct DevelopStage {
     k::DevelopJob job;
     float* G = nullptr;  // the grain draw's output (grain on), before the final blur
     float* D = nullptr;  // the pre-grain density (grain_amount != 1)
+    float* planes = nullptr;  // the PLANES draw's 9 planes (dye clouds)
     bool blend = false;
 };
 void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
                  std::vector<gpu::DevBuf>& keep, DevelopStage* st, Profiler* prof = nullptr,
                  const GrainRecord* rec = nullptr);
+// film_seed_dev (device, may be null) overrides the plan's film seed for the micro-structure field.
 void develop_post(const spk_plan& p, const DevelopStage& st, float* negative, gpu::Pool& pool, hipStream_t s,
-                  std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr);
+                  std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr, const uint32_t* film_seed_dev = nullptr);
 struct PrintStage {
     k::PrintJob job;
     const k::SpecMfma* enl_mfma = nullptr;  // set when the plan asks for FP16 spectral integrals

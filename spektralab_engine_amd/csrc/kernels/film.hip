@@ -589,6 +589,19 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
             out[o] = D;
             continue;
         }
+        if (q.grain_mode == SPK_GRAIN_SIMPLE) {  // fx_grain.metal:55-64: one draw on D + D_min,c
+            const float dl = D + q.dmin_c[c];
+            const float Dl = q.sat_d[c], nl = q.count[c];
+            float qq = fdiv_rn(dl, Dl);
+            qq = qq < 1e-6f ? 1e-6f : (qq > 0.999999f ? 0.999999f : qq);
+            const float sat = 1.0f - ((qq * q.uniformity[c]) * 0.999999f);
+            gs.v[3 * c][lane] = fdiv_rn(nl, sat) * qq;
+            gs.v[3 * c + 1][lane] = 0.0f;
+            gs.v[3 * c + 2][lane] = 0.0f;
+            mult[3 * c] = q.d_per_n[c] * sat;
+            mult[3 * c + 1] = mult[3 * c + 2] = 0.0f;
+            continue;
+        }
         const float vv = q.positive ? -D : D;
         const uint32_t GK = q.gk;
         float d[3];
@@ -643,11 +656,13 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
     const float hl2p = spkm::half_log_2pi();
     const uint32_t base = i - lane;
     uint32_t n_retry = 0, n_small = 0;
+    const bool simple = q.grain_mode == SPK_GRAIN_SIMPLE;  // draws 3 c only
 #pragma unroll 1
     for (uint32_t jd = 0; jd < 9; ++jd) {
-        const float lam = valid ? gs.v[jd][lane] : 0.0f;
-        const bool large = valid && lam >= 10.0f;
-        const bool small = valid && !(lam >= 10.0f);
+        const bool on = valid && (!simple || jd % 3u == 0u);
+        const float lam = on ? gs.v[jd][lane] : 0.0f;
+        const bool large = on && lam >= 10.0f;
+        const bool small = on && !(lam >= 10.0f);
         bool pending = large;
         if (large) {
             Ptrs P;
@@ -677,11 +692,51 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
     }
 #pragma unroll
     for (uint32_t c = 0; c < 3; ++c) {
+        if (simple) {  // G := (0 + draw) (1 / N_sub) - D_min,c
+            const float G = (0.0f + gs.v[3 * c][lane] * mult[3 * c]) * q.inv_n_sub + (-q.dmin_c[c]);
+            out[size_t(c) * n + i] = G;
+            continue;
+        }
         float val[3];
         for (uint32_t l = 0; l < 3; ++l) val[l] = gs.v[3 * c + l][lane] * mult[3 * c + l];
+        if (q.grain_mode == SPK_GRAIN_PLANES) {  // fx_grain.metal:86-89: one plane per sub-layer
+            for (uint32_t l = 0; l < 3; ++l) q.planes[size_t(3 * l + c) * n + i] = val[l];
+            continue;
+        }
         float G = ((0.0f + val[0]) + val[1]) + val[2];
-        G = G * 1.0f + (-q.dmin_c[c]);
+        if (q.grain_shift) G = G * 1.0f + (-q.dmin_c[c]);
         out[size_t(c) * n + i] = G;
+    }
+}
+
+// The grain micro-structure field (grain.cpp:228-240, fx_grain.metal:102-116 per channel):
+// phi_c of pixel i := lognormal(1, sigma) from words x, y of Philox block 0 of (i, 100 + c, seed).
+// seed_ptr (device, may be null) overrides seed.
+__global__ __launch_bounds__(256) void k_micro_field(float* __restrict__ phi, uint32_t n, float sigma, uint32_t seed,
+                                                     const uint32_t* __restrict__ seed_ptr) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (seed_ptr) seed = *seed_ptr;
+#pragma unroll
+    for (uint32_t c = 0; c < 3; ++c) {
+        const uint4 b = philox_block(i, 100u + c, 0u, seed);
+        phi[size_t(c) * n + i] = spkm::lognormal_f(1.0f, sigma, u01(b.x), u01(b.y));
+    }
+}
+
+// After dye clouds or micro-structure: G := G phi + shift_c (phi null: G + shift_c), the product
+// and the sum each rounded (grain.cpp:240-244 encode_mul, encode_affine with scale 1).
+__global__ __launch_bounds__(256) void k_grain_finish(float* __restrict__ G, const float* __restrict__ phi, uint32_t n,
+                                                      float s0, float s1, float s2) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float sh[3] = {s0, s1, s2};
+#pragma unroll
+    for (uint32_t c = 0; c < 3; ++c) {
+        const size_t o = size_t(c) * n + i;
+        float g = G[o];
+        if (phi) g = g * phi[o];
+        G[o] = g + sh[c];
     }
 }
 
@@ -809,6 +864,14 @@ void develop(const float* H, const float* acc, float* out, float* density, const
 void develop_batch(const DevelopJob* jobs, const uint32_t* tile_prefix, uint32_t nframes, const uint32_t* seeds,
                    uint32_t grid, uint32_t block, hipStream_t s) {
     hipLaunchKernelGGL(k_develop_batch, dim3(grid), dim3(block), 0, s, jobs, tile_prefix, nframes, seeds);
+}
+
+void micro_field(float* phi, uint32_t n, float sigma, uint32_t seed, const uint32_t* seed_ptr, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_micro_field, dim3((n + 255) / 256), dim3(256), 0, s, phi, n, sigma, seed, seed_ptr);
+}
+
+void grain_finish(float* G, const float* phi, uint32_t n, const float shift[3], hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_grain_finish, dim3((n + 255) / 256), dim3(256), 0, s, G, phi, n, shift[0], shift[1], shift[2]);
 }
 
 }  // namespace spk::k

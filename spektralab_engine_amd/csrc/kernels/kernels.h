@@ -73,7 +73,11 @@ struct DevelopArgs {
     uint32_t w, h, K;
     uint32_t mix_active;      // raw := mix_a raw + mix_b acc
     float mix_a[3], mix_b[3];
-    uint32_t grain_active;    // SUM draw into the output, minus D_min,c
+    uint32_t grain_active;    // grain draws (grain_mode: SPK_GRAIN_SUM, _PLANES or _SIMPLE)
+    uint32_t grain_mode;
+    uint32_t grain_shift;     // SUM: subtract D_min,c from the sum
+    float inv_n_sub;          // SIMPLE: G := draw (1 / N_sub) - D_min,c
+    float* planes;            // PLANES: draw value of sub-layer l, channel c, pixel i at [(3 l + c) n + i]
     uint32_t write_density;   // also write the pre-grain density (grain_amount != 1)
     uint32_t seed, positive, gk;
     float dmin_l[9], sat_d[9], count[9], uniformity[3], dmin_c[3];
@@ -96,7 +100,7 @@ struct DevelopArgs {
 // The per-draw density of one grain of sub-layer l, channel c: D_lc / n_lc rounded once in
 // binary32, as the draw's scale (D / n) sat needs it.
 inline void set_grain_scale(DevelopArgs& a) {
-    for (int i = 0; i < 9; ++i) a.d_per_n[i] = a.sat_d[i] / a.count[i];
+    for (int i = 0; i < 9; ++i) a.d_per_n[i] = a.count[i] > 0.0f ? a.sat_d[i] / a.count[i] : 0.0f;
 }
 
 // Per-run constants of the enlarger (they follow the live print fields), passed by value.
@@ -212,6 +216,10 @@ void lincomb(const float* x, const float* y, float* out, uint32_t n, float a, fl
 void copy(const float* x, float* out, size_t n, hipStream_t s);
 void develop(const float* H, const float* acc, float* out, float* density, const float* curves, const float* gaxis,
              const float* ginv, const float* glayers, const float* lgam, const DevelopArgs& a, hipStream_t s);
+// Grain micro-structure field: 3 planes of lognormal(1, sigma) on streams 100 + c (seed_ptr, device,
+// may be null, overrides seed); then G := G phi + shift_c over 3 planes of n (phi null: G + shift_c).
+void micro_field(float* phi, uint32_t n, float sigma, uint32_t seed, const uint32_t* seed_ptr, hipStream_t s);
+void grain_finish(float* G, const float* phi, uint32_t n, const float shift[3], hipStream_t s);
 // glare: the blurred glare field (one plane) or nullptr.
 void print_pointwise(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea, const float* paper,
                      const float* cmax, const float* glare, uint32_t w, uint32_t h, hipStream_t s);

@@ -217,6 +217,62 @@ SPKM_FN float lgamma_k(float x, float hl2p) {
     return gl;
 }
 SPKM_FN float half_log_2pi() { return 0.5f * logf_(6.283185307179586f); }
+
+// cos(x) in binary64 for |x| < 2^20: x = k pi/2 + r with pi/2 split into three parts (the first
+// of 32 bits, so k P1 is exact; the split is of pi's decimal expansion), |r| <= pi/4 + 2^-40;
+// cos r and sin r by their Taylor series through r^20 and r^21 (truncation < 1e-23), chosen by
+// k mod 4.
+SPKM_FN double cos_d(double x) {
+    SPKM_NO_CONTRACT
+    const double t = x * 0x1.45f306dc9c883p-1;  // 2 / pi
+    const double kd = t >= 0.0 ? double(int64_t(t + 0.5)) : -double(int64_t(-t + 0.5));
+    const double r = ((x - kd * 0x1.921fb54400000p+0) - kd * 0x1.0b4611a626331p-34) - kd * 0x1.1701b839a2520p-88;
+    const double r2 = r * r;
+    double c = 1.0 / 2432902008176640000.0;  // 1 / 20!
+    c = c * r2 - 1.0 / 6402373705728000.0;
+    c = c * r2 + 1.0 / 20922789888000.0;
+    c = c * r2 - 1.0 / 87178291200.0;
+    c = c * r2 + 1.0 / 479001600.0;
+    c = c * r2 - 1.0 / 3628800.0;
+    c = c * r2 + 1.0 / 40320.0;
+    c = c * r2 - 1.0 / 720.0;
+    c = c * r2 + 1.0 / 24.0;
+    c = c * r2 - 0.5;
+    c = c * r2 + 1.0;
+    double sn = 1.0 / 51090942171709440000.0;  // 1 / 21!
+    sn = sn * r2 - 1.0 / 121645100408832000.0;
+    sn = sn * r2 + 1.0 / 355687428096000.0;
+    sn = sn * r2 - 1.0 / 1307674368000.0;
+    sn = sn * r2 + 1.0 / 6227020800.0;
+    sn = sn * r2 - 1.0 / 39916800.0;
+    sn = sn * r2 + 1.0 / 362880.0;
+    sn = sn * r2 - 1.0 / 5040.0;
+    sn = sn * r2 + 1.0 / 120.0;
+    sn = sn * r2 - 1.0 / 6.0;
+    sn = (sn * r2) * r + r;
+    switch (int64_t(kd) & 3) {
+        case 0: return c;
+        case 1: return -sn;
+        case 2: return -c;
+        default: return sn;
+    }
+}
+
+// Natural log and cosine of a float, rounded once from binary64.
+SPKM_FN float lnf_(float x) { return float(log_d(double(x))); }
+SPKM_FN float cosf_(float x) { return float(cos_d(double(x))); }
+
+// The lognormal draw of the grain micro-structure (reference kernels/fx_random.h:100-115) from the
+// uniforms u1, u2 (words x, y of Philox block 0): z := sqrt(-2 ln u1) cos(2 pi u2),
+// sig := sqrt(ln(1 + s^2 / m^2)), mu := ln m - sig^2 / 2, exp(mu + sig z); binary32 in the written
+// order with the logarithm, cosine and exponential above, so the host and the GPU agree bit for bit.
+SPKM_FN float lognormal_f(float m, float s, float u1, float u2) {
+    SPKM_NO_CONTRACT
+    const float z = __builtin_sqrtf(-2.0f * lnf_(u1)) * cosf_(6.283185307179586f * u2);
+    const float sig = __builtin_sqrtf(lnf_(1.0f + (s * s) / (m * m)));
+    const float mu = lnf_(m) - (0.5f * sig) * sig;
+    return expf_(mu + sig * z);
+}
 inline constexpr int kLgammaTable = 1024;
 // log10 of the film log: for finite x > 0 the table logarithm above times 1 / ln 10 in binary64
 // (relative error < 2^-49 before the one rounding to binary32, no division); other x through

@@ -143,8 +143,6 @@ def test_halation_mixture_override(resources, oracle):
 def test_refuses_stages_it_does_not_build(resources):
     with pytest.raises(spk.SpkError, match="scan_film"):
         spk.host_plan(resources, delta(scan_film=True), 64, 64)
-    with pytest.raises(spk.SpkError, match="grain_sublayers_active"):
-        spk.host_plan(resources, delta(grain_active=True, grain_sublayers_active=False), 64, 64)
     # a paper without an EDR tone map
     with pytest.raises(spk.SpkError, match="extended_dynamic_range"):
         spk.host_plan(resources, delta(extended_dynamic_range=True), 64, 64)
