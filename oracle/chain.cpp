@@ -1433,7 +1433,7 @@ void print(const spk_plan& p, const T* neg, uint16_t* out, T* rgb_out, const The
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < int64_t(np); ++i) {
         T d0 = neg[i], d1 = neg[np + i], d2 = neg[2 * np + i];
-        {
+        if (p.printed) {  // the scanned-film path scans the negative itself (src/print/run.cpp:126-146)
             T r0 = 0, r1 = 0, r2 = 0;
             for (int l = 0; l < SPK_NL; ++l) {
                 const T D = ((d0 * eps[3 * l] + d1 * eps[3 * l + 1]) + d2 * eps[3 * l + 2]) + beta[l];

@@ -451,6 +451,7 @@ Tables upload_tables(const spk_plan& p, gpu::Pool& pool) {
 
     k::PrintConsts q{};
     q.paper_k = p.paper_k;
+    q.printed = p.printed;
     axis_hints(p.paper, 1, p.paper_k, q.paper_hints, q.paper_hint_lo, q.paper_hint_scale);
     for (int i = 0; i < SPK_NL3; ++i) q.scan_eps[i] = num::f32(-p.scan_eps64[i] * kLog2Of10);
     for (int i = 0; i < SPK_NL; ++i) q.scan_beta[i] = num::f32(-p.scan_beta64[i] * kLog2Of10);
@@ -872,7 +873,6 @@ void print_pre(const spk_plan& p, const Tables& t, const float* negative, gpu::P
     mark("start");
     const uint32_t w = p.width, h = p.height;
     const size_t np = size_t(w) * h;
-    if (!p.printed) fail_internal("print: the scanned-film path is not built");
     k::EnlargerArgs ea = enlarger_args(p);
     const k::PrintConsts* dq = gpu::as<const k::PrintConsts>(t.print_consts);
 

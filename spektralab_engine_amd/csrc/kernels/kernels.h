@@ -116,7 +116,8 @@ struct EnlargerArgs {
 
 // Pipeline constants of the print side, device-resident (uploaded with the pipeline's tables).
 struct PrintConsts {
-    uint32_t paper_k, pad0;
+    uint32_t paper_k;
+    uint32_t printed;         // 0: scanned film, the scan integral reads the negative itself
     float scan_eps[SPK_NL3], scan_beta[SPK_NL], scan_wt[SPK_NL3];
     float to_rgb[9];
     float m_to_xyz[9], m_to_rgb[9], m16[9], m16_inv[9], d_rgb[3];

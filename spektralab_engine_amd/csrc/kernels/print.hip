@@ -99,8 +99,9 @@ __device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __
                                 const float* __restrict__ cmax, const float* __restrict__ glare) {
     float d0 = neg[i], d1 = neg[size_t(n) + i], d2 = neg[2 * size_t(n) + i];
 
-    // enlarger: log10 exposure on the paper, then the print gain
-    {
+    // enlarger: log10 exposure on the paper, then the print gain (a scanned film skips the
+    // enlarger and the paper: the scan reads the negative's densities, src/print/run.cpp:126-146)
+    if (q.printed) {
         float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
 #pragma unroll 9
         for (uint32_t l = 0; l < SPK_NL; ++l) {
@@ -214,10 +215,13 @@ __global__ __launch_bounds__(256) void k_print16(const float* __restrict__ neg, 
     const uint32_t lane = threadIdx.x & 63u;
     if (blockIdx.x * blockDim.x + (threadIdx.x & ~63u) >= n) return;  // whole wavefront past the end
     const uint32_t p = min(i, n - 1);
-    const float lo = enl->lo, hi = enl->hi;
-    const F3 r = spectral_mfma(enl, cref(e).eps, fmaxf(fminf(neg[p], hi), lo),
-                               fmaxf(fminf(neg[size_t(n) + p], hi), lo), fmaxf(fminf(neg[2 * size_t(n) + p], hi), lo), lane);
-    const F3 d = expose_paper(r.x, r.y, r.z, cref(e), paper, q);
+    F3 d{neg[p], neg[size_t(n) + p], neg[2 * size_t(n) + p]};
+    if (q.printed) {  // wave-uniform: the MFMA steps below run on whole wavefronts
+        const float lo = enl->lo, hi = enl->hi;
+        const F3 r = spectral_mfma(enl, cref(e).eps, fmaxf(fminf(d.x, hi), lo), fmaxf(fminf(d.y, hi), lo),
+                                   fmaxf(fminf(d.z, hi), lo), lane);
+        d = expose_paper(r.x, r.y, r.z, cref(e), paper, q);
+    }
     const float sl = scn->lo, sh = scn->hi;
     const F3 x = spectral_mfma(scn, q.scan_eps, fmaxf(fminf(d.x, sh), sl), fmaxf(fminf(d.y, sh), sl),
                                fmaxf(fminf(d.z, sh), sl), lane);

@@ -140,10 +140,21 @@ def test_halation_mixture_override(resources, oracle):
     assert abs(oracle.field(plan, "mix_a", 3)[0] - 1.0 / (1.0 + float(A))) < 1e-15
 
 
-def test_refuses_stages_it_does_not_build(resources):
-    with pytest.raises(spk.SpkError, match="scan_film"):
-        spk.host_plan(resources, delta(scan_film=True), 64, 64)
-    # a paper without an EDR tone map
+def test_scan_film_plan(oracle, resources):
+    """scan_film: no enlarger or paper in the plan; the scan tables are the film's own under its
+    viewing light (not the paper's); glare draws no field and the levels of a scanned negative are
+    a pass-through (src/print/setup.cpp:267-270, run.cpp:153-161)."""
+    d = dict(glare_active=True, scanner_black_correction=True, scanner_white_correction=True)
+    scanned = spk.host_plan(resources, delta(scan_film=True, **d), 64, 64)
+    printed = spk.host_plan(resources, delta(**d), 64, 64)
+    assert oracle.field(printed, "printed")[0] == 1 and oracle.field(scanned, "printed")[0] == 0
+    assert oracle.field(printed, "glare_active")[0] == 1 and oracle.field(scanned, "glare_active")[0] == 0
+    assert oracle.field(printed, "levels_active")[0] == 1 and oracle.field(scanned, "levels_active")[0] == 0
+    assert oracle.field(scanned, "edr_active")[0] == 0
+    assert not np.array_equal(oracle.field(scanned, "scan_eps64", 243), oracle.field(printed, "scan_eps64", 243))
+
+
+def test_refuses_edr_without_a_tone_map(resources):
     with pytest.raises(spk.SpkError, match="extended_dynamic_range"):
         spk.host_plan(resources, delta(extended_dynamic_range=True), 64, 64)
 
