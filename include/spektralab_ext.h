@@ -143,6 +143,21 @@ SPK_API spk_status spkx_diff_forward(spkx_diff* diff, int32_t slot, const double
 SPK_API spk_status spkx_diff_backward(spkx_diff* diff, int32_t slot, const float* input, const float* grad_output,
                                       float* grad_input, double* grad_theta, void* stream);
 SPK_API size_t spkx_diff_plan(spkx_diff* diff, void* out, size_t cap);
+/* The enlarger's CMY filtration as quantities of the differentiable render (no counterpart in the
+ * reference, whose print exposure follows c/m/y_filter_neutral and m/y_filter_shift through
+ * src/print/setup.cpp:339-361).  filters = (c, m, y neutral CC, m shift, y shift), NULL: the
+ * session's at spkx_diff_create.  wt (243) receives the enlarger weights theta 40..282 that
+ * filtration gives -- the filtered lamp times the paper sensitivity, scaled by the exposure gain of
+ * that light over the gain the render holds -- and jac (243 x 5 row-major, may be NULL) d wt / d
+ * filters, so that a gradient with respect to theta 40..282 becomes one with respect to the filters
+ * by jac^T.  filters_out (5, may be NULL) receives the session's filtration.  SPK_ERR_USER with a
+ * preflash exposure (its offset is not a quantity). */
+SPK_API spk_status spkx_diff_filter_weights(spkx_diff* diff, const double* filters, double* wt, double* jac,
+                                            double* filters_out);
+/* The same from a resource folder and a parameter delta, without an engine or device (the render
+ * gain is the one of the delta's own filtration). */
+SPK_API spk_status spkx_filter_weights(const char* resources_dir, const char* params_delta_json, const double* filters,
+                                       double* wt, double* jac, double* filters_out);
 SPK_API spk_status spkx_diff_info(spkx_diff* diff, char** out_json);
 SPK_API void spkx_diff_destroy(spkx_diff* diff);
 

@@ -718,6 +718,20 @@ SPK_API spk_status spkx_diff_defaults(spkx_diff* diff, double* theta) {
     return call.run([&] { diff_of(diff)->defaults(theta); });
 }
 
+SPK_API spk_status spkx_diff_filter_weights(spkx_diff* diff, const double* filters, double* wt, double* jac,
+                                            double* filters_out) {
+    Call call;
+    if (!diff || !wt) return Call::refuse("a required argument is NULL");
+    return call.run([&] { diff_of(diff)->filter_weights(filters, wt, jac, filters_out); });
+}
+
+SPK_API spk_status spkx_filter_weights(const char* resources_dir, const char* params_delta_json, const double* filters,
+                                       double* wt, double* jac, double* filters_out) {
+    Call call;
+    if (!resources_dir || !wt) return Call::refuse("a required argument is NULL");
+    return call.run([&] { spk::host_filter_weights(resources_dir, params_delta_json, filters, wt, jac, filters_out); });
+}
+
 SPK_API spk_status spkx_diff_forward(spkx_diff* diff, int32_t slot, const double* theta, const float* input,
                                      float* output, void* stream) {
     Call call;

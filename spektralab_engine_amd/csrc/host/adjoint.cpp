@@ -67,6 +67,8 @@ std::unique_ptr<Diff> Diff::create(Session& s, int frames, const char* options) 
         p.levels_active || p.edr_active)
         fail_user("differentiable render: input decode, geometry, highlight boost and headroom, camera lens blur, "
                   "scanner levels and EDR are not differentiated; set them off");
+    d->print_ = s.print_model();
+    d->live_ = s.print_live();
     d->n_bounce_ = p.bounce_active ? p.n_bounce : 0u;
     d->grain_ = p.grain_active != 0u;
     d->unsharp_ = p.unsharp_sigma > 0.0 && p.unsharp_amount > 0.0;
@@ -127,6 +129,15 @@ Diff::~Diff() {
         gpu::DeviceScope ds(e_->device());
         (void)hipStreamSynchronize(own_->get());
     }
+}
+
+void Diff::filter_weights(const double* f, double* wt, double* jac, double* f_out) const {
+    const spk_plan& p = *bf_.plan;
+    double f0[5];
+    filtration(*print_, live_, f0);
+    std::vector<double> j(jac ? 0 : SPK_NL3 * 5);
+    setup::filter_weights(*print_, live_, f ? f : f0, {p.enl_g64[0], p.enl_g64[1], p.enl_g64[2]}, wt, jac ? jac : j.data());
+    if (f_out) std::copy(f0, f0 + 5, f_out);
 }
 
 void Diff::defaults(double* th) const {

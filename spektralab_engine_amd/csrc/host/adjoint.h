@@ -51,6 +51,10 @@ public:
     void backward(int slot, const float* input, const float* g_enc, float* g_input, double* g_theta, hipStream_t s);
     void sync();
     std::string info() const;
+    // The enlarger weights (theta 40..282) under the filtration f (5 values, NULL: the session's at
+    // creation) and d wt / d f (243 x 5, may be NULL); f_out (may be NULL) receives the session's
+    // filtration.  See setup::filter_weights.
+    void filter_weights(const double* f, double* wt, double* jac, double* f_out) const;
 
 private:
     struct Slot {
@@ -68,6 +72,8 @@ private:
     Session::BatchFrame bf_;
     std::unique_ptr<gpu::Stream> own_;
     std::vector<Slot> slots_;
+    std::shared_ptr<const setup::PrintModel> print_;  // the session's print model at creation
+    setup::PrintLive live_;
     uint32_t n_bounce_ = 0;
     bool grain_ = false, unsharp_ = false, lens_ = false;
     // The reverse pass's partial sums: one region per producing kernel, and the table that reduces

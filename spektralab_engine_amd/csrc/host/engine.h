@@ -70,6 +70,13 @@ setup::Overrides parse_overrides(const char* json, setup::Overrides base, bool* 
 // spkx_plan: the plan of a full-tier render, built without any device.
 size_t host_plan(const char* dir, const char* delta, const char* overrides, uint32_t w, uint32_t h, uint32_t film_seed,
                  spk_plan* out);
+// The filtration of a print model and its live fields: (c, m, y neutral CC, m shift, y shift).
+void filtration(const setup::PrintModel& m, const setup::PrintLive& live, double f[5]);
+// spkx_filter_weights: setup::filter_weights for the parameters delta, with the gain the kernels
+// hold (the run's at the delta's own filtration); filters NULL: that filtration; f_out (may be
+// NULL) receives the filtration used.
+void host_filter_weights(const char* dir, const char* delta, const double* filters, double* wt, double* jac,
+                         double* f_out);
 
 // spk_output_transform's reply (reference docs/API_SPEC.md section 1, "Output transform"): the
 // src -> dst matrix with CAT02 between unequal whites, both transfer modes, and dst's gamut-map
@@ -198,6 +205,9 @@ public:
     void draw_seeds(uint32_t* film, uint32_t* print);
     // Fills the table section of a plan made by batch_frame (it is only filled for the first upload).
     void fill_tables(spk_plan* p);
+    // The print model and live print fields of the current pipeline.
+    std::shared_ptr<const setup::PrintModel> print_model() const { return pipeline_.print; }
+    const setup::PrintLive& print_live() const { return pipeline_.live; }
 
 private:
     Session() = default;

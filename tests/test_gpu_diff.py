@@ -144,6 +144,32 @@ def test_gradients_match_forward_mode(case):
     assert gt[39] == 0.0
 
 
+def test_filter_gradients_match_forward_mode(case):
+    """The CMY filtration as quantities: theta's enlarger weights from the filters
+    (spk.filter_theta); the gradient with respect to each filter against the float64 forward-mode
+    derivative of the loss along that filter's column of d wt / d filters."""
+    import torch
+    name, s, d, plan, img, x, o = case
+    counts = o.grain_counts(plan, img)
+    wt0, jac, f0 = d.filter_weights()
+    assert np.array_equal(wt0, o.field(plan, "enl_wt64", 243))
+    rng = np.random.default_rng(9)
+    th_t = d.defaults()
+    th_t[spk.Diff.WT:] *= 1.0 + 0.05 * rng.normal(size=243)
+    target = spk.diff_render(d, x, torch.from_numpy(th_t)).detach().cpu().numpy()
+    f = torch.from_numpy(f0.copy()).requires_grad_(True)
+    theta = spk.filter_theta(d, torch.from_numpy(d.defaults()), f)
+    enc = spk.diff_render(d, x, theta, slot=1)
+    loss = ((enc - torch.from_numpy(target).cuda()) ** 2).mean()
+    loss.backward()
+    gf = f.grad.numpy()
+    none = np.zeros((1,) + img.shape)
+    ref = np.array([o.frozen_grad(plan, img, counts, target, jac[:, k], none)[1][39] for k in range(5)])
+    print(f"[{name}] filter gradients gpu {gf} f64 {ref}", flush=True)
+    scale = np.abs(ref).max()
+    assert np.all(np.abs(gf - ref) <= 1e-3 * np.abs(ref) + 1e-6 * scale), (gf, ref)
+
+
 def test_refusals(engine):
     img = synth.make_frame(32, 24, 1)
     with engine.open(img, delta(dir_couplers_active=True, grain_active=False, halation_active=False)) as s:

@@ -280,3 +280,40 @@ def test_plan_tables_filled_and_tails_zero(resources, oracle):
         v = oracle.field(plan, name, max(n, cap)).view(np.float32)
         assert np.count_nonzero(v[:n]) > n // 2, name
         assert not np.any(v[n:]), name
+
+
+FILTERS = ["c_filter_neutral", "m_filter_neutral", "y_filter_neutral", "m_filter_shift", "y_filter_shift"]
+
+
+def test_filter_weights_and_their_jacobian(oracle, resources):
+    """The enlarger weights of a CMY filtration (the differentiable render's filter quantities): at
+    the parameters' own filtration they are the plan's weights bit for bit; their Jacobian equals
+    central differences of plans built at moved filter parameters (the set-up path itself), the
+    weights scaled by the moved exposure gain over the held one."""
+    d = delta(grain_active=False, halation_active=False, c_filter_neutral=5.0, m_filter_shift=0.2,
+              y_filter_shift=-0.3, filter_shift_scale=1.5)
+    wt, jac, f0 = spk.filter_weights_host(resources, d)
+    plan = spk.host_plan(resources, d, 64, 64)
+    assert f0[0] == 5.0 and list(f0[3:]) == [0.2, -0.3]
+    assert np.array_equal(wt, oracle.field(plan, "enl_wt64", 243))
+    g0 = oracle.field(plan, "enl_g64", 3)
+    h = 0.01
+    for j, name in enumerate(FILTERS):
+        def moved(v):
+            dd = dict(d)
+            dd[name] = float(f0[j] + v)
+            p = spk.host_plan(resources, dd, 64, 64)
+            return (oracle.field(p, "enl_wt64", 243).reshape(81, 3) * oracle.field(p, "enl_g64", 3) / g0).ravel()
+        fd = (moved(h) - moved(-h)) / (2 * h)
+        assert np.abs(jac[:, j] - fd).max() <= 1e-6 * np.abs(fd).max(), name
+    # a shift moves the filter it shifts by filter_shift_scale times a neutral step
+    assert np.allclose(jac[:, 3], 1.5 * jac[:, 1], rtol=1e-12, atol=0)
+    assert np.allclose(jac[:, 4], 1.5 * jac[:, 2], rtol=1e-12, atol=0)
+    # the weights at another filtration are the plan's at it
+    f1 = f0 + np.array([3.0, -2.0, 4.0, 0.1, -0.1])
+    wt1, _, _ = spk.filter_weights_host(resources, d, f1)
+    p1 = spk.host_plan(resources, dict(d, **{n: float(v) for n, v in zip(FILTERS, f1)}), 64, 64)
+    want = (oracle.field(p1, "enl_wt64", 243).reshape(81, 3) * oracle.field(p1, "enl_g64", 3) / g0).ravel()
+    assert np.allclose(wt1, want, rtol=1e-14, atol=0)
+    with pytest.raises(spk.SpkError, match="preflash"):
+        spk.filter_weights_host(resources, dict(d, preflash_exposure=0.1))
