@@ -142,17 +142,34 @@ __device__ inline float fdiv_any(float a, float b) {
     return a / b;
 }
 
-// Hormann's PTRS constants for one lam >= 10 (reference kernels/fx_random.h:52-58), with
-// log2(1/alpha) estimated for the screened acceptance test below.
+// Hormann's PTRS constants for one lam >= 10 (reference kernels/fx_random.h:52-58), with ln(lam)
+// and log2(1/alpha) estimated by the hardware log2 for the screened acceptance test below (the
+// exact ln(lam) is only needed where the screen cannot decide).
 struct Ptrs {
-    float lam, loglam, b, a, vr, log2inv;
+    float lam, loglam_e, b, a, vr, log2inv;
 };
+
+// sqrt(x) correctly rounded for a positive normal x: the hardware square root (faithful) corrected
+// by the signs of x - s' s for the neighbours s' of its result s, which place sqrt(x) against the
+// midpoints (the products of neighbouring floats fall on the same side of x as the squared
+// midpoints, x being a multiple of ulp(s)^2).  No scaling for subnormal or special inputs.
+__device__ inline float sqrt_rn_pos(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float down = __int_as_float(__float_as_int(s) - 1), up = __int_as_float(__float_as_int(s) + 1);
+    float r = s;
+    if (fmaf(-down, s, x) <= 0.0f) r = down;
+    if (fmaf(-up, s, x) > 0.0f) r = up;
+    return r;
+}
 
 // Margin of the screened acceptance test: the estimate of L differs from the binary32 L of the
 // exact test by less than 3e-5 (hardware log2, about 1 ulp, of arguments whose logarithms are at
-// most ~51 in magnitude, plus binary32 roundings of L itself); any estimate farther than this
-// from Rt decides as the exact test does.
+// most ~51 in magnitude, plus binary32 roundings of L itself); the estimate of Rt, made with the
+// estimated ln(lam) (within 2.7e-7 relative of the exact one), differs from the exact Rt by less
+// than 5.1e-7 (k ln(lam) + lam + |Rt|), and kRtMargin is 2^-20 of that sum.  An estimate of L
+// farther than both margins from the estimate of Rt decides as the exact test does.
 constexpr float kPtrsMargin = 1e-3f;
+constexpr float kRtMargin = 0x1p-20f;
 
 // One PTRS try from the uniform pair (u, v): 1 = accept *k, 0 = next try.  lgamma(k + 1) comes
 // from the LDS table for k < 1024 (the table holds spkm::lgamma_k exactly).  The rejection test
@@ -168,13 +185,17 @@ __device__ inline bool ptrs_try(const Ptrs& P, float u, float v, const float* __
     if (us >= 0.07f && V <= P.vr) return true;
     if (k < 0.0f || (us < 0.013f && V > us)) return false;
     const float lg = k < float(spkm::kLgammaTable) ? lgam[int(k)] : spkm::lgamma_k(k + 1.0f, hl2p);
-    const float Rt = ((-P.lam + k * P.loglam) - lg);
+    const float kl = k * P.loglam_e;
+    const float Rte = ((-P.lam + kl) - lg);
+    const float m = kPtrsMargin + ((kl + P.lam) + fabsf(Rte)) * kRtMargin;
     const float uu = us * us;
     const float Qe = fmaf(P.a, __builtin_amdgcn_rcpf(uu), P.b);
     const float Le = ((__builtin_amdgcn_logf(V) + P.log2inv) - __builtin_amdgcn_logf(Qe)) * 0.69314718f;
-    if (Le <= Rt - kPtrsMargin) return true;
-    if (Le > Rt + kPtrsMargin) return false;
-    // V in (0, 1] and a / us^2 + b >= b > 0 (us >= 2^-25): logf_pos is logf_ there
+    if (Le <= Rte - m) return true;
+    if (Le > Rte + m) return false;
+    // the exact test: lam >= 10 is finite and positive, V in (0, 1] and a / us^2 + b >= b > 0
+    // (us >= 2^-25), so logf_pos is logf_ at every argument
+    const float Rt = ((-P.lam + k * spkm::logf_pos(P.lam)) - lg);
     const float invalpha = 1.1239f + fdiv_rn(1.1328f, P.b - 3.4f);
     const float Lh = ((spkm::logf_pos(V) + spkm::logf_pos(invalpha)) - spkm::logf_pos(fdiv_rn(P.a, uu) + P.b));
     return Lh <= Rt;
@@ -186,12 +207,12 @@ __device__ inline bool ptrs_try(const Ptrs& P, float u, float v, const float* __
 // most 512 steps, four per block.  NaN lam takes the second branch and returns 0.
 __device__ inline void ptrs_setup(Ptrs& P, float lam) {
     P.lam = lam;
-    const float slam = sqrtf(lam);
+    // lam >= 10 is finite and positive (lam is at most n / 2e-6 here)
+    const float slam = sqrt_rn_pos(lam);
     P.b = 0.931f + (2.53f * slam);
     P.a = -0.059f + (0.02483f * P.b);
     P.vr = 0.9277f - fdiv_rn(3.6224f, P.b - 2.0f);
-    // lam >= 10 is finite and positive (lam is at most n / 2e-6 here)
-    P.loglam = spkm::logf_pos(lam);
+    P.loglam_e = __builtin_amdgcn_logf(lam) * 0.69314718f;
     P.log2inv = __builtin_amdgcn_logf(fmaf(1.1328f, __builtin_amdgcn_rcpf(P.b - 3.4f), 1.1239f));
 }
 
