@@ -253,10 +253,12 @@ __device__ float poisson(float lam, uint32_t pixel, uint32_t stream, uint32_t se
 // ---- the grain draws of one wavefront, compacted ---------------------------------------------
 // A wavefront's 64 pixels make 576 draws (3 channels x 3 sub-layers).  Drawn where they arise,
 // lanes whose lam lies on the other side of 10 (the two methods) or that need more uniform blocks
-// than their neighbours sit idle.  So a draw with lam >= 10 makes only the two tries of its first
-// Philox block where it arises (they accept about 96 % of such draws, and nearly every lane has
-// one in each slot); what is left is listed by method in the wavefront's LDS slice, and each list
-// is drained by all 64 lanes, a lane taking the next draw as soon as its current one is done.
+// than their neighbours sit idle.  So a draw with lam >= 10 makes only the first try of its first
+// Philox block where it arises (the (x, y) pair: it accepts 55-85 % of such draws, and nearly every
+// lane has one in each slot); what is left is listed by method in the wavefront's LDS slice, and
+// each list is drained by all 64 lanes, a lane taking the next draw as soon as its current one is
+// done.  (Making the block's second try where the draws arise too measured 4 % slower: the drain
+// runs it with far fewer idle lanes.)
 // Every draw consumes the same Philox blocks in the same order as poisson() above, so the counts
 // are identical; only the lane doing the work changes.
 struct GrainScratch {
@@ -286,6 +288,7 @@ __device__ __forceinline__ void drain(GrainScratch& gs, uint32_t count, uint32_t
     uint32_t head = 0;
     bool busy = false;
     bool second = false;  // PTRS: the current block's (z, w) pair is next
+    bool resume_zw = false;  // PTRS: the first pass takes (z, w) of block 0
     uint32_t j = 0, src = 0, blk = 0, stream = 0, kz = 0, kw = 0;
     float lam = 0.0f, E = 0.0f, X = 0.0f, prod = 1.0f;
     Ptrs P{};
@@ -300,10 +303,13 @@ __device__ __forceinline__ void drain(GrainScratch& gs, uint32_t count, uint32_t
                 src = id & 63u;
                 lam = gs.v[j][src];
                 stream = j / 3u + 10u * (j % 3u);
-                blk = kLarge ? 1u : 0u;  // a listed lam >= 10 draw has rejected both tries of block 0
+                blk = 0u;
                 busy = true;
                 if (kLarge) {
+                    // a listed lam >= 10 draw has rejected the try of block 0's (x, y): it resumes
+                    // at that block's (z, w), recomputed with the block in the first pass
                     second = false;
+                    resume_zw = true;
                     ptrs_setup(P, lam);
                 } else {
                     E = spkm::expf_(-lam);
@@ -319,19 +325,27 @@ __device__ __forceinline__ void drain(GrainScratch& gs, uint32_t count, uint32_t
             float r = 0.0f;
             if (kLarge) {
                 uint32_t wu, wv;
+                bool zw = second;  // this pass's pair is its block's second
                 if (!second) {
                     const uint4 w = philox_block(base + src, stream, blk, seed);
-                    wu = w.x;
-                    wv = w.y;
-                    kz = w.z;
-                    kw = w.w;
+                    if (resume_zw) {
+                        wu = w.z;
+                        wv = w.w;
+                        zw = true;
+                        resume_zw = false;
+                    } else {
+                        wu = w.x;
+                        wv = w.y;
+                        kz = w.z;
+                        kw = w.w;
+                    }
                 } else {
                     wu = kz;
                     wv = kw;
                 }
                 if (ptrs_try(P, u01(wu), u01(wv), lgam, hl2p, &r)) {
                     done = true;
-                } else if (!second) {
+                } else if (!zw) {
                     second = true;
                 } else {
                     second = false;
@@ -669,10 +683,10 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
 #pragma unroll
         for (uint32_t jd = 0; jd < 9; ++jd) q.lambdas[size_t(jd) * n + i] = gs.v[jd][lane];
     }
-    // The draws with lam >= 10 first make the two tries of their Philox block 0 where they arise:
-    // every lane has one such draw in almost every slot jd, so this runs at full width.  A draw
-    // rejected twice (a few percent) is listed for the compacted drain from block 1 on, as are
-    // the lam < 10 draws; each list is then drained by the whole wavefront.
+    // The draws with lam >= 10 first make the first try of their Philox block 0 where they arise:
+    // every lane has one such draw in almost every slot jd, so this runs at full width.  A draw it
+    // rejects is listed for the compacted drain from that block's (z, w) on, as are the lam < 10
+    // draws; each list is then drained by the whole wavefront.
     wave_sync();
     const float hl2p = spkm::half_log_2pi();
     const uint32_t base = i - lane;
@@ -690,7 +704,7 @@ __device__ __forceinline__ void develop_wave(uint32_t i, bool valid, uint32_t n,
             ptrs_setup(P, lam);
             const uint4 w = philox_block(i, jd / 3u + 10u * (jd % 3u), 0u, seed);
             float r;
-            if (ptrs_try(P, u01(w.x), u01(w.y), s_lgam, hl2p, &r) || ptrs_try(P, u01(w.z), u01(w.w), s_lgam, hl2p, &r)) {
+            if (ptrs_try(P, u01(w.x), u01(w.y), s_lgam, hl2p, &r)) {
                 gs.v[jd][lane] = r;
                 pending = false;
             }
