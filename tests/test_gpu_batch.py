@@ -131,3 +131,36 @@ def test_batch_rejects_bad_arguments(engine):
             spk.Batch(engine, [s], [out.data_ptr()], [16])  # stride below the width
         with pytest.raises(spk.SpkError):
             spk.Batch(engine, [s], [out.data_ptr()], [32], options={"streams": 9})
+
+
+# Frames at pitches that switch the grain models on (film_format_mm 4): the single-layer model, dye
+# clouds on one plane and on all nine, and dye clouds with the micro-structure field.
+GRAIN_MODELS = [((512, 384), dict(grain_sublayers_active=False)), ((1024, 64), {}), ((4096, 64), {}),
+                ((9000, 24), {})]
+
+
+@pytest.mark.parametrize("schedule", ["phased", "streams"])
+def test_batch_grain_models_match_per_session_renders(engine, schedule):
+    """The batch's dye-cloud planes, micro-structure field (its seed read from the batch's device
+    seed table) and single-layer draws equal each frame's own session render, two runs in a row."""
+    base = dict(CONFIG1, grain_active=True, film_format_mm=4.0)
+    fr, deltas = [], []
+    for i, ((w, h), extra) in enumerate(GRAIN_MODELS):
+        fr.append((synth.make_frame(w, h, 90 + i), {}, None))
+        deltas.append(dict(base, **extra))
+    ss = [engine.open(img, d) for (img, _, _), d in zip(fr, deltas)]
+    refs = [engine.open(img, d) for (img, _, _), d in zip(fr, deltas)]
+    outs = device_outputs(fr)
+    try:
+        with spk.Batch(engine, ss, [o.data_ptr() for o in outs], [o.shape[1] for o in outs],
+                       options={"autotune": False, "schedule": schedule, "streams": 2, "grid_per_cu": 4,
+                                "block": 256}) as b:
+            for run in range(2):
+                b.run()
+                b.sync()
+                for i, (img, _, _) in enumerate(fr):
+                    want, _ = refs[i].render("full")
+                    assert_close(host(outs[i], img.shape[1]), want, f"run {run} frame {i}")
+    finally:
+        for s in ss + refs:
+            s.close()
