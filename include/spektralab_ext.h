@@ -170,6 +170,11 @@ SPK_API spk_status spkx_poisson(spk_engine* engine, const float* lam, uint32_t n
                                 float* out);
 SPK_API spk_status spkx_gaussian(spk_engine* engine, const float* in, float* out, uint32_t width, uint32_t height,
                                  const double* sigma3, double fir_crossover);
+/* A mixture of Gaussians as the render's halation runs it: out = sum_k weight_k G_{sigma_k}(in) per
+ * channel, accumulated in list order (sigma and weight: n x 3, channel fastest; 1 <= n <=
+ * SPK_PLAN_MAX_MIX; the branch rules of spkx_gaussian per component). */
+SPK_API spk_status spkx_mixture(spk_engine* engine, const float* in, float* out, uint32_t width, uint32_t height,
+                                const double* sigma, const double* weight, uint32_t n, double fir_crossover);
 /* The transpose of spkx_gaussian's operator, per channel (the reverse pass's blur of the
  * differentiable render: the FIR pass, which is symmetric, or the transposed recursion). */
 SPK_API spk_status spkx_gaussian_adjoint(spk_engine* engine, const float* in, float* out, uint32_t width,

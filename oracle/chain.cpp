@@ -1789,6 +1789,19 @@ void oracle_gaussian_f32(const float* in, float* out, int w, int h, const double
     for (size_t i = 0; i < np; ++i)
         for (int c = 0; c < 3; ++c) out[3 * i + c] = b[c * np + i];
 }
+// The halation mixture of an interleaved 3-channel image: out = sum_k w_k G_k(in) per channel in
+// list order (the develop stage's accumulation, gaussian.cpp:254-273); sigma, weight: n x 3.
+void oracle_mixture_f32(const float* in, float* out, int w, int h, const double* sigma, const double* weight, int n,
+                        double crossover) {
+    const size_t np = size_t(w) * h;
+    std::vector<float> a(3 * np), acc(3 * np, 0.0f);
+    for (size_t i = 0; i < np; ++i)
+        for (int c = 0; c < 3; ++c) a[c * np + i] = in[3 * i + c];
+    for (int k = 0; k < n; ++k)
+        blur3<float>(a.data(), acc.data(), sigma + 3 * k, k == 0 ? 1 : 2, acc.data(), weight + 3 * k, w, h, crossover);
+    for (size_t i = 0; i < np; ++i)
+        for (int c = 0; c < 3; ++c) out[3 * i + c] = acc[c * np + i];
+}
 // The transposed Gaussian of oracle_gaussian_f32: per channel, the FIR pass itself (with
 // half-sample reflection and symmetric taps it is a symmetric operator) or the transposed
 // recursion on rows, then columns.

@@ -36,6 +36,8 @@ def lib():
         l.oracle_print_f64.argtypes = [V, D, U16, D]
         l.oracle_gaussian_f32.argtypes = [F, F, C.c_int, C.c_int, D, C.c_double]
         l.oracle_gaussian_adj_f32.argtypes = [F, F, C.c_int, C.c_int, D, C.c_double]
+        l.oracle_mixture_f32.argtypes = [F, F, C.c_int, C.c_int, D, D, C.c_int, C.c_double]
+        l.oracle_mixture_f32.restype = None
         l.oracle_gaussian_adj_f32.restype = None
         l.oracle_philox.argtypes = [U, U, U, C.c_uint32]
         l.oracle_poisson.argtypes = [F, C.c_uint32, C.c_uint32, C.c_uint32, F]
@@ -188,6 +190,19 @@ def gaussian(image, sigma3, crossover=3.0):
     out = np.zeros_like(img)
     s = (C.c_double * 3)(*[float(v) for v in sigma3])
     lib().oracle_gaussian_f32(_p(img, C.c_float), _p(out, C.c_float), w, h, s, crossover)
+    return out
+
+
+def mixture(image, components, crossover=3.0):
+    """The halation mixture of an H x W x 3 image: components = [(sigma, weight), ...] (scalars or
+    3 per-channel values), accumulated in list order."""
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    h, w, _ = img.shape
+    out = np.zeros_like(img)
+    n = len(components)
+    sg = (C.c_double * (3 * n))(*[float(v) for s, _ in components for v in np.broadcast_to(s, 3)])
+    wt = (C.c_double * (3 * n))(*[float(v) for _, x in components for v in np.broadcast_to(x, 3)])
+    lib().oracle_mixture_f32(_p(img, C.c_float), _p(out, C.c_float), w, h, sg, wt, n, crossover)
     return out
 
 

@@ -608,6 +608,34 @@ SPK_API spk_status spkx_gaussian(spk_engine* engine, const float* in, float* out
     });
 }
 
+SPK_API spk_status spkx_mixture(spk_engine* engine, const float* in, float* out, uint32_t width, uint32_t height,
+                                const double* sigma, const double* weight, uint32_t n, double fir_crossover) {
+    Call call;
+    if (!engine || !in || !out || !sigma || !weight) return Call::refuse("a required argument is NULL");
+    static_assert(SPK_PLAN_MAX_MIX == 8, "the refusal below names the limit");
+    if (n < 1 || n > SPK_PLAN_MAX_MIX) return Call::refuse("a mixture has 1 to 8 components");
+    return call.run([&] {
+        Engine& e = *engine_of(engine);
+        spk::gpu::DeviceScope ds(e.device());
+        const size_t np = size_t(width) * height;
+        std::vector<float> p = planar(in, np);
+        spk::gpu::DevBuf a = e.pool().acquire(3 * np * 4), b = e.pool().acquire(3 * np * 4);
+        spk::gpu::check(hipMemcpy(a->ptr, p.data(), 3 * np * 4, hipMemcpyHostToDevice), "hipMemcpy");
+        std::vector<spk_mix_component> comps(n);
+        for (uint32_t k = 0; k < n; ++k)
+            for (int c = 0; c < 3; ++c) {
+                comps[k].sigma[c] = sigma[3 * k + c];
+                comps[k].weight[c] = weight[3 * k + c];
+            }
+        std::vector<spk::gpu::DevBuf> keep;
+        spk::chain::mixture(static_cast<float*>(a->ptr), static_cast<float*>(b->ptr), comps.data(), n, width, height,
+                            fir_crossover, e.pool(), nullptr, keep);
+        spk::gpu::check(hipDeviceSynchronize(), "mixture");
+        spk::gpu::check(hipMemcpy(p.data(), b->ptr, 3 * np * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        interleave(p, out, np);
+    });
+}
+
 SPK_API spk_status spkx_gaussian_adjoint(spk_engine* engine, const float* in, float* out, uint32_t width,
                                          uint32_t height, const double* sigma3, double fir_crossover) {
     Call call;

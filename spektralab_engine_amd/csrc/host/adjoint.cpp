@@ -382,6 +382,17 @@ void Diff::backward(int slot, const float* input, const float* g_enc, float* g_i
         float* tp = gA;  // g_G is no longer needed
         float* fp = buf(3 * np);
         float* fm = buf(3 * np);
+        // The FIR radius of component kk when its three channels share one sigma on the FIR
+        // branch (0 otherwise): two such components in a row take their transposed blurs, the
+        // g_H update and the weight gradients in one pass over g_acc (k::fir_pair_adjoint).
+        auto pair_radius = [&](uint32_t kk) {
+            const double s0 = th[k::kAdjBounceS + 3 * kk];
+            if (!(s0 == th[k::kAdjBounceS + 3 * kk + 1] && s0 == th[k::kAdjBounceS + 3 * kk + 2])) return 0;
+            if (!(s0 > 0.0) || !(s0 < p.fir_crossover)) return 0;
+            const int r = setup::fir_radius(s0);
+            return r <= k::kFirPairMaxR ? r : 0;
+        };
+        uint32_t paired_to = 0;  // components below this index already have their bounce terms
         for (uint32_t kk = 0; kk < n_bounce_; ++kk) {
             double sg[3];
             float wg[3], sc[3];
@@ -389,6 +400,24 @@ void Diff::backward(int slot, const float* input, const float* g_enc, float* g_i
                 sg[c] = th[k::kAdjBounceS + 3 * kk + c];
                 wg[c] = num::f32(th[k::kAdjBounceW + 3 * kk + c]);
                 sc[c] = num::f32(th[k::kAdjBounceW + 3 * kk + c] / (2.0 * kSigmaStep * sg[c]));
+            }
+            if (kk >= paired_to && kk + 1 < n_bounce_ && pair_radius(kk) > 0 && pair_radius(kk + 1) > 0 &&
+                k::fir_pair_tiles(w, h) <= k::adj_blocks(n)) {
+                k::FirPairArgs pa{};
+                pa.w = w;
+                pa.h = h;
+                k::FirPairAdj adj{};
+                adj.H = H;
+                adj.nrows = k::adj_blocks(n);
+                for (uint32_t j = 0; j < 2; ++j) {
+                    const std::vector<float> taps = setup::fir_taps(th[k::kAdjBounceS + 3 * (kk + j)]);
+                    pa.r[j] = uint32_t(taps.size() / 2);
+                    for (size_t i = 0; i < taps.size(); ++i) pa.taps[j][i] = taps[i];
+                    adj.part[j] = part + rg.bw[kk + j];
+                    for (int c = 0; c < 3; ++c) adj.w[j][c] = num::f32(th[k::kAdjBounceW + 3 * (kk + j) + c]);
+                }
+                k::fir_pair_adjoint(gAcc, gH, pa, adj, s);
+                paired_to = kk + 2;
             }
             auto blur = [&](const float* in, float* out, double step, bool adjoint) {
                 if (sg[0] == sg[1] && sg[1] == sg[2]) {
@@ -400,8 +429,10 @@ void Diff::backward(int slot, const float* input, const float* g_enc, float* g_i
                     chain::blur_planes(in + c * np, out + c * np, 1, sg[c] * (1.0 + step), sg[c], p.fir_crossover,
                                        adjoint, w, h, pool, s, keep);
             };
-            blur(gAcc, tp, 0.0, true);
-            k::adj_bounce(tp, H, gH, wg, n, part + rg.bw[kk], s);
+            if (kk >= paired_to) {
+                blur(gAcc, tp, 0.0, true);
+                k::adj_bounce(tp, H, gH, wg, n, part + rg.bw[kk], s);
+            }
             // a FIR-tile component (one sigma for the three channels): both blurs and the dot
             // product in one pass over tiles; otherwise the two blurs, then the dot product
             const int rr = setup::fir_radius(sg[0]);

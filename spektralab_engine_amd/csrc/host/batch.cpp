@@ -14,7 +14,7 @@ namespace spk {
 namespace {
 
 constexpr int kMaxStreams = 4;  // the device's hardware queues per process
-constexpr int kEvents = 32;  // 3 forks + 4 joins of up to 4 streams
+constexpr int kEvents = 32;  // 3 forks + 3 joins of up to 4 streams; the last 4 join a failed capture back
 
 BatchConfig parse_options(const char* text, bool* autotune, bool* record) {
     BatchConfig c;
@@ -187,8 +187,10 @@ std::unique_ptr<Batch::Graph> Batch::capture(const BatchConfig& c) {
     const int S = std::min(c.streams, kMaxStreams);
     hipStream_t o = origin_->get();
     int ev = 0;
+    bool forked = false;  // the S streams have joined the capture and not yet been joined back
     auto fork = [&]() {
         gpu::check(hipEventRecord(events_[size_t(ev)], o), "hipEventRecord(fork)");
+        forked = true;
         for (int k = 0; k < S; ++k) gpu::check(hipStreamWaitEvent(streams_[size_t(k)]->get(), events_[size_t(ev)], 0), "wait");
         ++ev;
     };
@@ -198,6 +200,7 @@ std::unique_ptr<Batch::Graph> Batch::capture(const BatchConfig& c) {
             gpu::check(hipStreamWaitEvent(o, events_[size_t(ev)], 0), "wait");
             ++ev;
         }
+        forked = false;
     };
     // Frames largest first onto the least-loaded stream; the same placement in every phase.
     std::vector<int> lane(n, 0);
@@ -293,6 +296,10 @@ std::unique_ptr<Batch::Graph> Batch::capture(const BatchConfig& c) {
         gpu::check(hipGetLastError(), "batch launches");
         gpu::check(hipStreamEndCapture(o, &g->graph), "hipStreamEndCapture");
     } catch (...) {
+        // A failure between a fork and its join (or inside the streams schedule) leaves the forked
+        // streams in the capture: join them back first, so that ending the capture on the origin
+        // stream takes every stream out of capture mode and they stay usable for the next capture.
+        if (forked || c.mode == 1) abandon_capture(S);
         hipGraph_t broken = nullptr;
         (void)hipStreamEndCapture(o, &broken);
         if (broken) (void)hipGraphDestroy(broken);
@@ -300,6 +307,15 @@ std::unique_ptr<Batch::Graph> Batch::capture(const BatchConfig& c) {
     }
     gpu::check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     return g;
+}
+
+void Batch::abandon_capture(int S) {
+    hipStream_t o = origin_->get();
+    for (int k = 0; k < S; ++k) {
+        hipEvent_t e = events_[size_t(kEvents - 1 - k)];
+        if (hipEventRecord(e, streams_[size_t(k)]->get()) == hipSuccess) (void)hipStreamWaitEvent(o, e, 0);
+    }
+    (void)hipGetLastError();
 }
 
 // Schedule 1 (inside the capture, after the seed copy): fork; per frame on its stream the film

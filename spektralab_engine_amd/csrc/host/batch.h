@@ -80,6 +80,7 @@ private:
     Batch() = default;
     std::unique_ptr<Graph> capture(const BatchConfig& c);
     void capture_streams(const BatchConfig& c, Graph& g, int S, const std::vector<int>& lane);
+    void abandon_capture(int S);  // joins the S forked streams back into a capture that failed
     double time_graph(Graph& g, int reps);
 
     Engine* e_ = nullptr;

@@ -347,15 +347,48 @@ void blur3(const float* in, float* out, const double sigma[3], uint32_t mode, co
 
 namespace {
 
-// M(in) accumulated in list order into out (src/fx/gaussian.cpp:254-273), out != in.
-void mixture(const float* in, float* out, const spk_mix_component* comps, uint32_t n, uint32_t w, uint32_t h,
-             double crossover, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep) {
-    for (uint32_t k = 0; k < n; ++k) {
-        blur3(in, out, comps[k].sigma, k == 0 ? 1u : 2u, out, comps[k].weight, w, h, crossover, pool, s, keep);
-    }
+// The FIR radius of a component that has one sigma and one weight for its three channels and
+// takes the FIR branch (0 otherwise).
+int uniform_fir_radius(const spk_mix_component& m, double crossover) {
+    const double sg = m.sigma[0];
+    if (!(sg == m.sigma[1] && sg == m.sigma[2] && m.weight[0] == m.weight[1] && m.weight[0] == m.weight[2])) return 0;
+    if (!(sg > 0.0) || !(sg < crossover)) return 0;
+    const int r = setup::fir_radius(sg);
+    return r <= k::kFirPairMaxR ? r : 0;
 }
 
 }  // namespace
+
+// M(in) accumulated in list order into out (src/fx/gaussian.cpp:254-273), out != in.  Two
+// consecutive components that both take the FIR branch with one sigma and weight for the three
+// channels run as one pass over `in` (k::fir_pair, the same float operations as the two
+// components' separate passes).
+void mixture(const float* in, float* out, const spk_mix_component* comps, uint32_t n, uint32_t w, uint32_t h,
+             double crossover, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep) {
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t mode = k == 0 ? 1u : 2u;
+        if (k + 1 < n && in != out) {
+            const int r0 = uniform_fir_radius(comps[k], crossover), r1 = uniform_fir_radius(comps[k + 1], crossover);
+            if (r0 > 0 && r1 > 0) {
+                k::FirPairArgs a{};
+                a.w = w;
+                a.h = h;
+                a.first_mode = mode;
+                for (int j = 0; j < 2; ++j) {
+                    const spk_mix_component& m = comps[k + uint32_t(j)];
+                    const std::vector<float> taps = setup::fir_taps(m.sigma[0]);
+                    a.r[j] = uint32_t(taps.size() / 2);
+                    for (size_t i = 0; i < taps.size(); ++i) a.taps[j][i] = taps[i];
+                    a.weight[j] = num::f32(m.weight[0]);
+                }
+                k::fir_pair(in, out, out, a, 3, s);
+                ++k;
+                continue;
+            }
+        }
+        blur3(in, out, comps[k].sigma, mode, out, comps[k].weight, w, h, crossover, pool, s, keep);
+    }
+}
 
 std::vector<float> lgamma_table() {
     std::vector<float> t(spkm::kLgammaTable);

@@ -119,6 +119,9 @@ void blur3(const float* in, float* out, const double sigma[3], uint32_t mode, co
 
 // FIR arguments of a w x h blur with the taps at sigma over the radius of branch_sigma (sigma
 // below the crossover; the radius at most k::kFirMaxR).
+// The mixture M(in) = sum_k w_k G_k(in), accumulated in list order into out (out != in).
+void mixture(const float* in, float* out, const spk_mix_component* comps, uint32_t n, uint32_t w, uint32_t h,
+             double crossover, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep);
 k::FirArgs fir_args_at(double sigma, double branch_sigma, uint32_t w, uint32_t h);
 
 // The Gaussian of nplanes contiguous w x h planes with the branch (identity, FIR of its radius, or

@@ -124,6 +124,43 @@ __device__ __forceinline__ void fir_window(int n, const float* __restrict__ taps
     }
 }
 
+// fir_window<true> for loads that are valid at every index up to n + 2 kT - 2 (the caller's
+// addresses stay inside their plane or LDS tile there): the ring is refilled without per-index
+// tests; values past the window, which no tap multiplies, are never used.  Same ring, same
+// ascending order, same sums.
+template <class Load>
+__device__ __forceinline__ void fir_window_ng(int n, const float* __restrict__ taps, Load load, float (&acc)[kT]) {
+    float ring[2 * kT];
+#pragma unroll
+    for (int j = 0; j < kT; ++j) {
+        acc[j] = 0.0f;
+        ring[j] = load(j);
+    }
+    for (int i0 = 0; i0 < n; i0 += 2 * kT) {
+#pragma unroll
+        for (int k = 0; k < kT; ++k) ring[kT + k] = load(i0 + kT + k);
+#pragma unroll
+        for (int ii = 0; ii < kT; ++ii) {
+            if (i0 + ii < n) {
+                const float tap = taps[i0 + ii];
+#pragma unroll
+                for (int j = 0; j < kT; ++j) acc[j] = fmaf(tap, ring[ii + j], acc[j]);
+            }
+        }
+        if (i0 + kT >= n) break;
+#pragma unroll
+        for (int k = 0; k < kT; ++k) ring[k] = load(i0 + 2 * kT + k);
+#pragma unroll
+        for (int ii = 0; ii < kT; ++ii) {
+            if (i0 + kT + ii < n) {
+                const float tap = taps[i0 + kT + ii];
+#pragma unroll
+                for (int j = 0; j < kT; ++j) acc[j] = fmaf(tap, ring[(kT + ii + j) % (2 * kT)], acc[j]);
+            }
+        }
+    }
+}
+
 // Vertical: a wave is 64 adjacent columns of one run of kT rows; loads are coalesced rows, and
 // rows outside the frame (wave-uniform) are folded by half-sample reflection.  nplanes
 // contiguous planes of w x h run in one launch.
@@ -412,6 +449,158 @@ __global__ __launch_bounds__(kF2NT) void k_fir2d(const float* __restrict__ in, f
         const int row = e / kF2TX, col = e - row * kF2TX;
         const int x = x0 + col, y = y0 + row;
         if (x < w && y < h) out[poff + size_t(y) * q.w + x] = fuse_v(q.acc_mode, av[u], q.weight, r[u]);
+    }
+}
+
+// Two consecutive FIR components of one mixture from one pass over the input plane:
+//   out = (a + w0 G0(in)) + w1 G1(in),  a = 0 (first_mode ZERO) or acc (first_mode PLANE),
+// the mixture's accumulation in list order with each component's own taps.  A workgroup makes a
+// kPX x kPY output tile of one plane.  Per component j (radius rj): the vertical pass over the
+// columns x0 - rj .. x0 + kPX + rj - 1 runs from memory through the register window into LDS
+// (waves take 64-column groups; rows outside the plane fold by half-sample reflection, columns
+// outside it are read at their reflected index), then the horizontal pass runs from LDS, 16
+// adjacent outputs per thread, and its results go back through LDS so that the stores (and the
+// accumulation plane's loads) are coalesced.  Each component's columns start at LDS column 0
+// and the row stride is a compile-time constant, so every LDS access is a per-lane base plus an
+// immediate offset.  The float operations per output are those of k_fir_v then k_fir_h (ascending
+// taps, fused multiply-add from 0), so the result equals the two components run as separate
+// passes bit for bit, while the input is read once and neither the vertical intermediate nor the
+// first component's result reaches memory.
+constexpr int kPX = kFirPairTileW, kPY = kFirPairTileH;
+constexpr int kPS = pair_stride(kFirPairMaxR);  // LDS floats per tile row
+static_assert(kPY == kT && kPX == 16 * kPY, "one 16-row strip, one horizontal task per thread");
+
+// kAdj: the reverse pass of two bounce components (their FIR blurs are symmetric operators, so
+// t_j = G_j(in) is the transpose applied to in = g_acc): out := acc + w_{0c} t_0 + w_{1c} t_1
+// (acc may be out) and the dot products <t_j, H> of each tile go to a.part[j], row = tile,
+// column = plane (the rows from the tile count up to a.nrows are zeroed).
+template <bool kAdj>
+__global__ __launch_bounds__(256) void k_fir_pair(const float* __restrict__ in, float* out, const float* accp,
+                                                  FirPairArgs q, uint32_t nplanes, FirPairAdj a) {
+    __shared__ float lds_p[kPY * kPS];
+    __shared__ double red[2][4];
+    const int w = int(q.w), h = int(q.h);
+    const uint32_t gx = (q.w + kPX - 1) / kPX, gy = (q.h + kPY - 1) / kPY;
+    const uint32_t total = gx * gy * nplanes;
+    const uint32_t item = xcd_item(blockIdx.x, total);
+    if (item >= total) return;  // the whole workgroup
+    // vertically adjacent tiles are consecutive items of one XCD: their halo rows stay in its L2
+    const uint32_t by = item % gy, rest = item / gy, bx = rest % gx, plane = rest / gx;
+    const size_t poff = size_t(plane) * q.w * q.h;
+    const int x0 = int(bx) * kPX, y0 = int(by) * kPY;
+    const int wave = int(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
+    const int lane = int(threadIdx.x & 63u);
+    const uint32_t plane_bytes = q.w * q.h * 4u, rowb = q.w * 4u;
+    const __amdgpu_buffer_rsrc_t rin = plane_rsrc(in + poff, plane_bytes);
+    // horizontal task of this thread: tile row hr, 16-column segment hs; its window starts at
+    // LDS column 16 hs (padded index 17 hs)
+    const int hr = int(threadIdx.x >> 4), hs = int(threadIdx.x & 15u);
+    const float* hrow = lds_p + hr * kPS + 17 * hs;
+    // coalesced phase: this thread's column of every tile row
+    const int cx = x0 + int(threadIdx.x);
+    const float* crow = lds_p + hpad(int(threadIdx.x));
+    float g0[kPY];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int rj = int(q.r[j]);
+        const float* taps = q.taps[j];
+        if (j) __syncthreads();  // the previous component's reads of the LDS tile are done
+        // vertical pass over LDS columns 0 .. kPX + 2 rj - 1 (frame columns x0 - rj + c)
+        const int ncol = kPX + 2 * rj;
+        for (int g = wave; g * 64 < ncol; g += 4) {
+            const int c = g * 64 + lane;
+            int xx = x0 - rj + c;
+            if (xx < 0 || xx >= w) xx = reflect_index(xx, w);
+            const uint32_t voff = uint32_t(xx) * 4u;
+            const int yb = y0 - rj;
+            float acc[kT];
+            // the ring loads window indices up to n + 2 kT - 2 = 2 rj + 2 kT - 1: all inside the
+            // plane, or every index folded by reflection
+            if (yb >= 0 && yb + 2 * rj + 2 * kT <= h) {
+                const uint32_t sb = uint32_t(yb) * rowb;
+                fir_window_ng(2 * rj + 1, taps, [&](int t) { return buf_ld(rin, voff, sb + uint32_t(t) * rowb); }, acc);
+            } else {
+                fir_window_ng(2 * rj + 1, taps, [&](int t) {
+                    return buf_ld(rin, voff, uint32_t(reflect_index(yb + t, h)) * rowb);
+                }, acc);
+            }
+            if (c < ncol) {
+                float* col = lds_p + hpad(c);
+#pragma unroll
+                for (int k = 0; k < kT; ++k) col[k * kPS] = acc[k];
+            }
+        }
+        __syncthreads();
+        // horizontal pass from the tile: window element t is LDS column 16 hs + t
+        float r[kT];
+        fir_window_ng(2 * rj + 1, taps, [&](int t) { return hrow[t + (t >> 4)]; }, r);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kT; ++k) lds_p[hr * kPS + 17 * hs + k] = r[k];
+        __syncthreads();
+        if (j == 0) {
+#pragma unroll
+            for (int u = 0; u < kPY; ++u) g0[u] = crow[u * kPS];
+        }
+    }
+    if (!kAdj) {
+        if (cx >= w) return;
+        const __amdgpu_buffer_rsrc_t rout = plane_rsrc(out + poff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t racc = plane_rsrc(accp ? accp + poff : in + poff, plane_bytes);
+        const uint32_t voff = uint32_t(cx) * 4u;
+        float av[kPY];
+#pragma unroll
+        for (int u = 0; u < kPY; ++u)
+            av[u] = (q.first_mode == 2u && y0 + u < h) ? buf_ld(racc, voff, uint32_t(y0 + u) * rowb) : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kPY; ++u) {
+            if (y0 + u < h) {
+                const float v0 = q.first_mode == 2u ? av[u] + q.weight[0] * g0[u] : 0.0f + q.weight[0] * g0[u];
+                buf_st(v0 + q.weight[1] * crow[u * kPS], rout, voff, uint32_t(y0 + u) * rowb);
+            }
+        }
+        return;
+    }
+    // reverse pass: g_H update and the two dot products with H
+    float d0 = 0.0f, d1 = 0.0f;
+    if (cx < w) {
+        const __amdgpu_buffer_rsrc_t rout = plane_rsrc(out + poff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t racc = plane_rsrc(accp + poff, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rh = plane_rsrc(a.H + poff, plane_bytes);
+        const uint32_t voff = uint32_t(cx) * 4u;
+        const float w0 = a.w[0][plane], w1 = a.w[1][plane];
+        float av[kPY], hv[kPY];
+#pragma unroll
+        for (int u = 0; u < kPY; ++u) {
+            const bool ok = y0 + u < h;
+            av[u] = ok ? buf_ld(racc, voff, uint32_t(y0 + u) * rowb) : 0.0f;
+            hv[u] = ok ? buf_ld(rh, voff, uint32_t(y0 + u) * rowb) : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kPY; ++u) {
+            if (y0 + u < h) {
+                const float t1 = crow[u * kPS];
+                buf_st((av[u] + w0 * g0[u]) + w1 * t1, rout, voff, uint32_t(y0 + u) * rowb);
+                d0 += g0[u] * hv[u];
+                d1 += t1 * hv[u];
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        d0 += __shfl_xor(d0, o, 64);
+        d1 += __shfl_xor(d1, o, 64);
+    }
+    if (lane == 0) {
+        red[0][wave] = double(d0);
+        red[1][wave] = double(d1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const uint32_t j = threadIdx.x, tiles = gx * gy, tile = item % tiles;
+        double* part = a.part[j];
+        part[size_t(tile) * 3 + plane] = ((red[j][0] + red[j][1]) + red[j][2]) + red[j][3];
+        for (uint32_t r = tile + tiles; r < a.nrows; r += tiles) part[size_t(r) * 3 + plane] = 0.0;
     }
 }
 
@@ -763,6 +952,18 @@ void fir2d(const float* in, float* out, const float* acc, const FirArgs& a, uint
         case 7: hipLaunchKernelGGL(k_fir2d<7>, grid, block, 0, s, in, out, acc, a, nplanes); break;
         default: hipLaunchKernelGGL(k_fir2d<8>, grid, block, 0, s, in, out, acc, a, nplanes); break;
     }
+}
+
+uint32_t fir_pair_tiles(uint32_t w, uint32_t h) { return ((w + kPX - 1) / kPX) * ((h + kPY - 1) / kPY); }
+
+void fir_pair(const float* in, float* out, const float* acc, const FirPairArgs& a, uint32_t nplanes, hipStream_t s) {
+    hipLaunchKernelGGL(k_fir_pair<false>, dim3(xcd_grid(fir_pair_tiles(a.w, a.h) * nplanes)), dim3(256), 0, s, in, out,
+                       acc, a, nplanes, FirPairAdj{});
+}
+
+void fir_pair_adjoint(const float* g_acc, float* g_H, const FirPairArgs& a, const FirPairAdj& adj, hipStream_t s) {
+    hipLaunchKernelGGL(k_fir_pair<true>, dim3(xcd_grid(fir_pair_tiles(a.w, a.h) * 3)), dim3(256), 0, s, g_acc, g_H,
+                       g_H, a, 3u, adj);
 }
 
 int iir_row_chunk(uint32_t n) {

@@ -198,6 +198,38 @@ void headroom(float* p, uint32_t n, float clip, float ev, hipStream_t s);
 // The passes run on nplanes contiguous planes of a.w x a.h with the same arguments.
 void fir_v(const float* in, float* out, const FirArgs& a, uint32_t nplanes, hipStream_t s);
 void fir_h(const float* in, float* out, const float* acc, const FirArgs& a, uint32_t nplanes, hipStream_t s);
+// Two FIR components of a mixture in one pass: out = (a + weight[0] G_{r0}(in)) + weight[1] G_{r1}(in)
+// with a = 0 (first_mode 1, ZERO) or acc (first_mode 2, PLANE); 1 <= r[j] <= kFirPairMaxR, each
+// component with its own taps.  Bit-identical to fir_v + fir_h of each component in turn.
+struct FirPairArgs {
+    uint32_t w, h;
+    uint32_t r[2];
+    uint32_t first_mode;
+    float weight[2];
+    float taps[2][kFirMaxTaps];
+};
+inline constexpr int kFirPairMaxR = kFirMaxR;
+inline constexpr int kFirPairTileW = 256, kFirPairTileH = 16;  // output tile of one workgroup
+// Floats per LDS row of the pair kernel's tile for radii up to R: at least the padded staged
+// width, and 16 mod 64 so that the four rows a wavefront reads fall in distinct banks.
+__host__ __device__ constexpr int pair_stride(int R) {
+    const int t = kFirPairTileW + 2 * R, s = t + (t >> 4);
+    return s + ((16 - s) & 63);
+}
+void fir_pair(const float* in, float* out, const float* acc, const FirPairArgs& a, uint32_t nplanes, hipStream_t s);
+// The reverse pass of two consecutive bounce components with one sigma each for the three
+// channels (planar 3-channel planes; FIR blurs are symmetric, so G_j(g_acc) is the transpose):
+// g_H += w_{0c} G_0(g_acc) + w_{1c} G_1(g_acc) in place, and part[j] row t, column c := the sum
+// over output tile t of plane c of G_j(g_acc) H; rows fir_pair_tiles(w, h) .. nrows - 1 are
+// zeroed (the caller keeps fir_pair_tiles(w, h) <= nrows).  a.first_mode and a.weight are unused.
+struct FirPairAdj {
+    const float* H;
+    double* part[2];
+    uint32_t nrows;
+    float w[2][3];
+};
+uint32_t fir_pair_tiles(uint32_t w, uint32_t h);
+void fir_pair_adjoint(const float* g_acc, float* g_H, const FirPairArgs& a, const FirPairAdj& adj, hipStream_t s);
 // Recursive passes: a.m is the compile-time chunk length iir_row_chunk(line length) /
 // iir_col_chunk(column height) returns (0: too long for one workgroup).  iir_row filters `lines`
 // contiguous lines of a.n samples.

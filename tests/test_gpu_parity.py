@@ -96,6 +96,36 @@ def test_gaussian_flat_field_and_impulse(engine):
         assert out[20, 20 + r + 1, 0] == 0.0 and want.shape == (2 * r + 1, 2 * r + 1)
 
 
+# Halation mixtures: consecutive FIR components with one sigma and weight for the three channels
+# run as one pass over the input (k_fir_pair); every other component as its own blur.  FIR-only
+# mixtures are bit-exact; with a recursive component the sum is held to a few float ulp.
+MIXTURES = [
+    ("pair", [(2.0, 0.6), (8.0, 0.3)], 16.5, True),
+    ("pair_big_first", [(8.0, 0.3), (2.0, 0.6)], 16.5, True),
+    ("pair_then_iir", [(2.0, 0.6), (8.0, 0.3), (32.0, 0.1)], 16.5, False),
+    ("pair_then_single", [(1.4, 0.6), (6.1, 0.3), (12.0, 0.1)], 16.5, True),
+    ("pair_radius_63", [(3.0, 0.5), (21.0, 0.5)], 30.0, True),
+    ("identity_then_fir", [(0.3, 0.5), (2.0, 0.5)], 16.5, True),
+    ("per_channel_first", [((2.0, 3.0, 4.0), 0.6), (8.0, 0.3)], 16.5, True),
+    ("two_pairs", [(2.0, 0.6), (8.0, 0.3), (2.5, 0.1), (9.0, (0.2, 0.2, 0.2))], 16.5, True),
+    ("iir_between", [(2.0, 0.6), (20.0, 0.3), (8.0, 0.1)], 16.5, False),
+    ("radius_1", [(0.35, 0.4), (0.4, 0.6)], 16.5, True),
+]
+
+
+@pytest.mark.parametrize("shape", [(200, 300), (129, 257), (301, 17), (3, 5), (600, 1024), (16, 256), (17, 257)])
+@pytest.mark.parametrize("name,comps,crossover,exact", MIXTURES, ids=[m[0] for m in MIXTURES])
+def test_mixture_matches_oracle(engine, oracle, shape, name, comps, crossover, exact):
+    rng = np.random.default_rng(hash((shape, name)) % 2 ** 32)
+    img = rng.uniform(0.0, 2.0, shape + (3,)).astype(np.float32)
+    gpu = engine.mixture(img, comps, crossover)
+    cpu = oracle.mixture(img, comps, crossover)
+    if exact:
+        assert np.array_equal(gpu.view(np.uint32), cpu.view(np.uint32)), (name, shape, int(ulps(gpu, cpu).max()))
+    else:
+        assert ulps(gpu, cpu).max() <= 4, (name, shape)
+
+
 def render_and_compare(engine, oracle, delta, w, h, overrides=None, seed=0):
     img = synth.make_frame(w, h, seed)
     with engine.open(img, delta) as s:
