@@ -19,6 +19,8 @@
 // bit-identical.  Compiled with -ffp-contract=off (the accumulation fuse rounds a product and a sum).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fir_tile.h"
 #include "kernels.h"
 #include "xcd.h"
@@ -467,16 +469,20 @@ __global__ __launch_bounds__(kF2NT) void k_fir2d(const float* __restrict__ in, f
 // passes bit for bit, while the input is read once and neither the vertical intermediate nor the
 // first component's result reaches memory.
 constexpr int kPX = kFirPairTileW, kPY = kFirPairTileH;
-constexpr int kPS = pair_stride(kFirPairMaxR);  // LDS floats per tile row
 static_assert(kPY == kT && kPX == 16 * kPY, "one 16-row strip, one horizontal task per thread");
+// Two tile sizes: radii up to kPairSmallR (a smaller LDS tile, more workgroups per CU) or up to
+// kFirPairMaxR.
+constexpr int kPairSmallR = 24;
 
 // kAdj: the reverse pass of two bounce components (their FIR blurs are symmetric operators, so
 // t_j = G_j(in) is the transpose applied to in = g_acc): out := acc + w_{0c} t_0 + w_{1c} t_1
 // (acc may be out) and the dot products <t_j, H> of each tile go to a.part[j], row = tile,
 // column = plane (the rows from the tile count up to a.nrows are zeroed).
-template <bool kAdj>
-__global__ __launch_bounds__(256) void k_fir_pair(const float* __restrict__ in, float* out, const float* accp,
-                                                  FirPairArgs q, uint32_t nplanes, FirPairAdj a) {
+template <bool kAdj, int kRMax>
+__global__ __launch_bounds__(256, kRMax <= kPairSmallR ? 6 : 5) void k_fir_pair(const float* __restrict__ in, float* out,
+                                                                          const float* accp, FirPairArgs q,
+                                                                          uint32_t nplanes, FirPairAdj a) {
+    constexpr int kPS = pair_stride(kRMax);  // LDS floats per tile row
     __shared__ float lds_p[kPY * kPS];
     __shared__ double red[2][4];
     const int w = int(q.w), h = int(q.h);
@@ -957,13 +963,19 @@ void fir2d(const float* in, float* out, const float* acc, const FirArgs& a, uint
 uint32_t fir_pair_tiles(uint32_t w, uint32_t h) { return ((w + kPX - 1) / kPX) * ((h + kPY - 1) / kPY); }
 
 void fir_pair(const float* in, float* out, const float* acc, const FirPairArgs& a, uint32_t nplanes, hipStream_t s) {
-    hipLaunchKernelGGL(k_fir_pair<false>, dim3(xcd_grid(fir_pair_tiles(a.w, a.h) * nplanes)), dim3(256), 0, s, in, out,
-                       acc, a, nplanes, FirPairAdj{});
+    const dim3 grid(xcd_grid(fir_pair_tiles(a.w, a.h) * nplanes));
+    if (std::max(a.r[0], a.r[1]) <= uint32_t(kPairSmallR))
+        hipLaunchKernelGGL((k_fir_pair<false, kPairSmallR>), grid, dim3(256), 0, s, in, out, acc, a, nplanes, FirPairAdj{});
+    else
+        hipLaunchKernelGGL((k_fir_pair<false, kFirPairMaxR>), grid, dim3(256), 0, s, in, out, acc, a, nplanes, FirPairAdj{});
 }
 
 void fir_pair_adjoint(const float* g_acc, float* g_H, const FirPairArgs& a, const FirPairAdj& adj, hipStream_t s) {
-    hipLaunchKernelGGL(k_fir_pair<true>, dim3(xcd_grid(fir_pair_tiles(a.w, a.h) * 3)), dim3(256), 0, s, g_acc, g_H,
-                       g_H, a, 3u, adj);
+    const dim3 grid(xcd_grid(fir_pair_tiles(a.w, a.h) * 3));
+    if (std::max(a.r[0], a.r[1]) <= uint32_t(kPairSmallR))
+        hipLaunchKernelGGL((k_fir_pair<true, kPairSmallR>), grid, dim3(256), 0, s, g_acc, g_H, g_H, a, 3u, adj);
+    else
+        hipLaunchKernelGGL((k_fir_pair<true, kFirPairMaxR>), grid, dim3(256), 0, s, g_acc, g_H, g_H, a, 3u, adj);
 }
 
 int iir_row_chunk(uint32_t n) {

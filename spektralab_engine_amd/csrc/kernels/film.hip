@@ -808,8 +808,9 @@ __device__ inline uint32_t frame_of_tile(const uint32_t* __restrict__ prefix, ui
 }
 
 // k_develop over many frames: persistent workgroups stride over every frame's 256-pixel tiles.
-// The lgamma table is the same for every frame (the sampler's own function).
-__global__ __launch_bounds__(256) void k_develop_batch(const DevelopJob* __restrict__ jobs,
+// The lgamma table is the same for every frame (the sampler's own function).  Held to k_develop's
+// 7 wavefronts a SIMD: the issue-bound draws need the occupancy more than the registers.
+__global__ __launch_bounds__(256, 7) void k_develop_batch(const DevelopJob* __restrict__ jobs,
                                                        const uint32_t* __restrict__ prefix, uint32_t nframes,
                                                        const uint32_t* __restrict__ seeds) {
     __shared__ float s_lgam[spkm::kLgammaTable];
