@@ -819,17 +819,19 @@ __global__ __launch_bounds__(128) void k_iir_row(const float* __restrict__ in, f
 
 constexpr uint32_t kColW = 16, kColChunks = 32;
 
-// A strip of 16 columns of one plane per workgroup (nplanes contiguous planes, strips walked in
-// XCD order); thread (tx, j) holds rows [j MC, j MC + MC) of column x in registers.  Loads clamp
+// A strip of CW columns of one plane per workgroup of 512 threads (512 / CW chunks: 16 columns x
+// 32 chunks, or 8 x 64 for taller columns at half the registers a thread; nplanes contiguous
+// planes, strips walked in XCD order); thread (tx, j) holds rows [j MC, j MC + MC) of column x in
+// registers.  Loads clamp
 // the row to the chunk's last valid row (lanes past the frame edge read column w-1), so every load
 // stays inside the plane; stores use a wave-uniform row offset and only rows < n.  The mixture
 // accumulation (out = acc + w y, or 0 + w y) is fused into the store.
-template <int MC, bool kAdj>
+template <int MC, bool kAdj, uint32_t CW>
 __global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, float* __restrict__ out,
                                                  const float* __restrict__ accp, IirArgs q, uint32_t nplanes) {
-    __shared__ double V[kColChunks * kColW * 3];
+    __shared__ double V[512 * 3];
     const uint32_t w = q.w, n = q.n, nch = q.nch;
-    const uint32_t nstrips = (w + kColW - 1) / kColW;
+    const uint32_t nstrips = (w + CW - 1) / CW;
     const uint32_t total = nstrips * nplanes;
     const uint32_t item = xcd_item(blockIdx.x, total);
     if (item >= total) return;
@@ -839,7 +841,7 @@ __global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, f
     out += poff;
     if (accp) accp += poff;
     const uint32_t tx = threadIdx.x, j = threadIdx.y;
-    const uint32_t x = strip * kColW + tx;
+    const uint32_t x = strip * CW + tx;
     const bool colok = x < w;
     const bool has = j < nch;
     const uint32_t start = j * uint32_t(MC);
@@ -856,7 +858,7 @@ __global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, f
 #pragma unroll
     for (int k = 0; k < MC; ++k) v[k] = buf_ld(rin, lbase + min(uint32_t(k), kmax) * rowb, 0u);
     const double x0 = double(buf_ld(rin, xc * 4u, 0u));
-    yvv_chunks<MC, kAdj>(v, j, nch, len, x0, q, V + tx * 3, kColW * 3);
+    yvv_chunks<MC, kAdj>(v, j, nch, len, x0, q, V + tx * 3, CW * 3);
     if (!colok || !has) return;
     const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc(out, (short)0, plane_bytes, 0x00020000);
     if (q.acc_mode == 2u) {
@@ -1003,21 +1005,31 @@ void iir_row(const float* in, float* out, const IirArgs& a, uint32_t lines, hipS
     else iir_row_launch<false>(in, out, a, lines, s);
 }
 
+// Chunk length for a column of h samples: 16 columns x 32 chunks of 16, 32 or 64 samples up to
+// 2048 rows; above that 8 columns x 64 chunks of 64 (up to 4096 rows) or 128 (up to 8192).
 int iir_col_chunk(uint32_t h) {
-    for (int mc : {16, 32, 64, 128})
+    for (int mc : {16, 32, 64})
         if ((h + uint32_t(mc) - 1) / uint32_t(mc) <= kColChunks) return mc;
+    for (int mc : {64, 128})
+        if ((h + uint32_t(mc) - 1) / uint32_t(mc) <= 2 * kColChunks) return mc;
     return 0;
 }
 
 template <bool kAdj>
 void iir_col_launch(const float* in, float* out, const float* acc, const IirArgs& a, uint32_t nplanes, hipStream_t s) {
-    const dim3 grid(xcd_grid(((a.w + kColW - 1) / kColW) * nplanes)), block(kColW, kColChunks);
-    switch (a.m) {
-        case 16: hipLaunchKernelGGL((k_iir_col<16, kAdj>), grid, block, 0, s, in, out, acc, a, nplanes); break;
-        case 32: hipLaunchKernelGGL((k_iir_col<32, kAdj>), grid, block, 0, s, in, out, acc, a, nplanes); break;
-        case 64: hipLaunchKernelGGL((k_iir_col<64, kAdj>), grid, block, 0, s, in, out, acc, a, nplanes); break;
-        default: hipLaunchKernelGGL((k_iir_col<128, kAdj>), grid, block, 0, s, in, out, acc, a, nplanes); break;
+    if (a.nch <= kColChunks) {
+        const dim3 grid(xcd_grid(((a.w + kColW - 1) / kColW) * nplanes)), block(kColW, kColChunks);
+        switch (a.m) {
+            case 16: hipLaunchKernelGGL((k_iir_col<16, kAdj, kColW>), grid, block, 0, s, in, out, acc, a, nplanes); break;
+            case 32: hipLaunchKernelGGL((k_iir_col<32, kAdj, kColW>), grid, block, 0, s, in, out, acc, a, nplanes); break;
+            default: hipLaunchKernelGGL((k_iir_col<64, kAdj, kColW>), grid, block, 0, s, in, out, acc, a, nplanes); break;
+        }
+        return;
     }
+    constexpr uint32_t CW = kColW / 2;
+    const dim3 grid(xcd_grid(((a.w + CW - 1) / CW) * nplanes)), block(CW, 2 * kColChunks);
+    if (a.m == 64) hipLaunchKernelGGL((k_iir_col<64, kAdj, CW>), grid, block, 0, s, in, out, acc, a, nplanes);
+    else hipLaunchKernelGGL((k_iir_col<128, kAdj, CW>), grid, block, 0, s, in, out, acc, a, nplanes);
 }
 
 void iir_col(const float* in, float* out, const float* acc, const IirArgs& a, uint32_t nplanes, hipStream_t s) {

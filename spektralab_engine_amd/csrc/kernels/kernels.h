@@ -231,8 +231,9 @@ struct FirPairAdj {
 uint32_t fir_pair_tiles(uint32_t w, uint32_t h);
 void fir_pair_adjoint(const float* g_acc, float* g_H, const FirPairArgs& a, const FirPairAdj& adj, hipStream_t s);
 // Recursive passes: a.m is the compile-time chunk length iir_row_chunk(line length) /
-// iir_col_chunk(column height) returns (0: too long for one workgroup).  iir_row filters `lines`
-// contiguous lines of a.n samples.
+// iir_col_chunk(column height) returns (0: too long for one workgroup), a.nch = ceil(a.n / a.m)
+// (the column pass takes up to 64 chunks, rows up to 128).  iir_row filters `lines` contiguous
+// lines of a.n samples.
 // Both passes in one kernel for 1 <= a.r <= kFir2dMaxR (a.acc_mode applies to the result).
 inline constexpr int kFir2dMaxR = 8;
 void fir2d(const float* in, float* out, const float* acc, const FirArgs& a, uint32_t nplanes, hipStream_t s);
