@@ -81,20 +81,20 @@ k::SpecMfma spec_mfma(const double* eps64, const double* beta64, const double* w
 }
 
 // Powers A^k of the recursion's companion matrix A = [[B1 B2 B3] [1 0 0] [0 1 0]] for the chunk
-// carries: pw[d] = A^(m 2^d) (d < 7) and last = A^len_last.  For wide Gaussians the three poles
+// carries: pw[d] = A^(m 2^d) (d < 8) and last = A^len_last.  For wide Gaussians the three poles
 // nearly coincide near 1 and A is far from normal, so repeated squaring loses most digits (at
 // sigma 100 the entries of A^288 come out ~5e-4 wrong in float64).  Column i of A^k is instead the
 // state after k zero-input steps of the recursion itself from unit state e_i, stepped in long
 // double: the same well-conditioned arithmetic as the serial recursion.
 struct IirPowers {
-    double pw[7][9];
+    double pw[8][9];
     double last[9];
 };
 
 IirPowers iir_powers(const setup::YvvCoeffs& c, uint32_t m, uint32_t len_last) {
     IirPowers r{};
     const long double b1 = c.b1, b2 = c.b2, b3 = c.b3;
-    const uint64_t kmax = uint64_t(m) << 6;
+    const uint64_t kmax = uint64_t(m) << 7;
     for (int col = 0; col < 3; ++col) {
         long double s0 = col == 0, s1 = col == 1, s2 = col == 2;
         uint64_t next = m;

@@ -764,27 +764,28 @@ __device__ inline uint32_t rpad(uint32_t i) { return i + (i >> 5); }
 __host__ inline size_t row_lds_floats(uint32_t n) { return size_t(n) + n / 32 + 1; }
 
 // One line of n = q.n samples per workgroup (lines of all planes are contiguous: line b starts at
-// b n); 128 threads, chunks of MC (nch <= 128).  The line is staged through LDS (one pad word per
-// 32), each thread's chunk held in registers.
-template <int MC, bool kAdj>
-__global__ __launch_bounds__(128) void k_iir_row(const float* __restrict__ in, float* __restrict__ out, IirArgs q) {
+// b n); NT threads (128 for lines up to 2048 samples, 256 above: shorter chunks, fewer registers
+// a thread), chunks of MC (nch <= NT).  The line is staged through LDS (one pad word per 32),
+// each thread's chunk held in registers.
+template <int MC, bool kAdj, uint32_t NT>
+__global__ __launch_bounds__(NT) void k_iir_row(const float* __restrict__ in, float* __restrict__ out, IirArgs q) {
     extern __shared__ float srow[];
-    __shared__ double V[kRowThreads * 3];
+    __shared__ double V[NT * 3];
     const uint32_t n = q.n, nch = q.nch;
     const float* src = in + size_t(blockIdx.x) * n;
     float* dst = out + size_t(blockIdx.x) * n;
     const uint32_t j = threadIdx.x;
     // kStage coalesced loads in flight per thread before their LDS stores
-    for (uint32_t i0 = 0; i0 < n; i0 += kRowThreads * kStage) {
+    for (uint32_t i0 = 0; i0 < n; i0 += NT * kStage) {
         float t[kStage];
 #pragma unroll
         for (uint32_t u = 0; u < kStage; ++u) {
-            const uint32_t i = i0 + u * kRowThreads + j;
+            const uint32_t i = i0 + u * NT + j;
             t[u] = i < n ? src[i] : 0.0f;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kStage; ++u) {
-            const uint32_t i = i0 + u * kRowThreads + j;
+            const uint32_t i = i0 + u * NT + j;
             if (i < n) srow[rpad(i)] = t[u];
         }
     }
@@ -802,16 +803,16 @@ __global__ __launch_bounds__(128) void k_iir_row(const float* __restrict__ in, f
     for (int k = 0; k < MC; ++k)
         if (uint32_t(k) < len) srow[rpad(start + uint32_t(k))] = v[k];
     __syncthreads();
-    for (uint32_t i0 = 0; i0 < n; i0 += kRowThreads * kStage) {
+    for (uint32_t i0 = 0; i0 < n; i0 += NT * kStage) {
         float t[kStage];
 #pragma unroll
         for (uint32_t u = 0; u < kStage; ++u) {
-            const uint32_t i = i0 + u * kRowThreads + j;
+            const uint32_t i = i0 + u * NT + j;
             t[u] = i < n ? srow[rpad(i)] : 0.0f;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kStage; ++u) {
-            const uint32_t i = i0 + u * kRowThreads + j;
+            const uint32_t i = i0 + u * NT + j;
             if (i < n) dst[i] = t[u];
         }
     }
@@ -980,23 +981,30 @@ void fir_pair_adjoint(const float* g_acc, float* g_H, const FirPairArgs& a, cons
         hipLaunchKernelGGL((k_fir_pair<true, kFirPairMaxR>), grid, dim3(256), 0, s, g_acc, g_H, g_H, a, 3u, adj);
 }
 
+// Chunk length for a line of n samples: chunks of 16 on 128 threads up to 2048 samples; above,
+// 256 threads with chunks of 16, 24, 32, 48 or 64 (up to 16384 samples).
 int iir_row_chunk(uint32_t n) {
-    for (int mc : {16, 32, 48, 64, 96, 128})
-        if ((n + uint32_t(mc) - 1) / uint32_t(mc) <= kRowThreads) return mc;
+    if (n <= 16 * kRowThreads) return 16;
+    for (int mc : {16, 24, 32, 48, 64})
+        if ((n + uint32_t(mc) - 1) / uint32_t(mc) <= 2 * kRowThreads) return mc;
     return 0;
 }
 
 template <bool kAdj>
 void iir_row_launch(const float* in, float* out, const IirArgs& a, uint32_t lines, hipStream_t s) {
     const size_t lds = row_lds_floats(a.n) * sizeof(float);
-    const dim3 grid(lines), block(kRowThreads);
+    const dim3 grid(lines);
+    if (a.nch <= kRowThreads) {
+        hipLaunchKernelGGL((k_iir_row<16, kAdj, kRowThreads>), grid, dim3(kRowThreads), lds, s, in, out, a);
+        return;
+    }
+    constexpr uint32_t NT = 2 * kRowThreads;
     switch (a.m) {
-        case 16: hipLaunchKernelGGL((k_iir_row<16, kAdj>), grid, block, lds, s, in, out, a); break;
-        case 32: hipLaunchKernelGGL((k_iir_row<32, kAdj>), grid, block, lds, s, in, out, a); break;
-        case 48: hipLaunchKernelGGL((k_iir_row<48, kAdj>), grid, block, lds, s, in, out, a); break;
-        case 64: hipLaunchKernelGGL((k_iir_row<64, kAdj>), grid, block, lds, s, in, out, a); break;
-        case 96: hipLaunchKernelGGL((k_iir_row<96, kAdj>), grid, block, lds, s, in, out, a); break;
-        default: hipLaunchKernelGGL((k_iir_row<128, kAdj>), grid, block, lds, s, in, out, a); break;
+        case 16: hipLaunchKernelGGL((k_iir_row<16, kAdj, NT>), grid, dim3(NT), lds, s, in, out, a); break;
+        case 24: hipLaunchKernelGGL((k_iir_row<24, kAdj, NT>), grid, dim3(NT), lds, s, in, out, a); break;
+        case 32: hipLaunchKernelGGL((k_iir_row<32, kAdj, NT>), grid, dim3(NT), lds, s, in, out, a); break;
+        case 48: hipLaunchKernelGGL((k_iir_row<48, kAdj, NT>), grid, dim3(NT), lds, s, in, out, a); break;
+        default: hipLaunchKernelGGL((k_iir_row<64, kAdj, NT>), grid, dim3(NT), lds, s, in, out, a); break;
     }
 }
 

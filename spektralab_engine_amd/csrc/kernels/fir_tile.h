@@ -31,33 +31,46 @@ struct FirTile {
     float v[TY * kS];
 };
 
-// Stages the input region whose output tile starts at (x0, y0) of a w x h plane; NT threads,
-// all loads of a thread in flight before its LDS stores.
+// Stages the input region whose output tile starts at (x0, y0) of a w x h plane; NT threads.
+// Row by row: wave v of the NT / 64 takes staged rows v, v + NT / 64, ...; lane l loads staged
+// columns l and l + 64 (when < kW) of each.  A lane's columns are folded once, a row's (wave-
+// uniform) only when the region crosses the top or bottom edge, and every load of a lane is in
+// flight before its LDS stores.
 template <int R, int TX, int TY, int NT>
 __device__ inline void fir_tile_load(FirTile<R, TX, TY>& t, const float* __restrict__ p, int w, int h, int x0,
                                      int y0) {
     using T = FirTile<R, TX, TY>;
-    constexpr int N = T::kH * T::kW, PER = (N + NT - 1) / NT;
-    const int tid = int(threadIdx.x);
-    float r[PER];
+    constexpr int NW = NT / 64, CPL = (T::kW + 63) / 64, RPW = (T::kH + NW - 1) / NW;
+    static_assert(NT % 64 == 0 && CPL <= 2, "whole wavefronts; at most two staged columns a lane");
+    const int wave = int(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)), lane = int(threadIdx.x & 63u);
+    int xs[CPL];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int e = tid + u * NT;
-        r[u] = 0.0f;
-        if (e < N) {
-            const int row = e / T::kW, col = e - row * T::kW;
-            int yy = y0 - R + row, xx = x0 - R + col;
-            if (yy < 0 || yy >= h) yy = reflect_index(yy, h);
-            if (xx < 0 || xx >= w) xx = reflect_index(xx, w);
-            r[u] = p[size_t(yy) * w + xx];
+    for (int k = 0; k < CPL; ++k) {
+        const int c = lane + 64 * k < T::kW ? lane + 64 * k : T::kW - 1;
+        int xx = x0 - R + c;
+        if (xx < 0 || xx >= w) xx = reflect_index(xx, w);
+        xs[k] = xx;
+    }
+    const bool rows_inside = y0 - R >= 0 && y0 - R + T::kH <= h;
+    float r[RPW][CPL];
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+        const int row = wave + NW * u;
+        if (row < T::kH) {
+            int yy = y0 - R + row;
+            if (!rows_inside) yy = reflect_index(yy, h);
+            const float* pr = p + size_t(yy) * size_t(w);
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) r[u][k] = pr[xs[k]];
         }
     }
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int e = tid + u * NT;
-        if (e < N) {
-            const int row = e / T::kW, col = e - row * T::kW;
-            t.in[row * T::kS + col] = r[u];
+    for (int u = 0; u < RPW; ++u) {
+        const int row = wave + NW * u;
+        if (row < T::kH) {
+#pragma unroll
+            for (int k = 0; k < CPL; ++k)
+                if (lane + 64 * k < T::kW) t.in[row * T::kS + lane + 64 * k] = r[u][k];
         }
     }
 }

@@ -30,7 +30,7 @@ struct FirArgs {
 
 // One channel of a Young-van Vliet pass along lines of length n, processed in nch chunks of m.
 // The carries between chunks use powers of the companion matrix A (row-major 3x3, state
-// (w_i, w_{i-1}, w_{i-2})): pw[d] = A^(m 2^d), last = A^(n - (nch-1) m).
+// (w_i, w_{i-1}, w_{i-2})): pw[d] = A^(m 2^d) (d < 8: up to 256 chunks), last = A^(n - (nch-1) m).
 struct IirArgs {
     uint32_t w, h;       // plane size
     uint32_t n, m, nch;  // line length, chunk length, chunk count
@@ -38,7 +38,7 @@ struct IirArgs {
     float weight;
     uint32_t adjoint;    // 1: the transposed pass (zero starts, edge terms moved to the far ends)
     double B, B1, B2, B3;
-    double pw[7][9];
+    double pw[8][9];
     double last[9];
 };
 
@@ -232,7 +232,7 @@ uint32_t fir_pair_tiles(uint32_t w, uint32_t h);
 void fir_pair_adjoint(const float* g_acc, float* g_H, const FirPairArgs& a, const FirPairAdj& adj, hipStream_t s);
 // Recursive passes: a.m is the compile-time chunk length iir_row_chunk(line length) /
 // iir_col_chunk(column height) returns (0: too long for one workgroup), a.nch = ceil(a.n / a.m)
-// (the column pass takes up to 64 chunks, rows up to 128).  iir_row filters `lines` contiguous
+// (the column pass takes up to 64 chunks, rows up to 256).  iir_row filters `lines` contiguous
 // lines of a.n samples.
 // Both passes in one kernel for 1 <= a.r <= kFir2dMaxR (a.acc_mode applies to the result).
 inline constexpr int kFir2dMaxR = 8;
