@@ -479,7 +479,7 @@ constexpr int kPairSmallR = 24;
 // (acc may be out) and the dot products <t_j, H> of each tile go to a.part[j], row = tile,
 // column = plane (the rows from the tile count up to a.nrows are zeroed).
 template <bool kAdj, int kRMax>
-__global__ __launch_bounds__(256, kRMax <= kPairSmallR ? 6 : 5) void k_fir_pair(const float* __restrict__ in, float* out,
+__global__ __launch_bounds__(256, 5) void k_fir_pair(const float* __restrict__ in, float* out,
                                                                           const float* accp, FirPairArgs q,
                                                                           uint32_t nplanes, FirPairAdj a) {
     constexpr int kPS = pair_stride(kRMax);  // LDS floats per tile row
