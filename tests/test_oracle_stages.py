@@ -262,3 +262,25 @@ def test_print_lut_trilinear_reproduces_an_affine_table(oracle):
     far = np.stack([np.full((1, 2), v, np.float32) for v in (-9.0, 9.0, 9.0)])
     o = oracle.print_lut(far, table, axes)
     np.testing.assert_array_equal(o[0, 0, :3], np.floor(np.clip(table[0, -1, -1], 0, 1) * 65535 + 0.5))
+
+
+# ---- halation mixture: list-order accumulation of the pinned Gaussians --------------------------
+@pytest.mark.parametrize("comps,crossover", [
+    ([(2.0, 0.6), (8.0, 0.3), (32.0, 0.1)], 16.5),
+    ([((2.0, 3.0, 4.0), (0.5, 0.6, 0.7)), (8.0, 0.3)], 16.5),
+    ([(21.0, 0.5), (3.0, 0.5)], 30.0),
+    ([(0.3, 0.4), (1.4, 0.6)], 3.0),
+])
+def test_mixture_is_the_list_order_sum_of_gaussians(oracle, comps, crossover):
+    """oracle.mixture = ((0 + w0 G0) + w1 G1) + ... in binary32, each G the oracle's per-channel
+    Gaussian (gaussian.cpp:254-273: the first component ZERO-mode, the rest accumulated)."""
+    rng = np.random.default_rng(5)
+    img = rng.uniform(0.0, 2.0, (37, 53, 3)).astype(np.float32)
+    got = oracle.mixture(img, comps, crossover)
+    acc = None
+    for s, w in comps:
+        g = oracle.gaussian(img, np.broadcast_to(s, 3), crossover)
+        term = np.broadcast_to(np.float32(w), (3,)).astype(np.float32) * g
+        acc = np.float32(0.0) + term if acc is None else acc + term
+    assert acc.dtype == np.float32
+    np.testing.assert_array_equal(got, acc)
