@@ -87,13 +87,14 @@ SPK_API spk_status spkx_output_transform_host(const char* resources_dir, const c
 /* Batched rendering.  spkx_batch_create snapshots the parameters of n open sessions of one engine
  * and captures one hipGraph that renders every session's full tier (develop + print, fresh seeds
  * from each session's own sequence at every run, as spk_render draws them) into dev_out[i]
- * (RGBA16, row pitch row_stride_px[i] >= width).  The develop and print kernels run as one
- * persistent launch over all frames; the per-frame blur passes run on forked streams.
- * options_json (NULL = defaults): {"autotune": bool (default true: the stream count and the
- * persistent launches' grid and workgroup size of the phased schedule are chosen by timing
- * candidate graphs), "schedule": "phased" (one develop and one print launch for all frames
- * between per-frame phases; the default) | "streams" (each frame's whole chain on its stream;
- * with autotune off, and no session in FP16 spectral precision), "streams": 1..4,
+ * (RGBA16, row pitch row_stride_px[i] >= width), on up to 4 forked streams.
+ * options_json (NULL = defaults): {"autotune": bool (default true: the schedule follows the
+ * sessions -- "streams" unless a session is in FP16 spectral precision, then "phased" -- and its
+ * stream count, persistent-launch grid and workgroup size are chosen by timing candidate graphs;
+ * "schedule" is then ignored), "schedule": "phased" (one develop and one print launch for all
+ * frames between per-frame phases; the default with autotune off) | "streams" (each frame's whole
+ * chain on its stream, its develop a persistent launch over that frame; no session in FP16
+ * spectral precision), "streams": 1..4,
  * "record_grain": bool (keep each frame's grain draws, spkx_batch_grain_record),
  * "grid_per_cu": 1..64, "block": 64|128|256}.  Parameter changes made to the
  * sessions after creation are not seen by the batch.  Returns NULL with the reason in

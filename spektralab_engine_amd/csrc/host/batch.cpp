@@ -78,7 +78,6 @@ std::unique_ptr<Batch> Batch::create(Engine& e, Session** sessions, int n, void*
     if (!sessions || !outs || !strides) fail(SPK_ERR_INVALID_ARG, "sessions, outputs and strides must not be NULL");
     bool autotune = true, record = false;
     BatchConfig fixed = parse_options(options, &autotune, &record);
-    if (autotune) fixed.mode = 0;
     std::unique_ptr<Batch> b(new Batch);
     b->e_ = &e;
     gpu::DeviceScope ds(e.device());
@@ -132,7 +131,16 @@ std::unique_ptr<Batch> Batch::create(Engine& e, Session** sessions, int n, void*
         b->graph_ = b->capture(fixed);
         return b;
     }
-    // Coordinate descent over (streams, grid per CU, block), each candidate timed as a whole graph.
+    // Coordinate descent over (streams, grid per CU, block) within one schedule, each candidate timed
+    // as a whole graph.  The schedule is chosen, not tuned: each frame's whole chain on its own
+    // stream when no frame is FP16-spectral (a frame's develop, issue-bound, then runs beside
+    // other frames' latency-bound blurs: 24.8 against 26.2 ms on the 64-frame workload), else the
+    // phased schedule.  One tuning sequence never captures both schedules: a sequence that did so
+    // with an FP16-spectral frame once ended in a host crash whose cause is not established.
+    bool any_fp16 = false;
+    for (const Frame& f : b->frames_) any_fp16 = any_fp16 || f.bf.plan->spectral_fp16;
+    fixed.mode = any_fp16 ? 0 : 1;
+    if (fixed.mode == 1) fixed.grid_per_cu = 64;
     double best = 1e300;
     auto eval = [&](const BatchConfig& c) {
         std::unique_ptr<Graph> g = b->capture(c);
@@ -144,9 +152,6 @@ std::unique_ptr<Batch> Batch::create(Engine& e, Session** sessions, int n, void*
             b->graph_ = std::move(g);
         }
     };
-    // The per-frame-streams schedule is not among the candidates: a tuning sequence that mixed it
-    // with the phased schedule (and an FP16-spectral frame) ended in a host crash whose cause is
-    // not established; it runs only when asked for explicitly (without FP16-spectral frames).
     eval(fixed);
     for (int s : {1, 2, 4})
         if (s != b->config_.streams) {

@@ -12,10 +12,10 @@
 //   6. per-frame unsharp blur and 16-bit encode into each frame's output on the S streams.
 // Per-frame parameters (print exposure, filters, halation sigmas, ...) travel in each frame's job
 // and blur arguments.  A second schedule (mode 1) skips the phase barriers: each frame's whole
-// chain runs on its stream, its develop as a persistent launch over that frame alone (explicit
-// option only; not with FP16-spectral sessions).  An autotuner picks the stream count S and the
-// persistent launches' grid and workgroup size of the phased schedule by timing candidate graphs
-// (coordinate descent) when the batch is created.
+// chain runs on its stream, its develop as a persistent launch over that frame alone (not with
+// FP16-spectral sessions).  Batches without FP16-spectral frames use it by default.  An
+// autotuner picks the stream count S and the persistent launches' grid and workgroup size of the
+// batch's schedule by timing candidate graphs (coordinate descent) when the batch is created.
 #pragma once
 
 #include <hip/hip_runtime.h>

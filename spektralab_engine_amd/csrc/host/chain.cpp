@@ -184,16 +184,20 @@ void fir_planes(const float* in, float* out, double sigma, uint32_t mode, const 
 // Recursive branch: a row pass into tmp, then the column pass (fused accumulation) into out;
 // columns taller than one workgroup's 32 chunks run as transpose, row pass, transpose.
 // adjoint: the transposed passes (the filter's transpose; rows and columns commute).
+// The row pass alone: in -> tmp, np planes of w x h.
+void iir_rows(const float* in, float* tmp, double sigma, uint32_t w, uint32_t h, uint32_t np, hipStream_t s,
+              bool adjoint = false) {
+    const int m = k::iir_row_chunk(w);
+    if (!m) fail_internal("recursive blur: rows longer than 16384 samples are not supported");
+    k::IirArgs a = iir_args(sigma, w, h, w, uint32_t(m), (w + uint32_t(m) - 1) / uint32_t(m));
+    a.adjoint = adjoint ? 1u : 0u;
+    k::iir_row(in, tmp, a, h * np, s);
+}
+
 void iir_planes(const float* in, float* out, double sigma, uint32_t mode, const float* acc, double weight, uint32_t w,
                 uint32_t h, uint32_t np, float* tmp, float* tmp2, hipStream_t s, bool adjoint = false) {
     const size_t npx = size_t(w) * h;
-    {
-        const int m = k::iir_row_chunk(w);
-        if (!m) fail_internal("recursive blur: rows longer than 16384 samples are not supported");
-        k::IirArgs a = iir_args(sigma, w, h, w, uint32_t(m), (w + uint32_t(m) - 1) / uint32_t(m));
-        a.adjoint = adjoint ? 1u : 0u;
-        k::iir_row(in, tmp, a, h * np, s);
-    }
+    iir_rows(in, tmp, sigma, w, h, np, s, adjoint);
     const int mc = k::iir_col_chunk(h);
     if (mc) {
         k::IirArgs a = iir_args(sigma, w, h, h, uint32_t(mc), (h + uint32_t(mc) - 1) / uint32_t(mc));
@@ -357,14 +361,23 @@ int uniform_fir_radius(const spk_mix_component& m, double crossover) {
     return r <= k::kFirPairMaxR ? r : 0;
 }
 
+// A component with one sigma and one weight for its three channels on the recursive branch.
+bool uniform_iir(const spk_mix_component& m, double crossover) {
+    const double sg = m.sigma[0];
+    return sg == m.sigma[1] && sg == m.sigma[2] && m.weight[0] == m.weight[1] && m.weight[0] == m.weight[2] &&
+           sg > 0.0 && !(sg < crossover);
+}
+
 }  // namespace
 
 // M(in) accumulated in list order into out (src/fx/gaussian.cpp:254-273), out != in.  Two
 // consecutive components that both take the FIR branch with one sigma and weight for the three
 // channels run as one pass over `in` (k::fir_pair, the same float operations as the two
-// components' separate passes).
+// components' separate passes).  With a fork, a recursive component right after such a pair
+// (one sigma and weight for the three channels) runs its row pass on the auxiliary stream while
+// the pair runs, and its column pass, which accumulates into out, after both.
 void mixture(const float* in, float* out, const spk_mix_component* comps, uint32_t n, uint32_t w, uint32_t h,
-             double crossover, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep) {
+             double crossover, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep, const Fork* fork) {
     for (uint32_t k = 0; k < n; ++k) {
         const uint32_t mode = k == 0 ? 1u : 2u;
         if (k + 1 < n && in != out) {
@@ -381,8 +394,28 @@ void mixture(const float* in, float* out, const spk_mix_component* comps, uint32
                     for (size_t i = 0; i < taps.size(); ++i) a.taps[j][i] = taps[i];
                     a.weight[j] = num::f32(m.weight[0]);
                 }
+                const bool rec = fork && k + 2 < n && uniform_iir(comps[k + 2], crossover) && k::iir_col_chunk(h);
+                if (!rec) {
+                    k::fir_pair(in, out, out, a, 3, s);
+                    ++k;
+                    continue;
+                }
+                const spk_mix_component& m2 = comps[k + 2];
+                const size_t np = size_t(w) * h;
+                gpu::DevBuf tmp = pool.acquire(3 * np * sizeof(float));
+                keep.push_back(tmp);
+                gpu::check(hipEventRecord(fork->forked, s), "hipEventRecord(fork)");
+                gpu::check(hipStreamWaitEvent(fork->aux, fork->forked, 0), "hipStreamWaitEvent(fork)");
+                iir_rows(in, gpu::as<float>(tmp), m2.sigma[0], w, h, 3, fork->aux);
+                gpu::check(hipEventRecord(fork->joined, fork->aux), "hipEventRecord(join)");
                 k::fir_pair(in, out, out, a, 3, s);
-                ++k;
+                gpu::check(hipStreamWaitEvent(s, fork->joined, 0), "hipStreamWaitEvent(join)");
+                const int mc = k::iir_col_chunk(h);
+                k::IirArgs ca = iir_args(m2.sigma[0], w, h, h, uint32_t(mc), (h + uint32_t(mc) - 1) / uint32_t(mc));
+                ca.acc_mode = 2u;
+                ca.weight = num::f32(m2.weight[0]);
+                k::iir_col(gpu::as<float>(tmp), out, out, ca, 3, s);
+                k += 2;
                 continue;
             }
         }
@@ -599,7 +632,8 @@ float device_max(const float* p, size_t n, gpu::Pool& pool, hipStream_t s, std::
 }
 
 void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
-                 std::vector<gpu::DevBuf>& keep, DevelopStage* st, Profiler* prof, const GrainRecord* rec) {
+                 std::vector<gpu::DevBuf>& keep, DevelopStage* st, Profiler* prof, const GrainRecord* rec,
+                 const Fork* fork) {
     auto mark = [&](const char* l) {
         if (prof && prof->on) prof->mark(l, s);
     };
@@ -672,12 +706,12 @@ void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* ne
     }
     float* acc = gpu::as<float>(accb);
     if (p.scatter_active) {
-        mixture(H, acc, p.scatter, p.n_scatter, w, h, p.fir_crossover, pool, s, keep);
+        mixture(H, acc, p.scatter, p.n_scatter, w, h, p.fir_crossover, pool, s, keep, fork);
         for (int c = 0; c < 3; ++c)
             k::lincomb(H + c * np, acc + c * np, H + c * np, uint32_t(np), num::f32(p.scatter_a[c]),
                        num::f32(p.scatter_b[c]), s);
     }
-    if (p.bounce_active) mixture(H, acc, p.bounce, p.n_bounce, w, h, p.fir_crossover, pool, s, keep);
+    if (p.bounce_active) mixture(H, acc, p.bounce, p.n_bounce, w, h, p.fir_crossover, pool, s, keep, fork);
     if (p.scatter_active || p.bounce_active) mark("halation");
 
     k::DevelopArgs da{};
@@ -864,9 +898,9 @@ size_t scratch_bound(const spk_plan& p, bool develop, bool tables, bool negative
 }
 
 void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
-             std::vector<gpu::DevBuf>& keep, Profiler* prof, const GrainRecord* rec) {
+             std::vector<gpu::DevBuf>& keep, Profiler* prof, const GrainRecord* rec, const Fork* fork) {
     DevelopStage st;
-    develop_pre(p, t, rgb, negative, pool, s, keep, &st, prof, rec);
+    develop_pre(p, t, rgb, negative, pool, s, keep, &st, prof, rec, fork);
     const k::DevelopJob& j = st.job;
     k::develop(j.H, j.acc, j.out, j.density, j.curves, j.gaxis, j.ginv, j.glayers, j.lgam, j.a, s);
     if (prof && prof->on) prof->mark("develop", s);

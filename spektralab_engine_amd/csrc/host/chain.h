@@ -67,8 +67,15 @@ struct GrainRecord {
 };
 
 // rgb: interleaved w x h x 3 on the device; negative: 3 planar planes (w*h*3 floats).
+// An auxiliary stream and two events the caller owns: independent stages of one render may run on
+// it, forked from the render's stream and joined back into it before their results are used.
+struct Fork {
+    hipStream_t aux = nullptr;
+    hipEvent_t forked = nullptr, joined = nullptr;
+};
 void develop(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
-             std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr, const GrainRecord* rec = nullptr);
+             std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr, const GrainRecord* rec = nullptr,
+             const Fork* fork = nullptr);
 // out: RGBA16 with a row pitch of `stride` pixels.
 void print(const spk_plan& p, const Tables& t, const float* negative, uint16_t* out, uint32_t stride, gpu::Pool& pool,
            hipStream_t s, std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr);
@@ -85,7 +92,7 @@ struct DevelopStage {
 };
 void develop_pre(const spk_plan& p, const Tables& t, const float* rgb, float* negative, gpu::Pool& pool, hipStream_t s,
                  std::vector<gpu::DevBuf>& keep, DevelopStage* st, Profiler* prof = nullptr,
-                 const GrainRecord* rec = nullptr);
+                 const GrainRecord* rec = nullptr, const Fork* fork = nullptr);
 // film_seed_dev (device, may be null) overrides the plan's film seed for the micro-structure field.
 void develop_post(const spk_plan& p, const DevelopStage& st, float* negative, gpu::Pool& pool, hipStream_t s,
                   std::vector<gpu::DevBuf>& keep, Profiler* prof = nullptr, const uint32_t* film_seed_dev = nullptr);
@@ -121,7 +128,8 @@ void blur3(const float* in, float* out, const double sigma[3], uint32_t mode, co
 // below the crossover; the radius at most k::kFirMaxR).
 // The mixture M(in) = sum_k w_k G_k(in), accumulated in list order into out (out != in).
 void mixture(const float* in, float* out, const spk_mix_component* comps, uint32_t n, uint32_t w, uint32_t h,
-             double crossover, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep);
+             double crossover, gpu::Pool& pool, hipStream_t s, std::vector<gpu::DevBuf>& keep,
+             const Fork* fork = nullptr);
 k::FirArgs fir_args_at(double sigma, double branch_sigma, uint32_t w, uint32_t h);
 
 // The Gaussian of nplanes contiguous w x h planes with the branch (identity, FIR of its radius, or

@@ -250,6 +250,9 @@ private:
     TierState tiers_[3];
     TierImage images_[3];
     std::unique_ptr<gpu::Stream> stream_;
+    // the auxiliary stream and events of the render's forked stages (chain::Fork)
+    std::unique_ptr<gpu::Stream> aux_;
+    hipEvent_t fork_ev_[2] = {nullptr, nullptr};
     std::unique_ptr<spk_plan> plan_;
     bool have_plan_ = false;
     chain::Profiler prof_;

@@ -102,6 +102,29 @@ def test_batch_autotune_and_fp16_frame(engine):
             s.close()
 
 
+def test_batch_autotune_without_fp16_takes_the_streams_schedule(engine):
+    """With no FP16-spectral frame the tuned schedule is "streams" (only that schedule is timed),
+    and every frame equals its own session's render, two runs in a row."""
+    fr = frames(4)
+    ss = open_all(engine, fr, CONFIG2)
+    refs = open_all(engine, fr, CONFIG2)
+    outs = device_outputs(fr)
+    try:
+        with spk.Batch(engine, ss, [o.data_ptr() for o in outs], [o.shape[1] for o in outs]) as b:
+            info = b.info()
+            assert info["config"]["schedule"] == "streams"
+            assert {t["schedule"] for t in info["tuning"]} == {"streams"} and len(info["tuning"]) >= 5
+            for run in range(2):
+                b.run()
+                b.sync()
+                for i, (img, _, _) in enumerate(fr):
+                    want, _ = refs[i].render("full")
+                    assert_close(host(outs[i], img.shape[1]), want, f"run {run} frame {i}")
+    finally:
+        for s in ss + refs:
+            s.close()
+
+
 @pytest.mark.skip(reason="the streams schedule inside an autotuning sequence, with an FP16-spectral frame, "
                          "ended in a host segmentation fault whose cause is not established; that combination "
                          "is switched off (not a tuning candidate; explicit streams + FP16 is refused)")
