@@ -827,10 +827,10 @@ constexpr uint32_t kColW = 16, kColChunks = 32;
 // the row to the chunk's last valid row (lanes past the frame edge read column w-1), so every load
 // stays inside the plane; stores use a wave-uniform row offset and only rows < n.  The mixture
 // accumulation (out = acc + w y, or 0 + w y) is fused into the store.
-template <int MC, bool kAdj, uint32_t CW>
-__global__ __launch_bounds__(512) void k_iir_col(const float* __restrict__ in, float* __restrict__ out,
-                                                 const float* __restrict__ accp, IirArgs q, uint32_t nplanes) {
-    __shared__ double V[512 * 3];
+template <int MC, bool kAdj, uint32_t CW, uint32_t NT = 512>
+__global__ __launch_bounds__(NT) void k_iir_col(const float* __restrict__ in, float* __restrict__ out,
+                                                const float* __restrict__ accp, IirArgs q, uint32_t nplanes) {
+    __shared__ double V[NT * 3];
     const uint32_t w = q.w, n = q.n, nch = q.nch;
     const uint32_t nstrips = (w + CW - 1) / CW;
     const uint32_t total = nstrips * nplanes;
@@ -1034,10 +1034,14 @@ void iir_col_launch(const float* in, float* out, const float* acc, const IirArgs
         }
         return;
     }
+    if (a.m == 64) {  // 16 columns x 64 chunks of 64: 64-byte row pieces a load instruction
+        const dim3 grid(xcd_grid(((a.w + kColW - 1) / kColW) * nplanes)), block(kColW, 2 * kColChunks);
+        hipLaunchKernelGGL((k_iir_col<64, kAdj, kColW, 1024>), grid, block, 0, s, in, out, acc, a, nplanes);
+        return;
+    }
     constexpr uint32_t CW = kColW / 2;
     const dim3 grid(xcd_grid(((a.w + CW - 1) / CW) * nplanes)), block(CW, 2 * kColChunks);
-    if (a.m == 64) hipLaunchKernelGGL((k_iir_col<64, kAdj, CW>), grid, block, 0, s, in, out, acc, a, nplanes);
-    else hipLaunchKernelGGL((k_iir_col<128, kAdj, CW>), grid, block, 0, s, in, out, acc, a, nplanes);
+    hipLaunchKernelGGL((k_iir_col<128, kAdj, CW>), grid, block, 0, s, in, out, acc, a, nplanes);
 }
 
 void iir_col(const float* in, float* out, const float* acc, const IirArgs& a, uint32_t nplanes, hipStream_t s) {
