@@ -94,9 +94,14 @@ __device__ __forceinline__ F3 edr_map(F3 v, const PrintConsts& q) {
 }
 
 // Pixel i (< n) of one frame: negative densities -> enlarger -> paper -> scan -> output RGB.
+// base_enl / base_scan: the two integrals' base densities (e.beta and q.scan_beta, or copies of
+// them in LDS: a bin's first FMA needs its base in a vector register, and an LDS broadcast read
+// puts it there without a vector-ALU move).
+template <class BaseE, class BaseS>
 __device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __restrict__ neg, float* __restrict__ rgb,
                                 CPC& q, CEA& e, const float* __restrict__ paper,
-                                const float* __restrict__ cmax, const float* __restrict__ glare) {
+                                const float* __restrict__ cmax, const float* __restrict__ glare,
+                                BaseE base_enl, BaseS base_scan) {
     float d0 = neg[i], d1 = neg[size_t(n) + i], d2 = neg[2 * size_t(n) + i];
 
     // enlarger: log10 exposure on the paper, then the print gain (a scanned film skips the
@@ -105,7 +110,7 @@ __device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __
         float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
 #pragma unroll 9
         for (uint32_t l = 0; l < SPK_NL; ++l) {
-            const float T = fexp2(fmaf(d2, e.eps[3 * l + 2], fmaf(d1, e.eps[3 * l + 1], fmaf(d0, e.eps[3 * l], e.beta[l]))));
+            const float T = fexp2(fmaf(d2, e.eps[3 * l + 2], fmaf(d1, e.eps[3 * l + 1], fmaf(d0, e.eps[3 * l], base_enl[l]))));
             r0 += T * e.wt[3 * l];
             r1 += T * e.wt[3 * l + 1];
             r2 += T * e.wt[3 * l + 2];
@@ -122,7 +127,7 @@ __device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __
 #pragma unroll 9
         for (uint32_t l = 0; l < SPK_NL; ++l) {
             const float T =
-                fexp2(fmaf(d2, q.scan_eps[3 * l + 2], fmaf(d1, q.scan_eps[3 * l + 1], fmaf(d0, q.scan_eps[3 * l], q.scan_beta[l]))));
+                fexp2(fmaf(d2, q.scan_eps[3 * l + 2], fmaf(d1, q.scan_eps[3 * l + 1], fmaf(d0, q.scan_eps[3 * l], base_scan[l]))));
             r0 += T * q.scan_wt[3 * l];
             r1 += T * q.scan_wt[3 * l + 1];
             r2 += T * q.scan_wt[3 * l + 2];
@@ -140,10 +145,15 @@ __global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, fl
                                                const PrintConsts* __restrict__ pcp, EnlargerArgs e,
                                                const float* __restrict__ paper, const float* __restrict__ cmax,
                                                const float* __restrict__ glare, uint32_t w, uint32_t h) {
+    __shared__ float s_base[2 * SPK_NL];  // the enlarger's and the scan's base densities
+    CPC& q = cref(*pcp);
+    CEA& ea = cref(e);
+    for (uint32_t t = threadIdx.x; t < 2 * SPK_NL; t += blockDim.x) s_base[t] = t < SPK_NL ? ea.beta[t] : q.scan_beta[t - SPK_NL];
+    __syncthreads();
     const uint32_t n = w * h;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    print_px(i, n, neg, rgb, cref(*pcp), cref(e), paper, cmax, glare);
+    print_px(i, n, neg, rgb, q, ea, paper, cmax, glare, s_base, s_base + SPK_NL);
 }
 
 // The frame of tile t: the last f with prefix[f] <= t (prefix ascending, prefix[0] = 0).
@@ -157,17 +167,23 @@ __device__ inline uint32_t frame_of_tile(const uint32_t* __restrict__ prefix, ui
     return lo;
 }
 
-// k_print over many frames: persistent workgroups stride over every frame's 256-pixel tiles.
+// k_print over many frames: persistent workgroups stride over every frame's 256-pixel tiles; each
+// tile's frame has its base densities staged in LDS, as k_print does.
 __global__ __launch_bounds__(256) void k_print_batch(const PrintJob* __restrict__ jobs,
                                                      const uint32_t* __restrict__ prefix, uint32_t nframes) {
+    __shared__ float s_base[2 * SPK_NL];
     const uint32_t total = prefix[nframes];
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
         const uint32_t f = frame_of_tile(prefix, nframes, t);
         const PrintJob SPK_CONST& j = ((const PrintJob SPK_CONST*)jobs)[f];
         const uint32_t n = j.w * j.h;
+        __syncthreads();  // the previous tile's reads of s_base are done
+        for (uint32_t u = threadIdx.x; u < 2 * SPK_NL; u += blockDim.x)
+            s_base[u] = u < SPK_NL ? j.e.beta[u] : j.pc->scan_beta[u - SPK_NL];
+        __syncthreads();
         for (uint32_t u = threadIdx.x; u < kBatchTile; u += blockDim.x) {
             const uint32_t i = (t - prefix[f]) * kBatchTile + u;
-            if (i < n) print_px(i, n, j.neg, j.rgb, *(CPC*)j.pc, j.e, j.paper, j.cmax, j.glare);
+            if (i < n) print_px(i, n, j.neg, j.rgb, *(CPC*)j.pc, j.e, j.paper, j.cmax, j.glare, s_base, s_base + SPK_NL);
         }
     }
 }
