@@ -291,7 +291,8 @@ void Diff::forward(int slot, const double* theta, const float* input, float* enc
                            pool, s, keep);
     }
     float* lin = plane(sl.lin, 3 * np);
-    k::print_pointwise(neg, lin, pc, ea, gpu::as<float>(t.paper), gpu::as<float>(t.cmax), glare, w, h, s);
+    float* rx = plane(sl.rx, 6 * np);  // R and the scanned XYZ: the reverse pass's linearisation point
+    k::print_pointwise(neg, lin, pc, ea, gpu::as<float>(t.paper), gpu::as<float>(t.cmax), glare, w, h, s, rx);
     const double one[3] = {1.0, 1.0, 1.0};
     if (lens_) {
         const double sb[3] = {p.scanner_lens_blur_px, p.scanner_lens_blur_px, p.scanner_lens_blur_px};
@@ -358,7 +359,8 @@ void Diff::backward(int slot, const float* input, const float* g_enc, float* g_i
     // print
     const k::EnlargerArgs ea = enlarger_args(th);
     float* gR = buf(3 * np);
-    k::adj_print_bwd(neg, gA, gpu::as<float>(sl.glare), gB, gR, pc, ea, gpu::as<float>(t.paper), gpu::as<float>(t.cmax),
+    k::adj_print_bwd(neg, gpu::as<const float>(sl.rx), gA, gpu::as<float>(sl.glare), gB, gR, pc, ea,
+                     gpu::as<float>(t.paper), gpu::as<float>(t.cmax),
                      n, part + rg.print, s);
     k::adj_wt(neg, gR, ea, n, part + rg.wt, s);
     std::swap(gA, gB);  // gA: g_neg
@@ -476,7 +478,7 @@ void Diff::sync() {
 std::string Diff::info() const {
     const spk_plan& p = *bf_.plan;
     const size_t np = size_t(bf_.w) * bf_.h;
-    size_t floats = 3 * np * 3 + (n_bounce_ ? 3 * np : 0) + (grain_ ? 9 * np : 0) + (unsharp_ ? 3 * np : 0) +
+    size_t floats = 3 * np * 3 + 6 * np + (n_bounce_ ? 3 * np : 0) + (grain_ ? 9 * np : 0) + (unsharp_ ? 3 * np : 0) +
                     (p.glare_active ? np : 0);
     std::ostringstream o;
     o << "{\"width\": " << bf_.w << ", \"height\": " << bf_.h << ", \"slots\": " << slots_.size()

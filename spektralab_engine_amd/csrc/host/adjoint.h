@@ -60,7 +60,7 @@ private:
     struct Slot {
         bool valid = false;
         std::vector<double> theta;
-        gpu::DevBuf H, acc, counts, neg, lin, ublur, glare;
+        gpu::DevBuf H, acc, counts, neg, lin, ublur, glare, rx;
     };
     Diff() = default;
     hipStream_t stream(hipStream_t s) const { return s ? s : own_->get(); }

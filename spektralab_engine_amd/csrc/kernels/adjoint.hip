@@ -109,30 +109,36 @@ __global__ __launch_bounds__(256) void k_adj_encode_bwd(const float* __restrict_
 }
 
 // ---- print: negative densities -> linear RGB, transposed ---------------------------------------
-// Recomputes the pixel's forward values, takes the gamut map's Jacobian by forward mode, and
+// Reads the pixel's enlarger response R and scanned XYZ from the forward pass's checkpoint rx,
+// recomputes the values between them, takes the gamut map's Jacobian by forward mode, and
 // carries g_lin back through the scan integral, the paper curve, the print gain, the log
 // exposure and the enlarger integral.  Writes g_neg and the enlarger response gradient g_R (for
 // the weights' gradient, k_adj_wt); accumulates k_gain and the paper maps.
-__global__ __launch_bounds__(256) void k_adj_print_bwd(const float* __restrict__ neg, const float* __restrict__ g_lin,
+__global__ __launch_bounds__(256) void k_adj_print_bwd(const float* __restrict__ neg, const float* __restrict__ rx,
+                                                       const float* __restrict__ g_lin,
                                                        const float* __restrict__ glare, float* __restrict__ g_neg,
                                                        float* __restrict__ g_R, const PrintConsts* __restrict__ pcp,
                                                        EnlargerArgs ea, const float* __restrict__ paper,
                                                        const float* __restrict__ cmax, uint32_t n,
                                                        double* __restrict__ part) {
+    // the two integrals' base densities in LDS: each bin's density is one FMA chain from its base,
+    // which a broadcast read puts in a vector register (as k_print does)
+    __shared__ float s_base[2 * SPK_NL];
+    {
+        CPC& q = cref(*pcp);
+        CEA& e = cref(ea);
+        for (uint32_t t = threadIdx.x; t < 2 * SPK_NL; t += blockDim.x) s_base[t] = t < SPK_NL ? e.beta[t] : q.scan_beta[t - SPK_NL];
+    }
+    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float g_kg = 0.0f, g_pg[3] = {0.0f, 0.0f, 0.0f}, g_po[3] = {0.0f, 0.0f, 0.0f};
     if (i < n) {
         CPC& q = cref(*pcp);
         CEA& e = cref(ea);
         const float d[3] = {neg[i], neg[size_t(n) + i], neg[2 * size_t(n) + i]};
-        // enlarger integral
-        float R[3] = {0.0f, 0.0f, 0.0f};
-        for (uint32_t l = 0; l < SPK_NL; ++l) {
-            const float T = fexp2(((d[0] * e.eps[3 * l] + d[1] * e.eps[3 * l + 1]) + d[2] * e.eps[3 * l + 2]) + e.beta[l]);
-            R[0] += T * e.wt[3 * l];
-            R[1] += T * e.wt[3 * l + 1];
-            R[2] += T * e.wt[3 * l + 2];
-        }
+        // the forward pass's enlarger integral and scan integral
+        const float R[3] = {rx[i], rx[size_t(n) + i], rx[2 * size_t(n) + i]};
+        const float X[3] = {rx[3 * size_t(n) + i], rx[4 * size_t(n) + i], rx[5 * size_t(n) + i]};
         float y[3], lg[3], z[3], lp[3], slope[3], Dp[3];
         for (int c = 0; c < 3; ++c) {
             y[c] = R[c] * e.g[c] + e.o[c];
@@ -140,15 +146,6 @@ __global__ __launch_bounds__(256) void k_adj_print_bwd(const float* __restrict__
             z[c] = fexp2(lg[c] * kLog2Of10) * e.k_gain;
             lp[c] = flog10(fmaxf(z[c], 0.0f) + kTiny);
             Dp[c] = paper_lookup(paper, q, uint32_t(c), e.pg[c] * lp[c] + e.po[c], &slope[c]);
-        }
-        // scan integral
-        float X[3] = {0.0f, 0.0f, 0.0f};
-        for (uint32_t l = 0; l < SPK_NL; ++l) {
-            const float T = fexp2(((Dp[0] * q.scan_eps[3 * l] + Dp[1] * q.scan_eps[3 * l + 1]) + Dp[2] * q.scan_eps[3 * l + 2]) +
-                                  q.scan_beta[l]);
-            X[0] += T * q.scan_wt[3 * l];
-            X[1] += T * q.scan_wt[3 * l + 1];
-            X[2] += T * q.scan_wt[3 * l + 2];
         }
         S3<D3> xyz{D3(fmaxf(X[0], 0.0f) + kTiny, 1.0f, 0.0f, 0.0f), D3(fmaxf(X[1], 0.0f) + kTiny, 0.0f, 1.0f, 0.0f),
                    D3(fmaxf(X[2], 0.0f) + kTiny, 0.0f, 0.0f, 1.0f)};
@@ -167,8 +164,8 @@ __global__ __launch_bounds__(256) void k_adj_print_bwd(const float* __restrict__
         // scan integral, transposed: dX_c/dDp_k = ln2 sum_l seps_k(l) T(l) swt_c(l)
         float gD[3] = {0.0f, 0.0f, 0.0f};
         for (uint32_t l = 0; l < SPK_NL; ++l) {
-            const float T = fexp2(((Dp[0] * q.scan_eps[3 * l] + Dp[1] * q.scan_eps[3 * l + 1]) + Dp[2] * q.scan_eps[3 * l + 2]) +
-                                  q.scan_beta[l]);
+            const float T = fexp2(fmaf(Dp[2], q.scan_eps[3 * l + 2],
+                                       fmaf(Dp[1], q.scan_eps[3 * l + 1], fmaf(Dp[0], q.scan_eps[3 * l], s_base[SPK_NL + l]))));
             const float u = ((gX[0] * q.scan_wt[3 * l] + gX[1] * q.scan_wt[3 * l + 1]) + gX[2] * q.scan_wt[3 * l + 2]) * T;
             gD[0] += u * q.scan_eps[3 * l];
             gD[1] += u * q.scan_eps[3 * l + 1];
@@ -189,7 +186,7 @@ __global__ __launch_bounds__(256) void k_adj_print_bwd(const float* __restrict__
         // enlarger integral, transposed
         float gd[3] = {0.0f, 0.0f, 0.0f};
         for (uint32_t l = 0; l < SPK_NL; ++l) {
-            const float T = fexp2(((d[0] * e.eps[3 * l] + d[1] * e.eps[3 * l + 1]) + d[2] * e.eps[3 * l + 2]) + e.beta[l]);
+            const float T = fexp2(fmaf(d[2], e.eps[3 * l + 2], fmaf(d[1], e.eps[3 * l + 1], fmaf(d[0], e.eps[3 * l], s_base[l]))));
             const float u = ((gRr[0] * e.wt[3 * l] + gRr[1] * e.wt[3 * l + 1]) + gRr[2] * e.wt[3 * l + 2]) * T;
             gd[0] += u * e.eps[3 * l];
             gd[1] += u * e.eps[3 * l + 1];
@@ -584,10 +581,10 @@ void adj_encode_bwd(const float* lin, const float* blur, const float* g_enc, flo
                        n, part);
 }
 
-void adj_print_bwd(const float* neg, const float* g_lin, const float* glare, float* g_neg, float* g_R,
+void adj_print_bwd(const float* neg, const float* rx, const float* g_lin, const float* glare, float* g_neg, float* g_R,
                    const PrintConsts* pc, const EnlargerArgs& ea, const float* paper, const float* cmax, uint32_t n,
                    double* part, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_print_bwd, dim3(blocks_of(n)), dim3(kAdjBlock), 0, s, neg, g_lin, glare, g_neg, g_R, pc,
+    hipLaunchKernelGGL(k_adj_print_bwd, dim3(blocks_of(n)), dim3(kAdjBlock), 0, s, neg, rx, g_lin, glare, g_neg, g_R, pc,
                        ea, paper, cmax, n, part);
 }
 

@@ -254,9 +254,10 @@ void develop(const float* H, const float* acc, float* out, float* density, const
 // may be null, overrides seed); then G := G phi + shift_c over 3 planes of n (phi null: G + shift_c).
 void micro_field(float* phi, uint32_t n, float sigma, uint32_t seed, const uint32_t* seed_ptr, hipStream_t s);
 void grain_finish(float* G, const float* phi, uint32_t n, const float shift[3], hipStream_t s);
-// glare: the blurred glare field (one plane) or nullptr.
+// glare: the blurred glare field (one plane) or nullptr.  rx (6 planes, may be nullptr): the
+// enlarger response R and the scanned XYZ before clamping, per channel, for the reverse pass.
 void print_pointwise(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea, const float* paper,
-                     const float* cmax, const float* glare, uint32_t w, uint32_t h, hipStream_t s);
+                     const float* cmax, const float* glare, uint32_t w, uint32_t h, hipStream_t s, float* rx = nullptr);
 // The same with both spectral integrals on the matrix cores (enl: enlarger, scan: scan operands).
 void print_pointwise_fp16(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea,
                           const SpecMfma* enl, const SpecMfma* scan, const float* paper, const float* cmax,
@@ -339,8 +340,9 @@ void adj_encode_fwd(const float* lin, const float* blur, float* enc, const Print
 // g_v (3 planes) from g_enc (interleaved).  Partials (1): d/d unsharp amount.
 void adj_encode_bwd(const float* lin, const float* blur, const float* g_enc, float* g_v, const PrintConsts* pc, float ua,
                     float ub, uint32_t n, double* part, hipStream_t s);
-// g_neg, g_R (3 planes each) from g_lin.  Partials (7): k_gain, paper gamma 0-2, paper offset 0-2.
-void adj_print_bwd(const float* neg, const float* g_lin, const float* glare, float* g_neg, float* g_R,
+// g_neg, g_R (3 planes each) from g_lin, at the forward values rx (print_pointwise's 6 planes).
+// Partials (7): k_gain, paper gamma 0-2, paper offset 0-2.
+void adj_print_bwd(const float* neg, const float* rx, const float* g_lin, const float* glare, float* g_neg, float* g_R,
                    const PrintConsts* pc, const EnlargerArgs& ea, const float* paper, const float* cmax, uint32_t n,
                    double* part, hipStream_t s);
 // Partials (243): sum_px g_R_c T_l at [3 l + c].

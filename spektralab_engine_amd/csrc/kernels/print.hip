@@ -101,7 +101,7 @@ template <class BaseE, class BaseS>
 __device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __restrict__ neg, float* __restrict__ rgb,
                                 CPC& q, CEA& e, const float* __restrict__ paper,
                                 const float* __restrict__ cmax, const float* __restrict__ glare,
-                                BaseE base_enl, BaseS base_scan) {
+                                BaseE base_enl, BaseS base_scan, float* __restrict__ rx = nullptr) {
     float d0 = neg[i], d1 = neg[size_t(n) + i], d2 = neg[2 * size_t(n) + i];
 
     // enlarger: log10 exposure on the paper, then the print gain (a scanned film skips the
@@ -114,6 +114,11 @@ __device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __
             r0 += T * e.wt[3 * l];
             r1 += T * e.wt[3 * l + 1];
             r2 += T * e.wt[3 * l + 2];
+        }
+        if (rx) {
+            rx[i] = r0;
+            rx[size_t(n) + i] = r1;
+            rx[2 * size_t(n) + i] = r2;
         }
         const F3 d = expose_paper(r0, r1, r2, e, paper, q);
         d0 = d.x;
@@ -132,6 +137,11 @@ __device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __
             r1 += T * q.scan_wt[3 * l + 1];
             r2 += T * q.scan_wt[3 * l + 2];
         }
+        if (rx) {
+            rx[3 * size_t(n) + i] = r0;
+            rx[4 * size_t(n) + i] = r1;
+            rx[5 * size_t(n) + i] = r2;
+        }
         xyz = {fmaxf(r0, 0.0f) + kTiny, fmaxf(r1, 0.0f) + kTiny, fmaxf(r2, 0.0f) + kTiny};
     }
     xyz = levels_glare(xyz, e, glare, i);
@@ -144,7 +154,8 @@ __device__ __forceinline__ void print_px(uint32_t i, uint32_t n, const float* __
 __global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, float* __restrict__ rgb,
                                                const PrintConsts* __restrict__ pcp, EnlargerArgs e,
                                                const float* __restrict__ paper, const float* __restrict__ cmax,
-                                               const float* __restrict__ glare, uint32_t w, uint32_t h) {
+                                               const float* __restrict__ glare, uint32_t w, uint32_t h,
+                                               float* __restrict__ rx) {
     __shared__ float s_base[2 * SPK_NL];  // the enlarger's and the scan's base densities
     CPC& q = cref(*pcp);
     CEA& ea = cref(e);
@@ -153,7 +164,7 @@ __global__ __launch_bounds__(256) void k_print(const float* __restrict__ neg, fl
     const uint32_t n = w * h;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    print_px(i, n, neg, rgb, q, ea, paper, cmax, glare, s_base, s_base + SPK_NL);
+    print_px(i, n, neg, rgb, q, ea, paper, cmax, glare, s_base, s_base + SPK_NL, rx);
 }
 
 // The frame of tile t: the last f with prefix[f] <= t (prefix ascending, prefix[0] = 0).
@@ -333,9 +344,9 @@ __global__ __launch_bounds__(kUNT) void k_unsharp_encode2d(const float* __restri
 }  // namespace
 
 void print_pointwise(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea, const float* paper,
-                     const float* cmax, const float* glare, uint32_t w, uint32_t h, hipStream_t s) {
+                     const float* cmax, const float* glare, uint32_t w, uint32_t h, hipStream_t s, float* rx) {
     const uint32_t n = w * h;
-    hipLaunchKernelGGL(k_print, dim3((n + 255) / 256), dim3(256), 0, s, neg, rgb, pc, ea, paper, cmax, glare, w, h);
+    hipLaunchKernelGGL(k_print, dim3((n + 255) / 256), dim3(256), 0, s, neg, rgb, pc, ea, paper, cmax, glare, w, h, rx);
 }
 
 void print_pointwise_fp16(const float* neg, float* rgb, const PrintConsts* pc, const EnlargerArgs& ea,
