@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the render chain on one MI355X.
 
-Workload (the benchmark's configuration 2): one 6000 x 4000 float32 linear ProPhoto frame of the
+Workload (the single-frame chain): one 6000 x 4000 float32 linear ProPhoto frame of the
 synthetic generator (seed 0) on the synthetic 3-layer stock; Philox Poisson grain (film format
 35 mm); halation = one mixture of 3 Gaussians sigma 2/8/32 px, weights 0.6/0.3/0.1, scatter off;
 FIR for sigma <= 16 (crossover 16.5), recursive above; print, scan, CAM16 gamut, unsharp, RGBA16.
@@ -14,9 +14,9 @@ dominant kernel, the same kernel against the vector-ALU issue rate that binds it
 its VALU instructions per wavefront from the hardware counters in profiles/, the FMA issue rate
 measured live by spkx_valu_roofs), and a CPU baseline: the oracle (oracle/chain.cpp, binary32) timed on a bounded
 sample (a 1536 x 1024 crop of the same workload) on this host's cores.  It also carries the
-batched workload (configuration 3, "batched": 64 frames of mixed sizes and per-frame parameters,
+batched workload ("batched": 64 frames of mixed sizes and per-frame parameters,
 one hipGraph launch, against 64 sequential spk_render and spkx_render_device calls).
-`--study precision` runs configuration 4 instead (a reprint sweep, FP16 vs FP32 spectral
+`--study precision` runs a reprint sweep instead (one developed negative reprinted, FP16 vs FP32 spectral
 integrals against the float64 oracle).
 """
 from __future__ import annotations
@@ -76,7 +76,7 @@ def main():
     ap.add_argument("--study", choices=["precision", "diff"], default=None,
                     help="precision: one developed negative, a reprint sweep of print_exposure 0.5..2.0 in FP32 and "
                          "FP16, each against the float64 oracle (16-bit LSB and CAM16-UCS dE); diff: only the "
-                         "differentiable-render measurement (configuration 5), one JSON line")
+                         "differentiable-render measurement (8 frames of 2048x1365), one JSON line")
     ap.add_argument("--sweep", type=int, default=32, help="reprints in the precision study")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-rendering measurement")
     ap.add_argument("--batch-frames", type=int, default=64)
@@ -221,7 +221,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"config 2: plain chain {W}x{H}, Poisson grain + 3-Gaussian halation (2/8/32 px) "
+        "config": {"workload": f"plain chain {W}x{H}, Poisson grain + 3-Gaussian halation (2/8/32 px) "
                                f"+ print/scan + CAM16 + unsharp -> RGBA16, policy {args.policy}",
                    "width": W, "height": H, "fir_crossover": crossover, "film_format_mm": 35.0,
                    "stock": "synthetic 3-layer negative / synthetic paper"},
@@ -296,7 +296,7 @@ def cpu_baseline(res, delta, overrides, frame, args):
 
 
 def batch_measure(eng, res, args):
-    """The benchmark's configuration 3: `batch_frames` frames of log-uniform area 256x256 ..
+    """The batched workload: `batch_frames` frames of log-uniform area 256x256 ..
     4096x2730 with mixed aspect ratios and per-frame print exposure, filter shifts and halation
     sigmas (synth.batch_workload), each in its own open session.  Times three ways of rendering
     every frame once, inputs resident on the device:
@@ -365,7 +365,7 @@ def batch_measure(eng, res, args):
     log(f"[bench] batch: spk_render x{len(wl)} {t_render:.1f} ms, render_device x{len(wl)} {t_device:.1f} ms, "
         f"batch {t_batch:.1f} ms (create + tune {t_create:.2f} s)")
     n = len(wl)
-    return {"workload": f"config 3: {n} frames, {px / 1e6:.1f} MP total, log-uniform area 256x256..4096x2730, "
+    return {"workload": f"batch of {n} frames, {px / 1e6:.1f} MP total, log-uniform area 256x256..4096x2730, "
                         "per-frame print_exposure/filters/halation sigmas",
             "frames": n, "pixels": px,
             "frames_per_s": round(n / (t_batch / 1000.0), 1), "mpx_s": round(px / (t_batch / 1000.0) / 1e6, 1),
@@ -379,7 +379,7 @@ def batch_measure(eng, res, args):
 
 
 def diff_measure(eng, res, args, W=2048, H=1365):
-    """The benchmark's configuration 5: `diff_frames` frames of 2048x1365 through the differentiable
+    """The differentiable-render workload: `diff_frames` frames of 2048x1365 through the differentiable
     render (grain frozen to each forward pass's counts, the chain workload's halation), an L2 loss
     against targets rendered with the 40 scalar quantities and the enlarger weights perturbed by
     N(0, 0.05) relative.  Times, inputs resident on the device: the differentiable forward pass of
@@ -446,7 +446,7 @@ def diff_measure(eng, res, args, W=2048, H=1365):
         f"plain render {t_p:.1f} ms")
     d.close()
     sess.close()
-    return {"workload": f"config 5: {n} frames {W}x{H}, grain frozen, 3-Gaussian halation, L2 loss vs targets at "
+    return {"workload": f"differentiable render of {n} frames {W}x{H}, grain frozen, 3-Gaussian halation, L2 loss vs targets at "
                         "quantities perturbed N(0, 0.05) relative; gradients w.r.t. 39 scalars, 243 enlarger weights "
                         "and the input pixels",
             "frames": n, "pixels": px, "quantities": d.QUANTITIES,
@@ -457,7 +457,7 @@ def diff_measure(eng, res, args, W=2048, H=1365):
 
 
 def precision_study(args):
-    """The benchmark's configuration 4: one developed W x H negative of the chain workload, then
+    """The reprint study: one developed W x H negative of the chain workload, then
     `sweep` spk_reprint calls at print_exposure 0.5..2.0, each in FP32 and in FP16 spectral
     precision.  Every reprint is compared with the float64 oracle's print of the same negative on a
     central crop (the print is pointwise; the unsharp blur's 2-pixel reach is excluded at the crop
@@ -536,7 +536,7 @@ def precision_study(args):
             "reprint_ms": round(float(np.median(times[mode])), 4),
             "reprint_mpx_s": round(W * H / (np.median(times[mode]) / 1000.0) / 1e6, 1)}
     line = {"study": "precision", "config": {
-        "workload": f"config 4: one developed {W}x{H} negative of the chain workload, {len(exposures)} reprints "
+        "workload": f"one developed {W}x{H} negative of the chain workload, {len(exposures)} reprints "
                     f"at print_exposure 0.5..2.0, FP32 vs FP16 (matrix cores, FP32 accumulation) spectral integrals",
         "reference": f"float64 oracle print of the same negative on the central {cw}x{ch} crop, {m}-px border "
                      "excluded", "pixels_compared": int(hist["fp32"].sum() // 3)},

@@ -47,9 +47,9 @@ def delta(**kw):
     return d
 
 
-CONFIG1 = delta(grain_active=False, halation_active=False)
-CONFIG2 = delta(grain_active=True, halation_active=True)
-CONFIG2_OVERRIDES = {"halation_mixture": [[2.0, 0.6], [8.0, 0.3], [32.0, 0.1]], "halation_scatter": False}
+PLAIN = delta(grain_active=False, halation_active=False)
+GRAIN_HALATION = delta(grain_active=True, halation_active=True)
+MIX3_OVERRIDES = {"halation_mixture": [[2.0, 0.6], [8.0, 0.3], [32.0, 0.1]], "halation_scatter": False}
 
 
 @pytest.fixture(scope="session")

@@ -21,17 +21,17 @@ sys.path.insert(0, str(HERE.parent))
 
 import spektralab_engine_amd as spk  # noqa: E402
 from spektralab_engine_amd import synth  # noqa: E402
-from conftest import CONFIG1, CONFIG2, CONFIG2_OVERRIDES, delta  # noqa: E402
+from conftest import PLAIN, GRAIN_HALATION, MIX3_OVERRIDES, delta  # noqa: E402
 
 SEED = 0x4ABEA221  # the first film seed of a fresh pipeline
 
 # name -> (delta, overrides, width, height)
 CASES = {
-    "config1_512x512": (CONFIG1, None, 512, 512),
-    "config2_600x400": (CONFIG2, CONFIG2_OVERRIDES, 600, 400),
-    "config2_6000x4000": (dict(CONFIG2, film_format_mm=35.0), dict(CONFIG2_OVERRIDES, fir_crossover=16.5), 6000, 4000),
-    "config5_2048x1365": (delta(grain_active=True, halation_active=True, film_format_mm=35.0),
-                          dict(CONFIG2_OVERRIDES, fir_crossover=16.5), 2048, 1365),
+    "plain_512x512": (PLAIN, None, 512, 512),
+    "grain_halation_600x400": (GRAIN_HALATION, MIX3_OVERRIDES, 600, 400),
+    "grain_halation_6000x4000": (dict(GRAIN_HALATION, film_format_mm=35.0), dict(MIX3_OVERRIDES, fir_crossover=16.5), 6000, 4000),
+    "diff_2048x1365": (delta(grain_active=True, halation_active=True, film_format_mm=35.0),
+                          dict(MIX3_OVERRIDES, fir_crossover=16.5), 2048, 1365),
 }
 
 

@@ -12,7 +12,7 @@ import pytest
 import torch
 
 import spektralab_engine_amd as spk
-from conftest import CONFIG1, CONFIG2, CONFIG2_OVERRIDES
+from conftest import PLAIN, GRAIN_HALATION, MIX3_OVERRIDES
 from spektralab_engine_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -35,7 +35,7 @@ def open_all(engine, fr, delta, fp16_index=None):
         d = dict(delta)
         d.update(params)
         s = engine.open(img, d)
-        ov = dict(CONFIG2_OVERRIDES, halation_mixture=mix, fir_crossover=16.5)
+        ov = dict(MIX3_OVERRIDES, halation_mixture=mix, fir_crossover=16.5)
         if i == fp16_index:
             ov["spectral_precision"] = "fp16"
         s.set_overrides(ov)
@@ -57,7 +57,7 @@ def assert_close(a, b, what):
 
 
 @pytest.mark.parametrize("schedule", ["phased", "streams"])
-@pytest.mark.parametrize("delta", [CONFIG1, CONFIG2], ids=["grain_off", "grain_on"])
+@pytest.mark.parametrize("delta", [PLAIN, GRAIN_HALATION], ids=["grain_off", "grain_on"])
 def test_batch_matches_per_session_renders(engine, delta, schedule):
     fr = frames()
     batch_sessions = open_all(engine, fr, delta)
@@ -80,8 +80,8 @@ def test_batch_matches_per_session_renders(engine, delta, schedule):
 
 def test_batch_autotune_and_fp16_frame(engine):
     fr = frames(4)
-    ss = open_all(engine, fr, CONFIG1, fp16_index=1)
-    refs = open_all(engine, fr, CONFIG1, fp16_index=1)
+    ss = open_all(engine, fr, PLAIN, fp16_index=1)
+    refs = open_all(engine, fr, PLAIN, fp16_index=1)
     outs = device_outputs(fr)
     try:
         with spk.Batch(engine, ss, [o.data_ptr() for o in outs], [o.shape[1] for o in outs]) as b:
@@ -106,8 +106,8 @@ def test_batch_autotune_without_fp16_takes_the_streams_schedule(engine):
     """With no FP16-spectral frame the tuned schedule is "streams" (only that schedule is timed),
     and every frame equals its own session's render, two runs in a row."""
     fr = frames(4)
-    ss = open_all(engine, fr, CONFIG2)
-    refs = open_all(engine, fr, CONFIG2)
+    ss = open_all(engine, fr, GRAIN_HALATION)
+    refs = open_all(engine, fr, GRAIN_HALATION)
     outs = device_outputs(fr)
     try:
         with spk.Batch(engine, ss, [o.data_ptr() for o in outs], [o.shape[1] for o in outs]) as b:
@@ -134,7 +134,7 @@ def test_batch_autotune_over_both_schedules_with_fp16_frame(engine):
 
 def test_batch_streams_schedule_refuses_fp16_sessions(engine):
     fr = frames(2)
-    ss = open_all(engine, fr, CONFIG1, fp16_index=0)
+    ss = open_all(engine, fr, PLAIN, fp16_index=0)
     outs = device_outputs(fr)
     try:
         with pytest.raises(spk.SpkError) as e:
@@ -148,7 +148,7 @@ def test_batch_streams_schedule_refuses_fp16_sessions(engine):
 
 def test_batch_rejects_bad_arguments(engine):
     img = synth.make_frame(32, 32, 1)
-    with engine.open(img, CONFIG1) as s:
+    with engine.open(img, PLAIN) as s:
         out = torch.zeros((32, 32, 4), dtype=torch.int16, device="cuda")
         with pytest.raises(spk.SpkError):
             spk.Batch(engine, [s], [out.data_ptr()], [16])  # stride below the width
@@ -166,7 +166,7 @@ GRAIN_MODELS = [((512, 384), dict(grain_sublayers_active=False)), ((1024, 64), {
 def test_batch_grain_models_match_per_session_renders(engine, schedule):
     """The batch's dye-cloud planes, micro-structure field (its seed read from the batch's device
     seed table) and single-layer draws equal each frame's own session render, two runs in a row."""
-    base = dict(CONFIG1, grain_active=True, film_format_mm=4.0)
+    base = dict(PLAIN, grain_active=True, film_format_mm=4.0)
     fr, deltas = [], []
     for i, ((w, h), extra) in enumerate(GRAIN_MODELS):
         fr.append((synth.make_frame(w, h, 90 + i), {}, None))

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import spektralab_engine_amd as spk
-from conftest import CONFIG2, CONFIG2_OVERRIDES
+from conftest import GRAIN_HALATION, MIX3_OVERRIDES
 from spektralab_engine_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -74,19 +74,19 @@ def check_frame(oracle, name, plan, img, out, lam_g, cnt_g):
 
 
 @pytest.mark.timeout(600)
-def test_config2_full_size_end_to_end(engine, oracle):
+def test_grain_halation_6000x4000_end_to_end(engine, oracle):
     w, h = 6000, 4000
-    print(f"config 2 {w}x{h}: render", flush=True)
+    print(f"grain+halation {w}x{h}: render", flush=True)
     img = synth.make_frame(w, h, 0)
-    d = dict(CONFIG2, film_format_mm=35.0)
+    d = dict(GRAIN_HALATION, film_format_mm=35.0)
     with engine.open(img, d) as s:
-        s.set_overrides(dict(CONFIG2_OVERRIDES, fir_crossover=16.5, record_grain=True))
+        s.set_overrides(dict(MIX3_OVERRIDES, fir_crossover=16.5, record_grain=True))
         out, _ = s.render("full")
         plan = s.last_plan()
         lam_g, cnt_g = s.grain_record("full")
     assert out.shape == (h, w, 4) and np.all(out[..., 3] == 65535)
-    print("config 2: oracle", flush=True)
-    check_frame(oracle, "config2_6000x4000", plan, img, out, lam_g, cnt_g)
+    print("grain+halation: oracle", flush=True)
+    check_frame(oracle, "grain_halation_6000x4000", plan, img, out, lam_g, cnt_g)
 
 
 # Frames spanning the batched workload's sizes (log-uniform areas 256x256 .. 4096x2730, both
@@ -102,7 +102,7 @@ def test_batch_matches_oracle_per_frame(engine, oracle, resources):
     sessions, outs, imgs, deltas, overrides = [], [], [], [], []
     try:
         for i, (w, h) in enumerate(BATCH):
-            params = dict(CONFIG2, film_format_mm=35.0, print_exposure=float(rng.uniform(0.5, 2.0)),
+            params = dict(GRAIN_HALATION, film_format_mm=35.0, print_exposure=float(rng.uniform(0.5, 2.0)),
                           m_filter_shift=float(rng.uniform(-0.3, 0.3)), y_filter_shift=float(rng.uniform(-0.3, 0.3)))
             mixture = [[sg * float(rng.uniform(0.5, 2.0)), wt] for sg, wt in ((2.0, 0.6), (8.0, 0.3), (32.0, 0.1))]
             ov = {"halation_mixture": mixture, "halation_scatter": False, "fir_crossover": 16.5 if i % 2 else 3.0}

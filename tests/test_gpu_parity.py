@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import spektralab_engine_amd as spk
-from conftest import CONFIG1, CONFIG2, CONFIG2_OVERRIDES
+from conftest import PLAIN, GRAIN_HALATION, MIX3_OVERRIDES
 from spektralab_engine_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -139,8 +139,8 @@ def render_and_compare(engine, oracle, delta, w, h, overrides=None, seed=0):
     return img, plan, out, neg, neg_o, out_o
 
 
-def test_config1_render_matches_oracle(engine, oracle):
-    img, plan, out, neg, neg_o, out_o = render_and_compare(engine, oracle, CONFIG1, 512, 512)
+def test_plain_render_matches_oracle(engine, oracle):
+    img, plan, out, neg, neg_o, out_o = render_and_compare(engine, oracle, PLAIN, 512, 512)
     assert out.shape == (512, 512, 4) and np.all(out[..., 3] == 65535)
     assert np.array_equal(neg, neg_o), f"negative differs at {np.count_nonzero(neg != neg_o)} values"
     d = np.abs(out.astype(np.int32) - out_o.astype(np.int32))
@@ -148,9 +148,9 @@ def test_config1_render_matches_oracle(engine, oracle):
 
 
 @pytest.mark.parametrize("crossover", [3.0, 16.5])
-def test_config2_small_matches_oracle(engine, oracle, crossover):
-    ov = dict(CONFIG2_OVERRIDES, fir_crossover=crossover)
-    img, plan, out, neg, neg_o, out_o = render_and_compare(engine, oracle, CONFIG2, 600, 400, ov, seed=1)
+def test_grain_halation_small_matches_oracle(engine, oracle, crossover):
+    ov = dict(MIX3_OVERRIDES, fir_crossover=crossover)
+    img, plan, out, neg, neg_o, out_o = render_and_compare(engine, oracle, GRAIN_HALATION, 600, 400, ov, seed=1)
     assert oracle.field(plan, "grain_active")[0] == 1
     assert oracle.field(plan, "grain_seed")[0] == 0x4ABEA221  # the session's first film seed
     diff = np.abs(neg.astype(np.float64) - neg_o)
@@ -168,7 +168,7 @@ def test_config2_small_matches_oracle(engine, oracle, crossover):
 
 def test_reprint_reuses_negative_and_live_params(engine):
     img = synth.make_frame(256, 192, 3)
-    with engine.open(img, CONFIG2) as s:
+    with engine.open(img, GRAIN_HALATION) as s:
         a, fa = s.render("full")
         b, fb = s.render("full", reprint=True)
         assert not fa["reprint"] and fb["reprint"] and fb["negative_was_cached"]
@@ -182,7 +182,7 @@ def test_reprint_reuses_negative_and_live_params(engine):
 
 def test_grain_off_render_is_deterministic(engine, resources):
     img = synth.make_frame(200, 160, 4)
-    with engine.open(img, CONFIG1) as s1, engine.open(img, CONFIG1) as s2:
+    with engine.open(img, PLAIN) as s1, engine.open(img, PLAIN) as s2:
         a, _ = s1.render("full")
         b, _ = s2.render("full")
         c, _ = s1.render("full")
@@ -192,7 +192,7 @@ def test_grain_off_render_is_deterministic(engine, resources):
 def test_render_device_matches_host_render(engine):
     import torch
     img = synth.make_frame(320, 240, 5)
-    with engine.open(img, CONFIG1) as s:
+    with engine.open(img, PLAIN) as s:
         host, _ = s.render("full")
         buf = torch.zeros((240, 336, 4), dtype=torch.int16, device="cuda")
         w, h = s.render_device(buf.data_ptr(), 336)
@@ -203,7 +203,7 @@ def test_render_device_matches_host_render(engine):
 
 def test_errors_map_to_statuses(engine):
     img = synth.make_frame(64, 48, 6)
-    with engine.open(img, CONFIG1) as s:
+    with engine.open(img, PLAIN) as s:
         with pytest.raises(spk.SpkError) as e:
             s.render("bogus")
         assert e.value.status == -1

@@ -9,7 +9,7 @@ a print-side choice).
 import numpy as np
 import pytest
 
-from conftest import CONFIG1, CONFIG2, CONFIG2_OVERRIDES
+from conftest import PLAIN, GRAIN_HALATION, MIX3_OVERRIDES
 from spektralab_engine_amd import colour, synth
 
 pytestmark = pytest.mark.gpu
@@ -22,13 +22,13 @@ def errors(out, ref):
     return s
 
 
-@pytest.mark.parametrize("cfg,w,h,seed", [("c1", 512, 512, 0), ("c2", 640, 427, 5)])
+@pytest.mark.parametrize("cfg,w,h,seed", [("plain", 512, 512, 0), ("grain_halation", 640, 427, 5)])
 def test_fp16_spectral_against_float64(engine, oracle, cfg, w, h, seed):
     img = synth.make_frame(w, h, seed)
-    dl = CONFIG1 if cfg == "c1" else CONFIG2
+    dl = PLAIN if cfg == "plain" else GRAIN_HALATION
     with engine.open(img, dl) as s:
-        if cfg == "c2":
-            s.set_overrides(CONFIG2_OVERRIDES)
+        if cfg == "grain_halation":
+            s.set_overrides(MIX3_OVERRIDES)
         o32, _ = s.render("full")
         plan32 = s.last_plan()
         neg = s.negative("full")
@@ -51,7 +51,7 @@ def test_fp16_matches_fp32_on_a_flat_field(engine):
     """A uniform frame prints to a uniform image in both precisions (the matrix steps mix 64 pixels
     per wavefront: a lane-layout slip would show as spatial structure), and both agree."""
     img = np.full((70, 130, 3), 0.18, np.float32)
-    with engine.open(img, CONFIG1) as s:
+    with engine.open(img, PLAIN) as s:
         s.set_overrides({"spectral_precision": "fp16"})
         o16, _ = s.render("full")
         s.set_overrides({"spectral_precision": "fp32"})
@@ -67,7 +67,7 @@ def test_fp16_pixel_order(engine, oracle):
     w, h = 97, 61
     ramp = np.exp2(np.linspace(-6.0, 3.0, w * h)).reshape(h, w)
     img = np.stack([ramp, ramp[::-1, ::-1], np.roll(ramp, 7, axis=1)], axis=-1).astype(np.float32)
-    with engine.open(img, CONFIG1) as s:
+    with engine.open(img, PLAIN) as s:
         o32, _ = s.render("full")
         s.set_overrides({"spectral_precision": "fp16"})
         o16, _ = s.render("full", reprint=True)

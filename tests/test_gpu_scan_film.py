@@ -8,25 +8,25 @@ mode's bounds of the float64 oracle.  Glare draws its print seed but does not ac
 import numpy as np
 import pytest
 
-from conftest import CONFIG1, CONFIG2, CONFIG2_OVERRIDES, delta
+from conftest import PLAIN, GRAIN_HALATION, MIX3_OVERRIDES, delta
 from spektralab_engine_amd import colour, synth
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("cfg,w,h,seed", [("c1", 512, 384, 61), ("c2", 640, 427, 62)])
+@pytest.mark.parametrize("cfg,w,h,seed", [("plain", 512, 384, 61), ("grain_halation", 640, 427, 62)])
 def test_scan_film_matches_oracle(engine, oracle, cfg, w, h, seed):
     img = synth.make_frame(w, h, seed)
-    base = CONFIG1 if cfg == "c1" else CONFIG2
+    base = PLAIN if cfg == "plain" else GRAIN_HALATION
     with engine.open(img, dict(base, scan_film=True)) as s:
-        if cfg == "c2":
-            s.set_overrides(CONFIG2_OVERRIDES)
+        if cfg == "grain_halation":
+            s.set_overrides(MIX3_OVERRIDES)
         out, _ = s.render("full")
         plan = s.last_plan()
         neg = s.negative("full")
     assert int(oracle.field(plan, "printed")[0]) == 0
     neg_o = oracle.develop(plan, img)
-    if cfg == "c1":
+    if cfg == "plain":
         assert np.array_equal(neg, neg_o)
     out_o = oracle.print_(plan, neg)
     d = np.abs(out.astype(np.int32) - out_o.astype(np.int32))
@@ -37,7 +37,7 @@ def test_scan_film_matches_oracle(engine, oracle, cfg, w, h, seed):
 def test_scan_film_negative_is_the_printed_one(engine):
     """scan_film is a print-layer choice: the cached negative stays, the picture changes."""
     img = synth.make_frame(300, 200, 63)
-    with engine.open(img, CONFIG1) as s:
+    with engine.open(img, PLAIN) as s:
         printed, _ = s.render("full")
         neg = s.negative("full")
         r = s.set_params({"scan_film": True})
@@ -51,10 +51,10 @@ def test_scan_film_glare_draws_but_does_not_act(engine, oracle):
     """Under scan_film the print seed is still drawn when glare could draw (API_SPEC 2.8), and the
     picture is the glare-free scan."""
     img = synth.make_frame(256, 160, 64)
-    with engine.open(img, dict(CONFIG1, scan_film=True, glare_active=True)) as s:
+    with engine.open(img, dict(PLAIN, scan_film=True, glare_active=True)) as s:
         a, _ = s.render("full")
         plan = s.last_plan()
-    with engine.open(img, dict(CONFIG1, scan_film=True, glare_active=False)) as s:
+    with engine.open(img, dict(PLAIN, scan_film=True, glare_active=False)) as s:
         b, _ = s.render("full")
     assert int(oracle.field(plan, "glare_active")[0]) == 0
     assert np.array_equal(a, b)
@@ -62,7 +62,7 @@ def test_scan_film_glare_draws_but_does_not_act(engine, oracle):
 
 def test_scan_film_fp16_against_float64(engine, oracle):
     img = synth.make_frame(512, 384, 65)
-    with engine.open(img, dict(CONFIG1, scan_film=True)) as s:
+    with engine.open(img, dict(PLAIN, scan_film=True)) as s:
         o32, _ = s.render("full")
         plan = s.last_plan()
         neg = s.negative("full")

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import spektralab_engine_amd as spk
-from conftest import CONFIG1, delta
+from conftest import PLAIN, delta
 from spektralab_engine_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -55,7 +55,7 @@ def test_seeds_without_grain_or_glare(engine, oracle):
 def test_pure_function_without_grain_and_glare(engine):
     """Grain and glare off: render and reprint give byte-identical pictures, and so do two sessions."""
     img = synth.make_frame(240, 160, 23)
-    with engine.open(img, CONFIG1) as a, engine.open(img, CONFIG1) as b:
+    with engine.open(img, PLAIN) as a, engine.open(img, PLAIN) as b:
         pa, _ = a.render("full")
         pr, _ = a.render("full", reprint=True)
         pb, _ = b.render("full")
@@ -64,7 +64,7 @@ def test_pure_function_without_grain_and_glare(engine):
 
 def test_progress_record(engine):
     img = synth.make_frame(200, 140, 24)
-    with engine.open(img, CONFIG1) as s:
+    with engine.open(img, PLAIN) as s:
         p0 = s.progress()
         assert p0["progress_id"] == "" and p0["effect_versions"] == "" and p0["done"] is False
         _, f = s.render("full")
@@ -86,7 +86,7 @@ def test_cancel_before_a_render_is_lost(engine):
     """A cancel marks the current render; the next render resets the record, so a cancel that
     arrived before it is lost and the render completes.  A cancel naming another id does nothing."""
     img = synth.make_frame(200, 140, 25)
-    with engine.open(img, CONFIG1) as s:
+    with engine.open(img, PLAIN) as s:
         _, f = s.render("full")
         s.cancel("p-not-this-one")
         assert s.progress()["cancelled"] is False
@@ -101,7 +101,7 @@ def test_cancel_from_another_thread(engine):
     """spk_cancel is callable while a render holds the session lock: the render either completes or
     returns SPK_ERR_CANCELLED with nothing cached of its own (API_SPEC.md 2.9)."""
     img = synth.make_frame(1600, 1200, 26)
-    with engine.open(img, CONFIG1) as s:
+    with engine.open(img, PLAIN) as s:
         result = {}
 
         def work():
@@ -129,7 +129,7 @@ def test_layers_and_invalidation(engine):
     """A shoot-layer change drops every cached negative (reprint develops again); a print-layer
     change keeps them (reprint reuses); {} changes nothing (API_SPEC.md 2.3)."""
     img = synth.make_frame(240, 160, 27)
-    with engine.open(img, CONFIG1) as s:
+    with engine.open(img, PLAIN) as s:
         s.render("full")
         r = s.set_params({"print_exposure": 1.2})
         assert r["invalidated"] == "print"

@@ -11,7 +11,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import CONFIG1, delta
+from conftest import PLAIN, delta
 from spektralab_engine_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -39,7 +39,7 @@ def test_downscale_bit_exact(engine, oracle, shape, L):
 
 def test_live_and_preview_tiers(engine, oracle):
     img = synth.make_frame(2400, 1600, 7)
-    with engine.open(img, CONFIG1) as s:
+    with engine.open(img, PLAIN) as s:
         out, flags = s.render("live")
         assert (flags["width"], flags["height"]) == downscale_size(2400, 1600, 1600)
         plan = s.last_plan()

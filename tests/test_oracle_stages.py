@@ -25,7 +25,7 @@ import numpy as np
 import pytest
 
 import spektralab_engine_amd as spk
-from conftest import CONFIG1, delta
+from conftest import PLAIN, delta
 from test_setup import cam16_consts, cam16_forward, cam16_inverse
 
 
@@ -149,7 +149,7 @@ def test_boost_three_stops_takes_ten_to_eighty(oracle):
 
 # ---- paper curves -------------------------------------------------------------------------------
 def test_paper_lookup_nodes_and_plateaus(resources, oracle):
-    plan = spk.host_plan(resources, CONFIG1, 64, 64)
+    plan = spk.host_plan(resources, PLAIN, 64, 64)
     K = int(oracle.field(plan, "paper_k")[0])
     p = oracle.field(plan, "paper", 5 * K).astype(np.float32)
     x, y = p[:K], p[2 * K:].reshape(K, 3)
@@ -166,7 +166,7 @@ def test_paper_lookup_nodes_and_plateaus(resources, oracle):
 # ---- CAM16-UCS ----------------------------------------------------------------------------------
 @pytest.fixture(scope="module")
 def plan1(resources):
-    return spk.host_plan(resources, CONFIG1, 64, 64)
+    return spk.host_plan(resources, PLAIN, 64, 64)
 
 
 def test_cam16_forward_matches_the_published_equations(oracle, plan1):

@@ -12,12 +12,12 @@ import pytest
 
 import spektralab_engine_amd as spk
 from spektralab_engine_amd import synth
-from conftest import CONFIG1, CONFIG2, CONFIG2_OVERRIDES, delta
+from conftest import PLAIN, GRAIN_HALATION, MIX3_OVERRIDES, delta
 
 
 @pytest.fixture(scope="module")
 def plan1(resources):
-    return spk.host_plan(resources, CONFIG1, 512, 512)
+    return spk.host_plan(resources, PLAIN, 512, 512)
 
 
 def test_cam16_prophoto_constants(oracle, plan1):
@@ -118,7 +118,7 @@ def test_scan_normalised_to_white(oracle, plan1):
 
 
 def test_grain_constants(resources, oracle):
-    plan = spk.host_plan(resources, CONFIG2, 600, 400, film_seed=7)
+    plan = spk.host_plan(resources, GRAIN_HALATION, 600, 400, film_seed=7)
     assert oracle.field(plan, "grain_active")[0] == 1 and oracle.field(plan, "grain_seed")[0] == 7
     dmin_l = oracle.field(plan, "dmin_l", 9).reshape(3, 3)
     np.testing.assert_allclose(dmin_l.sum(axis=0), 0.03, rtol=1e-6)   # the fog split by layer fractions
@@ -130,7 +130,7 @@ def test_grain_constants(resources, oracle):
 
 
 def test_halation_mixture_override(resources, oracle):
-    plan = spk.host_plan(resources, CONFIG2, 600, 400, overrides=CONFIG2_OVERRIDES)
+    plan = spk.host_plan(resources, GRAIN_HALATION, 600, 400, overrides=MIX3_OVERRIDES)
     assert oracle.field(plan, "scatter_active")[0] == 0
     assert oracle.field(plan, "bounce_active")[0] == 1 and oracle.field(plan, "n_bounce")[0] == 3
     comps = oracle.field(plan, "bounce", 3 * 6).reshape(3, 2, 3)
@@ -217,8 +217,8 @@ def test_plan_bytes_unchanged(resources):
 
 def test_plan_is_a_function_of_its_inputs(resources):
     """Two plans of the same inputs are byte-identical (the unused table tails are zero)."""
-    a = bytes(spk.host_plan(resources, CONFIG1, 96, 64))
-    b = bytes(spk.host_plan(resources, CONFIG1, 96, 64))
+    a = bytes(spk.host_plan(resources, PLAIN, 96, 64))
+    b = bytes(spk.host_plan(resources, PLAIN, 96, 64))
     assert a == b
 
 
@@ -271,7 +271,7 @@ def test_output_transform_options_and_refusals(resources):
 def test_plan_tables_filled_and_tails_zero(resources, oracle):
     """Every table of a grain + halation plan holds its data (not all zero) and nothing past it: the
     zeroing of unused tails must not reach a table another step filled."""
-    plan = spk.host_plan(resources, CONFIG2, 120, 80, 0x4ABEA221, CONFIG2_OVERRIDES)
+    plan = spk.host_plan(resources, GRAIN_HALATION, 120, 80, 0x4ABEA221, MIX3_OVERRIDES)
     L, K = int(oracle.field(plan, "L")[0]), int(oracle.field(plan, "K")[0])
     gk, pk = int(oracle.field(plan, "grain_k")[0]), int(oracle.field(plan, "paper_k")[0])
     assert gk == K > 0 and pk > 0

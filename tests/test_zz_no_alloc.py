@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import spektralab_engine_amd as spk
-from conftest import CONFIG2, CONFIG2_OVERRIDES, delta
+from conftest import GRAIN_HALATION, MIX3_OVERRIDES, delta
 from spektralab_engine_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -52,8 +52,8 @@ def _render(engine, frame, params, overrides, static, reprint_after=False):
 
 
 CASES = [
-    ("config2_600x400", 600, 400, CONFIG2, CONFIG2_OVERRIDES),
-    ("config2_smaller_320x200", 320, 200, CONFIG2, CONFIG2_OVERRIDES),
+    ("grain_halation_600x400", 600, 400, GRAIN_HALATION, MIX3_OVERRIDES),
+    ("grain_halation_320x200", 320, 200, GRAIN_HALATION, MIX3_OVERRIDES),
 ]
 
 
@@ -80,11 +80,11 @@ def test_static_reprint_with_grain_and_glare(engine):
 
 def test_static_render_refuses_a_frame_beyond_the_region(engine):
     import torch
-    w, h = 3200, 2400  # the chain of configuration 2 needs about 1 GB of scratch at this size
-    params = dict(CONFIG2, film_format_mm=135.0)  # keeps the pitch above the dye-cloud limit
+    w, h = 3200, 2400  # the grain + 3-Gaussian halation chain needs about 1 GB of scratch at this size
+    params = dict(GRAIN_HALATION, film_format_mm=135.0)  # keeps the pitch above the dye-cloud limit
     frame = np.full((h, w, 3), 0.18, np.float32)
     with engine.open(frame, params) as s:
-        s.set_overrides(CONFIG2_OVERRIDES)
+        s.set_overrides(MIX3_OVERRIDES)
         out = torch.zeros((h, w, 4), dtype=torch.int16, device="cuda")
         with pytest.raises(spk.SpkError) as e:
             s.render_device_static(out.data_ptr(), w)
